@@ -1,0 +1,279 @@
+"""ctypes binding of libgm.so (include/gm_capi.h).
+
+Mirrors Goma's entry points for this path: ``matrix_fill_full``
+(src/mm_fill.c:158) and ``solve_nonlinear_problem`` (src/mm_sol_nonlinear.c:596),
+on a context built from a :class:`goma_amd.problems.Problem`.
+
+There is no fallback: if libgm.so is missing or cannot be loaded, importing
+this module's ``lib()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgm.so")
+
+GM_OK = 0
+GM_ERR_ELEM = -1
+GM_ERR_ARG = -2
+GM_ERR_HIP = -3
+GM_ERR_PIVOT = -4
+GM_ERR_NAN = -5
+GM_PREC_FP64 = 0
+GM_PREC_MIXED = 1
+GM_PC_NONE = 0
+GM_PC_ILU0 = 1
+K_FILL, K_SPMV, K_ILU_APPLY, K_ILU_FACT, K_ORTHO = range(5)
+KERNEL_NAMES = ["fill", "spmv", "ilu_apply", "ilu_factor", "ortho"]
+
+
+class GmMesh(C.Structure):
+    _fields_ = [("dim", C.c_int), ("num_nodes", C.c_int), ("num_elems", C.c_int),
+                ("coor", C.c_void_p), ("connect", C.c_void_p)]
+
+
+class GmPhysics(C.Structure):
+    _fields_ = [("energy", C.c_int), ("visc_model", C.c_int), ("rho", C.c_double),
+                ("mu0", C.c_double), ("muinf", C.c_double), ("lam", C.c_double),
+                ("nexp", C.c_double), ("aexp", C.c_double), ("Cp", C.c_double),
+                ("kcond", C.c_double), ("hsrc", C.c_double), ("g", C.c_double * 3),
+                ("beta", C.c_double), ("Tref", C.c_double)]
+
+
+class GmBC(C.Structure):
+    _fields_ = [("num_dbc", C.c_int), ("node", C.c_void_p), ("var", C.c_void_p),
+                ("value", C.c_void_p), ("datum_elem", C.c_int), ("datum_value", C.c_double)]
+
+
+class GmSolverOpts(C.Structure):
+    _fields_ = [("max_newton", C.c_int), ("eps_resid", C.c_double), ("eps_corr", C.c_double),
+                ("damp", C.c_double), ("kspace", C.c_int), ("max_its", C.c_int),
+                ("lin_tol", C.c_double), ("precond", C.c_int), ("patch_bins", C.c_int),
+                ("precision", C.c_int), ("verbose", C.c_int)]
+
+
+MAX_REPORT = 64
+
+
+class GmNewtonReport(C.Structure):
+    _fields_ = [("newton_its", C.c_int), ("converged", C.c_int), ("lin_its_total", C.c_int),
+                ("lin_its", C.c_int * MAX_REPORT), ("resid_L2", C.c_double * MAX_REPORT),
+                ("resid_Loo", C.c_double * MAX_REPORT), ("unscaled_L2", C.c_double * MAX_REPORT),
+                ("lin_relres", C.c_double * MAX_REPORT), ("t_assemble", C.c_double),
+                ("t_scale", C.c_double), ("t_precond", C.c_double), ("t_solve", C.c_double),
+                ("t_update", C.c_double), ("t_total", C.c_double)]
+
+    def summary(self) -> dict:
+        n = min(self.newton_its, MAX_REPORT)
+        return dict(newton_its=self.newton_its, converged=bool(self.converged),
+                    lin_its_total=self.lin_its_total, lin_its=list(self.lin_its[:n]),
+                    resid_L2=list(self.resid_L2[:n]), resid_Loo=list(self.resid_Loo[:n]),
+                    unscaled_L2=list(self.unscaled_L2[:n]), t_assemble=self.t_assemble,
+                    t_scale=self.t_scale, t_precond=self.t_precond, t_solve=self.t_solve,
+                    t_update=self.t_update, t_total=self.t_total)
+
+
+# every exported symbol of include/gm_capi.h
+EXPORTS = [
+    "gm_ctx_create", "gm_ctx_destroy", "gm_last_error", "gm_num_unknowns", "gm_msr_size",
+    "gm_copy_msr_graph", "gm_copy_first_unknown", "gm_copy_dirichlet", "gm_num_elem_colours",
+    "gm_copy_elem_colours", "gm_copy_patch_permutation", "gm_set_physics",
+    "gm_matrix_fill_full_dev", "gm_matrix_fill_full", "gm_row_sum_scale_dev", "gm_msr_matvec_dev",
+    "gm_gmres_dev", "gm_ilu0_factor_dev", "gm_solve_nonlinear_dev", "gm_solve_nonlinear",
+    "gm_default_solver_opts", "gm_prof_enable", "gm_prof_get",
+]
+
+_lib = None
+
+
+def lib():
+    """Load libgm.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with goma_amd/build.py "
+                           "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp, ip = C.c_void_p, C.c_int
+    L.gm_last_error.restype = C.c_char_p
+    L.gm_ctx_create.argtypes = [C.POINTER(GmMesh), C.POINTER(GmPhysics), C.POINTER(GmBC),
+                                C.POINTER(vp)]
+    L.gm_ctx_destroy.argtypes = [vp]
+    for f in ("gm_num_unknowns", "gm_msr_size", "gm_num_elem_colours"):
+        getattr(L, f).argtypes = [vp]
+    for f in ("gm_copy_msr_graph", "gm_copy_first_unknown", "gm_copy_elem_colours"):
+        getattr(L, f).argtypes = [vp, vp]
+    L.gm_copy_dirichlet.argtypes = [vp, vp, vp]
+    L.gm_copy_patch_permutation.argtypes = [vp, ip, vp, vp]
+    L.gm_set_physics.argtypes = [vp, C.POINTER(GmPhysics)]
+    L.gm_matrix_fill_full_dev.argtypes = [vp, vp, vp, vp]
+    L.gm_matrix_fill_full.argtypes = [vp, vp, vp, vp]
+    L.gm_row_sum_scale_dev.argtypes = [vp, vp, vp, vp]
+    L.gm_msr_matvec_dev.argtypes = [vp, vp, vp, vp]
+    L.gm_gmres_dev.argtypes = [vp, vp, vp, vp, C.POINTER(GmSolverOpts), C.POINTER(ip),
+                               C.POINTER(C.c_double)]
+    L.gm_ilu0_factor_dev.argtypes = [vp, vp, ip, vp, vp, vp]
+    L.gm_solve_nonlinear_dev.argtypes = [vp, vp, C.POINTER(GmSolverOpts),
+                                         C.POINTER(GmNewtonReport)]
+    L.gm_solve_nonlinear.argtypes = [vp, vp, C.POINTER(GmSolverOpts), C.POINTER(GmNewtonReport)]
+    L.gm_default_solver_opts.argtypes = [C.POINTER(GmSolverOpts)]
+    L.gm_default_solver_opts.restype = None
+    L.gm_prof_enable.argtypes = [vp, ip]
+    L.gm_prof_get.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(C.c_double)]
+    _lib = L
+    return L
+
+
+class GomaError(RuntimeError):
+    pass
+
+
+def _check(rc: int, what: str) -> int:
+    if rc < 0:
+        msg = lib().gm_last_error().decode(errors="replace")
+        raise GomaError(f"{what} failed ({rc}): {msg}")
+    return rc
+
+
+def solver_opts(**kw) -> GmSolverOpts:
+    o = GmSolverOpts()
+    lib().gm_default_solver_opts(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+def physics_struct(ph) -> GmPhysics:
+    return GmPhysics(ph.energy, ph.visc_model, ph.rho, ph.mu0, ph.muinf, ph.lam, ph.nexp, ph.aexp,
+                     ph.Cp, ph.kcond, ph.hsrc, (C.c_double * 3)(*ph.g), ph.beta, ph.Tref)
+
+
+class Context:
+    """A gm_ctx for one problem.  Device-pointer methods take integer addresses
+    (e.g. ``tensor.data_ptr()`` of a torch CUDA tensor)."""
+
+    def __init__(self, prob):
+        L = lib()
+        self.prob = prob
+        m = prob.mesh
+        self._coor = np.ascontiguousarray(m.coor, dtype=np.float64)
+        self._conn = np.ascontiguousarray(m.connect, dtype=np.int32)
+        self._bn = np.ascontiguousarray(prob.dbc_node, dtype=np.int32)
+        self._bv = np.ascontiguousarray(prob.dbc_var, dtype=np.int32)
+        self._bx = np.ascontiguousarray(prob.dbc_value, dtype=np.float64)
+        gm = GmMesh(m.dim, m.num_nodes, m.num_elems, self._coor.ctypes.data, self._conn.ctypes.data)
+        ph = physics_struct(prob.phys)
+        bc = GmBC(len(self._bn), self._bn.ctypes.data, self._bv.ctypes.data, self._bx.ctypes.data,
+                  prob.datum_elem, prob.datum_value)
+        h = C.c_void_p()
+        _check(L.gm_ctx_create(C.byref(gm), C.byref(ph), C.byref(bc), C.byref(h)), "gm_ctx_create")
+        self.h = h
+        self.N = L.gm_num_unknowns(h)
+        self.nval = L.gm_msr_size(h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- host-side views of the set-up
+    def bindx(self) -> np.ndarray:
+        out = np.zeros(self.nval, np.int32)
+        _check(lib().gm_copy_msr_graph(self.h, out.ctypes.data), "gm_copy_msr_graph")
+        return out
+
+    def first_unknown(self) -> np.ndarray:
+        out = np.zeros(self.prob.mesh.num_nodes + 1, np.int32)
+        _check(lib().gm_copy_first_unknown(self.h, out.ctypes.data), "gm_copy_first_unknown")
+        return out
+
+    def dirichlet(self):
+        mask = np.zeros(self.N, np.int8)
+        val = np.zeros(self.N)
+        _check(lib().gm_copy_dirichlet(self.h, mask.ctypes.data, val.ctypes.data), "gm_copy_dirichlet")
+        return mask, val
+
+    def elem_colours(self) -> np.ndarray:
+        out = np.zeros(self.prob.mesh.num_elems, np.int32)
+        _check(lib().gm_copy_elem_colours(self.h, out.ctypes.data), "gm_copy_elem_colours")
+        return out
+
+    def patch_permutation(self, bins: int = 0):
+        perm = np.zeros(self.N, np.int32)
+        cptr = np.zeros(4097, np.int32)
+        nc = _check(lib().gm_copy_patch_permutation(self.h, bins, perm.ctypes.data, cptr.ctypes.data),
+                    "gm_copy_patch_permutation")
+        return perm, cptr[: nc + 1].copy()
+
+    def set_physics(self, phys):
+        ph = physics_struct(phys)
+        _check(lib().gm_set_physics(self.h, C.byref(ph)), "gm_set_physics")
+
+    # ---- compute (device pointers)
+    def fill_dev(self, x_ptr: int, resid_ptr: int, val_ptr: int) -> int:
+        return lib().gm_matrix_fill_full_dev(self.h, x_ptr, resid_ptr, val_ptr)
+
+    def fill_host(self, x: np.ndarray, resid: np.ndarray, val: np.ndarray) -> int:
+        for a in (x, resid, val):
+            assert a.dtype == np.float64 and a.flags.c_contiguous
+        return lib().gm_matrix_fill_full(self.h, x.ctypes.data, resid.ctypes.data, val.ctypes.data)
+
+    def row_sum_scale_dev(self, val_ptr, b_ptr, scale_ptr):
+        _check(lib().gm_row_sum_scale_dev(self.h, val_ptr, b_ptr, scale_ptr), "gm_row_sum_scale_dev")
+
+    def matvec_dev(self, val_ptr, x_ptr, y_ptr):
+        _check(lib().gm_msr_matvec_dev(self.h, val_ptr, x_ptr, y_ptr), "gm_msr_matvec_dev")
+
+    def gmres_dev(self, val_ptr, b_ptr, x_ptr, **opts):
+        o = solver_opts(**opts)
+        its = C.c_int(0)
+        rr = C.c_double(0.0)
+        _check(lib().gm_gmres_dev(self.h, val_ptr, b_ptr, x_ptr, C.byref(o), C.byref(its),
+                                  C.byref(rr)), "gm_gmres_dev")
+        return its.value, rr.value
+
+    def ilu0_factor_dev(self, val_ptr, bins: int = 0):
+        nnz = self.nval - 1
+        rowptr = np.zeros(self.N + 1, np.int32)
+        col = np.zeros(nnz, np.int32)
+        lu = np.zeros(nnz)
+        _check(lib().gm_ilu0_factor_dev(self.h, val_ptr, bins, rowptr.ctypes.data, col.ctypes.data,
+                                        lu.ctypes.data), "gm_ilu0_factor_dev")
+        return rowptr, col, lu
+
+    def solve_dev(self, x_ptr: int, **opts):
+        o = solver_opts(**opts)
+        rep = GmNewtonReport()
+        rc = lib().gm_solve_nonlinear_dev(self.h, x_ptr, C.byref(o), C.byref(rep))
+        _check(rc, "gm_solve_nonlinear_dev")
+        return rc, rep
+
+    def solve_host(self, x: np.ndarray, **opts):
+        o = solver_opts(**opts)
+        rep = GmNewtonReport()
+        rc = lib().gm_solve_nonlinear(self.h, x.ctypes.data, C.byref(o), C.byref(rep))
+        _check(rc, "gm_solve_nonlinear")
+        return rc, rep
+
+    # ---- kernel timing
+    def prof_enable(self, on: bool = True):
+        _check(lib().gm_prof_enable(self.h, 1 if on else 0), "gm_prof_enable")
+
+    def prof_get(self, kid: int):
+        cnt = C.c_int(0)
+        ms = C.c_double(0.0)
+        _check(lib().gm_prof_get(self.h, kid, C.byref(cnt), C.byref(ms)), "gm_prof_get")
+        return cnt.value, ms.value

@@ -1,0 +1,413 @@
+// C ABI of libgm (include/gm_capi.h): context life cycle, the
+// matrix_fill_full and solve_nonlinear_problem replacements, kernel timing.
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "gm_internal.h"
+
+static thread_local char g_err[512] = "";
+
+void gm_set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+extern "C" const char *gm_last_error(void) { return g_err; }
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void gm_prof_begin(gm_ctx *c, int id) {
+  if (!c->prof) return;
+  hipEventRecord(c->ev_b[id], c->stream);
+}
+
+void gm_prof_end(gm_ctx *c, int id) {
+  if (!c->prof) return;
+  hipEventRecord(c->ev_e[id], c->stream);
+  hipEventSynchronize(c->ev_e[id]);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, c->ev_b[id], c->ev_e[id]);
+  c->prof_ms[id] += ms;
+  c->prof_count[id]++;
+}
+
+extern "C" void gm_default_solver_opts(gm_solver_opts *o) {
+  o->max_newton = 25;
+  o->eps_resid = 1e-10;
+  o->eps_corr = 1e10;
+  o->damp = 1.0;
+  o->kspace = 50;
+  o->max_its = 2000;
+  o->lin_tol = 1e-6;
+  o->precond = GM_PC_ILU0;
+  o->patch_bins = 0;
+  o->precision = GM_PREC_FP64;
+  o->verbose = 0;
+}
+
+static void free_ctx(gm_ctx *c) {
+  if (!c) return;
+  gm_free_linalg(c);
+  void *ptrs[] = {c->d_coor, c->d_conn,   c->d_fu,    c->d_centroid, c->d_dbc,   c->d_dbc_val,
+                  c->d_bindx, c->d_ecol_elems, c->d_e2nz, c->d_flag, c->d_tables, c->d_val,
+                  c->d_resid, c->d_dx,    c->d_scale, c->d_val32,    c->d_red,   c->d_xwork};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  if (c->h_red) hipHostFree(c->h_red);
+  for (int i = 0; i < GM_K_COUNT; i++) {
+    if (c->ev_b[i]) hipEventDestroy(c->ev_b[i]);
+    if (c->ev_e[i]) hipEventDestroy(c->ev_e[i]);
+  }
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+template <typename T>
+static int upload(T **d, const T *h, size_t n) {
+  GM_HIP(hipMalloc(d, std::max<size_t>(n, 1) * sizeof(T)));
+  if (n) GM_HIP(hipMemcpy(*d, h, n * sizeof(T), hipMemcpyHostToDevice));
+  return GM_OK;
+}
+
+static int create_impl(const gm_mesh *m, const gm_physics *ph, const gm_bc *bc, gm_ctx *c) {
+  if (!m || !ph || !bc || (m->dim != 2 && m->dim != 3) || m->num_nodes <= 0 || m->num_elems <= 0 ||
+      !m->coor || !m->connect) {
+    gm_set_error("invalid mesh description");
+    return GM_ERR_ARG;
+  }
+  c->dim = m->dim;
+  c->nve = m->dim == 2 ? 9 : 27;
+  c->np = m->dim + 1;
+  c->energy = ph->energy ? 1 : 0;
+  c->nv = c->dim + c->energy;
+  c->num_nodes = m->num_nodes;
+  c->num_elems = m->num_elems;
+  c->phys = *ph;
+  c->coor.assign(m->coor, m->coor + (size_t)m->dim * m->num_nodes);
+  c->conn.assign(m->connect, m->connect + (size_t)m->num_elems * c->nve);
+  for (int v : c->conn)
+    if (v < 0 || v >= m->num_nodes) {
+      gm_set_error("connectivity references node %d outside [0, %d)", v, m->num_nodes);
+      return GM_ERR_ARG;
+    }
+  int rc = gm_build_host(c);
+  if (rc) return rc;
+  // Dirichlet map per unknown; pressure datum on the datum element's first
+  // P dof (find_and_set_Dirichlet, src/mm_bc.c:132, 386-457).
+  c->dbc.assign(c->N, 0);
+  c->dbc_val.assign(c->N, 0.0);
+  for (int i = 0; i < bc->num_dbc; i++) {
+    const int n = bc->node[i], v = bc->var[i];
+    if (n < 0 || n >= c->num_nodes) {
+      gm_set_error("Dirichlet condition %d on node %d outside the mesh", i, n);
+      return GM_ERR_ARG;
+    }
+    int off = -1;
+    if (v >= GM_VAR_U1 && v < GM_VAR_U1 + c->dim) off = v - GM_VAR_U1;
+    else if (v == GM_VAR_T && c->energy) off = c->dim;
+    if (off < 0) {
+      gm_set_error("Dirichlet condition %d on variable %d that is not solved for", i, v);
+      return GM_ERR_ARG;
+    }
+    const int u = c->first_unknown[n] + off;
+    c->dbc[u] = 1;
+    c->dbc_val[u] = bc->value[i];
+  }
+  if (bc->datum_elem >= 0) {
+    if (bc->datum_elem >= c->num_elems) {
+      gm_set_error("pressure datum element %d outside the mesh", bc->datum_elem);
+      return GM_ERR_ARG;
+    }
+    const int u = c->first_unknown[c->elem_centroid[bc->datum_elem]] + c->dim + c->energy;
+    c->dbc[u] = 1;
+    c->dbc_val[u] = bc->datum_value;
+  }
+
+  GM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (int i = 0; i < GM_K_COUNT; i++) {
+    GM_HIP(hipEventCreate(&c->ev_b[i]));
+    GM_HIP(hipEventCreate(&c->ev_e[i]));
+  }
+  if ((rc = upload(&c->d_coor, c->coor.data(), c->coor.size()))) return rc;
+  if ((rc = upload(&c->d_conn, c->conn.data(), c->conn.size()))) return rc;
+  if ((rc = upload(&c->d_fu, c->first_unknown.data(), c->first_unknown.size()))) return rc;
+  if ((rc = upload(&c->d_centroid, c->elem_centroid.data(), c->elem_centroid.size()))) return rc;
+  if ((rc = upload(&c->d_dbc, c->dbc.data(), c->dbc.size()))) return rc;
+  if ((rc = upload(&c->d_dbc_val, c->dbc_val.data(), c->dbc_val.size()))) return rc;
+  if ((rc = upload(&c->d_bindx, c->bindx.data(), c->bindx.size()))) return rc;
+  if ((rc = upload(&c->d_ecol_elems, c->ecol_elems.data(), c->ecol_elems.size()))) return rc;
+  GM_HIP(hipMalloc(&c->d_flag, 8 * sizeof(int)));
+  GM_HIP(hipMemset(c->d_flag, 0, 8 * sizeof(int)));
+  GM_HIP(hipMalloc(&c->d_red, 2 * 1024 * sizeof(double)));
+  GM_HIP(hipHostMalloc(&c->h_red, 128 * 1024 * sizeof(double), hipHostMallocDefault));
+  return gm_fill_setup_device(c);
+}
+
+extern "C" int gm_ctx_create(const gm_mesh *m, const gm_physics *ph, const gm_bc *bc, gm_ctx **out) {
+  if (!out) {
+    gm_set_error("null output pointer");
+    return GM_ERR_ARG;
+  }
+  *out = nullptr;
+  gm_ctx *c = new gm_ctx();
+  int rc = create_impl(m, ph, bc, c);
+  if (rc) {
+    free_ctx(c);
+    return rc;
+  }
+  *out = c;
+  return GM_OK;
+}
+
+extern "C" void gm_ctx_destroy(gm_ctx *c) { free_ctx(c); }
+
+extern "C" int gm_num_unknowns(const gm_ctx *c) { return c ? c->N : GM_ERR_ARG; }
+extern "C" int gm_msr_size(const gm_ctx *c) { return c ? (int)c->nval : GM_ERR_ARG; }
+
+extern "C" int gm_copy_msr_graph(const gm_ctx *c, int *bindx) {
+  if (!c || !bindx) return GM_ERR_ARG;
+  std::memcpy(bindx, c->bindx.data(), c->bindx.size() * sizeof(int));
+  return GM_OK;
+}
+
+extern "C" int gm_copy_first_unknown(const gm_ctx *c, int *fu) {
+  if (!c || !fu) return GM_ERR_ARG;
+  std::memcpy(fu, c->first_unknown.data(), c->first_unknown.size() * sizeof(int));
+  return GM_OK;
+}
+
+extern "C" int gm_copy_dirichlet(const gm_ctx *c, signed char *mask, double *value) {
+  if (!c || !mask || !value) return GM_ERR_ARG;
+  std::memcpy(mask, c->dbc.data(), c->N);
+  std::memcpy(value, c->dbc_val.data(), c->N * sizeof(double));
+  return GM_OK;
+}
+
+extern "C" int gm_num_elem_colours(const gm_ctx *c) { return c ? c->n_ecol : GM_ERR_ARG; }
+
+extern "C" int gm_copy_elem_colours(const gm_ctx *c, int *colour) {
+  if (!c || !colour) return GM_ERR_ARG;
+  std::memcpy(colour, c->elem_colour.data(), c->num_elems * sizeof(int));
+  return GM_OK;
+}
+
+extern "C" int gm_copy_patch_permutation(gm_ctx *c, int bins, int *perm, int *colour_ptr) {
+  if (!c || !perm || !colour_ptr) return GM_ERR_ARG;
+  std::vector<int> p, cp, pr;
+  int ncol = gm_patch_order(c, bins > 0 ? bins : gm_default_bins(c), p, cp, pr);
+  if (ncol < 0) return ncol;
+  std::memcpy(perm, p.data(), c->N * sizeof(int));
+  for (int k = 0; k <= ncol; k++) colour_ptr[k] = pr[cp[k]];  // unknown offsets
+  return ncol;
+}
+
+extern "C" int gm_set_physics(gm_ctx *c, const gm_physics *ph) {
+  if (!c || !ph) return GM_ERR_ARG;
+  if ((ph->energy ? 1 : 0) != c->energy) {
+    gm_set_error("the energy equation cannot be switched on an existing context");
+    return GM_ERR_ARG;
+  }
+  c->phys = *ph;
+  return GM_OK;
+}
+
+extern "C" int gm_matrix_fill_full_dev(gm_ctx *c, const double *x, double *resid, double *val) {
+  if (!c || !x || !resid || !val) return GM_ERR_ARG;
+  return gm_fill_launch(c, x, resid, val, nullptr);
+}
+
+extern "C" int gm_matrix_fill_full(gm_ctx *c, const double *x, double *resid, double *val) {
+  if (!c || !x || !resid || !val) return GM_ERR_ARG;
+  double *dx = nullptr, *dr = nullptr, *dv = nullptr;
+  int rc = GM_OK;
+  auto cleanup = [&]() {
+    if (dx) hipFree(dx);
+    if (dr) hipFree(dr);
+    if (dv) hipFree(dv);
+  };
+  if (hipMalloc(&dx, c->N * sizeof(double)) != hipSuccess ||
+      hipMalloc(&dr, c->N * sizeof(double)) != hipSuccess ||
+      hipMalloc(&dv, c->nval * sizeof(double)) != hipSuccess) {
+    cleanup();
+    gm_set_error("device allocation failed");
+    return GM_ERR_HIP;
+  }
+  hipMemcpy(dx, x, c->N * sizeof(double), hipMemcpyHostToDevice);
+  hipMemcpy(dr, resid, c->N * sizeof(double), hipMemcpyHostToDevice);
+  hipMemcpy(dv, val, c->nval * sizeof(double), hipMemcpyHostToDevice);
+  rc = gm_fill_launch(c, dx, dr, dv, nullptr);
+  if (rc == GM_OK) {
+    hipMemcpy(resid, dr, c->N * sizeof(double), hipMemcpyDeviceToHost);
+    hipMemcpy(val, dv, c->nval * sizeof(double), hipMemcpyDeviceToHost);
+  }
+  cleanup();
+  return rc;
+}
+
+extern "C" int gm_row_sum_scale_dev(gm_ctx *c, double *val, double *b, double *scale) {
+  if (!c || !val || !b || !scale) return GM_ERR_ARG;
+  int rc = gm_scale_launch(c, val, nullptr, b, scale);
+  if (rc) return rc;
+  GM_HIP(hipStreamSynchronize(c->stream));
+  return GM_OK;
+}
+
+extern "C" int gm_msr_matvec_dev(gm_ctx *c, const double *val, const double *x, double *y) {
+  if (!c || !val || !x || !y) return GM_ERR_ARG;
+  int rc = gm_spmv_launch(c, val, x, y);
+  if (rc) return rc;
+  GM_HIP(hipStreamSynchronize(c->stream));
+  return GM_OK;
+}
+
+extern "C" int gm_gmres_dev(gm_ctx *c, const double *val, const double *b, double *x,
+                            const gm_solver_opts *o, int *its, double *relres) {
+  if (!c || !val || !b || !x || !o || !its || !relres) return GM_ERR_ARG;
+  if (o->precision != GM_PREC_FP64) {
+    gm_set_error("gm_gmres_dev takes an FP64 matrix");
+    return GM_ERR_ARG;
+  }
+  int rc = gm_gmres_run(c, val, nullptr, b, x, o, its, relres);
+  if (rc) return rc;
+  GM_HIP(hipStreamSynchronize(c->stream));
+  return GM_OK;
+}
+
+extern "C" int gm_ilu0_factor_dev(gm_ctx *c, const double *val, int bins, int *rowptr, int *col,
+                                  double *lu) {
+  if (!c || !val || !rowptr || !col || !lu) return GM_ERR_ARG;
+  return gm_ilu_factor_export(c, val, bins > 0 ? bins : gm_default_bins(c), rowptr, col, lu);
+}
+
+static int ensure_newton_buffers(gm_ctx *c, int prec) {
+  if (!c->d_resid) {
+    GM_HIP(hipMalloc(&c->d_resid, c->N * sizeof(double)));
+    GM_HIP(hipMalloc(&c->d_dx, c->N * sizeof(double)));
+    GM_HIP(hipMalloc(&c->d_scale, c->N * sizeof(double)));
+  }
+  if (prec == GM_PREC_MIXED) {
+    if (!c->d_val32) GM_HIP(hipMalloc(&c->d_val32, c->nval * sizeof(float)));
+  } else {
+    if (!c->d_val) GM_HIP(hipMalloc(&c->d_val, c->nval * sizeof(double)));
+  }
+  return GM_OK;
+}
+
+// solve_nonlinear_problem, steady, NLS_FULL_STEP (src/mm_sol_nonlinear.c:1108-2913)
+extern "C" int gm_solve_nonlinear_dev(gm_ctx *c, double *x, const gm_solver_opts *o,
+                                      gm_newton_report *rep) {
+  if (!c || !x || !o || !rep) return GM_ERR_ARG;
+  std::memset(rep, 0, sizeof(*rep));
+  const int prec = o->precision;
+  int rc = ensure_newton_buffers(c, prec);
+  if (rc) return rc;
+  const double t0 = now_s();
+  int converged = 0, inewton = 0;
+  while (!converged && inewton < o->max_newton) {
+    double ta = now_s();
+    if ((rc = gm_zero(c, c->d_resid, c->N))) return rc;
+    if ((rc = gm_zero(c, c->d_dx, c->N))) return rc;
+    if (prec == GM_PREC_MIXED) rc = gm_zero32(c, c->d_val32, c->nval);
+    else rc = gm_zero(c, c->d_val, c->nval);
+    if (rc) return rc;
+    rc = gm_fill_launch(c, x, c->d_resid, c->d_val, prec == GM_PREC_MIXED ? c->d_val32 : nullptr);
+    if (rc) return rc;
+    double un, unoo;
+    if ((rc = gm_norms(c, c->d_resid, c->N, &un, &unoo))) return rc;
+    double tb = now_s();
+    rep->t_assemble += tb - ta;
+    if ((rc = gm_scale_launch(c, c->d_val, prec == GM_PREC_MIXED ? c->d_val32 : nullptr, c->d_resid,
+                              c->d_scale)))
+      return rc;
+    double L2, Loo;
+    if ((rc = gm_norms(c, c->d_resid, c->N, &L2, &Loo))) return rc;
+    double tc = now_s();
+    rep->t_scale += tc - tb;
+    if (inewton < GM_MAX_REPORT) {
+      rep->resid_L2[inewton] = L2;
+      rep->resid_Loo[inewton] = Loo;
+      rep->unscaled_L2[inewton] = un;
+    }
+    if (inewton > 0 && L2 < o->eps_resid && Loo < o->eps_resid) converged = o->eps_corr > 1.0;
+    if (!std::isfinite(L2) || !std::isfinite(Loo)) {
+      gm_set_error("non-finite residual norm");
+      rep->newton_its = inewton;
+      rep->t_total = now_s() - t0;
+      return GM_ERR_NAN;
+    }
+    if (!converged) {
+      int its = 0;
+      double rr = 0.0;
+      if (o->precond != GM_PC_NONE) {
+        const int bins = o->patch_bins > 0 ? o->patch_bins : gm_default_bins(c);
+        if ((rc = gm_ilu_plan(c, bins))) return rc;
+      }
+      gm_solver_opts oo = *o;
+      // factorisation timed separately, then GMRES without re-factoring
+      if (o->precond != GM_PC_NONE) {
+        if ((rc = gm_ilu_factor(c, c->d_val, c->d_val32, prec))) return rc;
+      }
+      double td = now_s();
+      rep->t_precond += td - tc;
+      oo.precond = o->precond;
+      rc = gm_gmres_run_nofactor(c, c->d_val, c->d_val32, c->d_resid, c->d_dx, &oo, &its, &rr);
+      if (rc) return rc;
+      double te = now_s();
+      rep->t_solve += te - td;
+      if (inewton < GM_MAX_REPORT) {
+        rep->lin_its[inewton] = its;
+        rep->lin_relres[inewton] = rr;
+      }
+      rep->lin_its_total += its;
+      double cL2r, cLoor;
+      if ((rc = gm_corr_norms(c, c->d_dx, x, &cL2r, &cLoor))) return rc;
+      if ((rc = gm_axpy_update(c, x, c->d_dx, o->damp))) return rc;
+      GM_HIP(hipStreamSynchronize(c->stream));
+      rep->t_update += now_s() - te;
+      converged = (L2 < o->eps_resid) && (Loo < o->eps_resid) && (cL2r < o->eps_corr) &&
+                  (cLoor < o->eps_corr);
+    }
+    if (o->verbose)
+      fprintf(stderr, "[gm] newton %2d |R|2 %.3e |R|oo %.3e unscaled %.3e lin_its %d\n", inewton, L2,
+              Loo, un, inewton < GM_MAX_REPORT ? rep->lin_its[inewton] : -1);
+    inewton++;
+  }
+  rep->newton_its = inewton;
+  rep->converged = converged;
+  rep->t_total = now_s() - t0;
+  return inewton;
+}
+
+extern "C" int gm_solve_nonlinear(gm_ctx *c, double *x, const gm_solver_opts *o,
+                                  gm_newton_report *rep) {
+  if (!c || !x) return GM_ERR_ARG;
+  if (!c->d_xwork) GM_HIP(hipMalloc(&c->d_xwork, c->N * sizeof(double)));
+  GM_HIP(hipMemcpy(c->d_xwork, x, c->N * sizeof(double), hipMemcpyHostToDevice));
+  int rc = gm_solve_nonlinear_dev(c, c->d_xwork, o, rep);
+  GM_HIP(hipMemcpy(x, c->d_xwork, c->N * sizeof(double), hipMemcpyDeviceToHost));
+  return rc;
+}
+
+extern "C" int gm_prof_enable(gm_ctx *c, int on) {
+  if (!c) return GM_ERR_ARG;
+  c->prof = on != 0;
+  for (int i = 0; i < GM_K_COUNT; i++) {
+    c->prof_count[i] = 0;
+    c->prof_ms[i] = 0.0;
+  }
+  return GM_OK;
+}
+
+extern "C" int gm_prof_get(gm_ctx *c, int id, int *count, double *ms) {
+  if (!c || id < 0 || id >= GM_K_COUNT || !count || !ms) return GM_ERR_ARG;
+  *count = c->prof_count[id];
+  *ms = c->prof_ms[id];
+  return GM_OK;
+}

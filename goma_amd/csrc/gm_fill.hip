@@ -1,0 +1,638 @@
+// Element assembly (matrix_fill_full) for QUAD9 / HEX27 Q2-P1 Navier-Stokes
+// with optional energy, as HIP kernels for gfx950.
+//
+// One workgroup assembles one element.  Elements are launched one colour at a
+// time (no two elements of a colour share a node), so every element owns the
+// nonzeros it scatters into during its launch and the scatter is a plain
+// read-modify-write: deterministic, no atomics.
+//
+// Per element:
+//   1. nodal coordinates / unknowns -> LDS
+//   2. per quadrature point: map Jacobian, inverse, dA = detJ*w  (beer_belly)
+//   3. physical basis gradients grad_phi[q][k][p]                (load_bf_grad)
+//   4. per quadrature point: v, grad v, T, grad T, P, mu(gammadot), body force
+//   5. residual rows, then the Jacobian: thread t owns node pair (i,j) and
+//      accumulates every (row variable, column variable) block of that pair;
+//      pressure couplings are separate task lists.
+//   6. Dirichlet rows: J row -> unit diagonal, R -> x - V (per element, as
+//      put_dirichlet_in_matrix does, src/bc_dirich.c:44-153).
+//   7. scatter through the element-to-nonzero map into the MSR arrays.
+//
+// Weak forms (Goma's, restated):
+//   momentum  (src/mm_fill_momentum.c:474-672, J_m_v 1564-1735, J_m_T 746-915,
+//             J_m_P 2052-2117; fluid_stress 2834-3737):
+//     R_a,i = int[ -rho phi_i (v.grad v_a) - grad phi_i . Pi[a][.] + phi_i f_a ] dA
+//     Pi = -P I + mu (grad v + grad v^T),  f_a = -rho g_a beta (T - Tref)
+//   continuity (src/mm_fill_continuity.c:435-700):  R_k = int psi_k div v dA
+//   energy   (src/mm_fill_energy.c:280-700, heat_flux 1629):
+//     R_i = int[ -rho Cp phi_i (v.grad T) - k grad phi_i . grad T + phi_i h ] dA
+#include <cmath>
+#include <vector>
+
+#include "gm_internal.h"
+
+template <int DIM>
+struct El {
+  static constexpr int NVE = DIM == 2 ? 9 : 27;
+  static constexpr int NQ = DIM == 2 ? 9 : 27;
+  static constexpr int NP = DIM + 1;
+};
+
+struct FillArgs {
+  const double *coor;       // [dim][num_nodes]
+  const int *conn;          // [num_elems][nve]
+  const int *fu;            // first unknown per node
+  const int *centroid;      // centroid node per element
+  const signed char *dbc;   // [N]
+  const double *dbc_val;    // [N]
+  const double *tables;     // phi, dphidxi, psi, wt
+  const int *e2nz;          // [num_elems][E]
+  const double *x;
+  double *resid;
+  void *val;
+  int *flag;
+  int num_nodes;
+  long long E;
+  long long off_mp, off_cv, off_cp;
+  double rho, mu0, muinf, lam, nexp, aexp, Cp, kcond, hsrc, g[3], beta, Tref;
+};
+
+// Reference tables: phi[NQ][NVE], dphidxi[NQ][NVE][DIM], psi[NQ][NP], wt[NQ].
+template <int DIM>
+struct Tab {
+  static constexpr int NVE = El<DIM>::NVE, NQ = El<DIM>::NQ, NP = El<DIM>::NP;
+  static constexpr int PHI = 0;
+  static constexpr int DPHI = PHI + NQ * NVE;
+  static constexpr int PSI = DPHI + NQ * NVE * DIM;
+  static constexpr int WT = PSI + NQ * NP;
+  static constexpr int SIZE = WT + NQ;
+};
+
+template <int DIM, int EN, int VISC, typename JT, int NT>
+__global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__ elems) {
+  constexpr int NVE = El<DIM>::NVE, NQ = El<DIM>::NQ, NP = El<DIM>::NP;
+  constexpr int NV = DIM + EN;
+  constexpr int NVE2 = NVE * NVE;
+  using T = Tab<DIM>;
+  const int e = elems[blockIdx.x];
+  const int tid = threadIdx.x;
+  const double *__restrict__ tb = A.tables;
+
+  __shared__ double s_x[DIM][NVE];
+  __shared__ double s_u[NV][NVE];
+  __shared__ double s_p[NP];
+  __shared__ int s_gun[NV][NVE];
+  __shared__ int s_gp[NP];
+  __shared__ double s_B[NQ][DIM][DIM];
+  __shared__ double s_dA[NQ];
+  __shared__ double s_g[NQ][NVE][DIM];
+  __shared__ double s_v[NQ][DIM];
+  __shared__ double s_gv[NQ][DIM][DIM];  // gv[p][a] = d v_a / d x_p
+  __shared__ double s_gT[NQ][DIM];
+  __shared__ double s_P[NQ];
+  __shared__ double s_mu[NQ];
+  __shared__ double s_dmu[NQ];           // (dmu/dgammadot) / gammadot, 0 if gammadot == 0
+  __shared__ double s_f[NQ][DIM];
+  __shared__ double s_div[NQ];
+
+  // 1. gather
+  if (tid < NVE) {
+    const int n = A.conn[(size_t)e * NVE + tid];
+    const int f = A.fu[n];
+#pragma unroll
+    for (int k = 0; k < DIM; k++) s_x[k][tid] = A.coor[(size_t)k * A.num_nodes + n];
+#pragma unroll
+    for (int a = 0; a < NV; a++) {
+      s_gun[a][tid] = f + a;
+      s_u[a][tid] = A.x[f + a];
+    }
+  } else if (tid >= 64 && tid < 64 + NP) {
+    const int k = tid - 64;
+    const int g = A.fu[A.centroid[e]] + DIM + EN + k;
+    s_gp[k] = g;
+    s_p[k] = A.x[g];
+  }
+  __syncthreads();
+
+  // 2. map per quadrature point
+  if (tid < NQ) {
+    const int q = tid;
+    double J[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; i++)
+#pragma unroll
+      for (int j = 0; j < DIM; j++) {
+        double s = 0.0;
+        for (int k = 0; k < NVE; k++) s += s_x[j][k] * tb[T::DPHI + (q * NVE + k) * DIM + i];
+        J[i][j] = s;
+      }
+    double det;
+    if constexpr (DIM == 2) {
+      det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      s_B[q][0][0] = J[1][1] / det;
+      s_B[q][0][1] = -J[0][1] / det;
+      s_B[q][1][0] = -J[1][0] / det;
+      s_B[q][1][1] = J[0][0] / det;
+    } else {
+      det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+            J[0][1] * (J[1][0] * J[2][2] - J[2][0] * J[1][2]) +
+            J[0][2] * (J[1][0] * J[2][1] - J[2][0] * J[1][1]);
+      s_B[q][0][0] = (J[1][1] * J[2][2] - J[2][1] * J[1][2]) / det;
+      s_B[q][0][1] = -(J[0][1] * J[2][2] - J[2][1] * J[0][2]) / det;
+      s_B[q][0][2] = (J[0][1] * J[1][2] - J[1][1] * J[0][2]) / det;
+      s_B[q][1][0] = -(J[1][0] * J[2][2] - J[2][0] * J[1][2]) / det;
+      s_B[q][1][1] = (J[0][0] * J[2][2] - J[2][0] * J[0][2]) / det;
+      s_B[q][1][2] = -(J[0][0] * J[1][2] - J[1][0] * J[0][2]) / det;
+      s_B[q][2][0] = (J[1][0] * J[2][1] - J[2][0] * J[1][1]) / det;
+      s_B[q][2][1] = -(J[0][0] * J[2][1] - J[2][0] * J[0][1]) / det;
+      s_B[q][2][2] = (J[0][0] * J[1][1] - J[1][0] * J[0][1]) / det;
+    }
+    if (!(det > 0.0)) atomicOr(&A.flag[0], 1);
+    s_dA[q] = det * tb[T::WT + q];
+  }
+  __syncthreads();
+
+  // 3. physical gradients
+  for (int idx = tid; idx < NQ * NVE; idx += NT) {
+    const int q = idx / NVE, k = idx - q * NVE;
+    double dxi[DIM];
+#pragma unroll
+    for (int r = 0; r < DIM; r++) dxi[r] = tb[T::DPHI + (q * NVE + k) * DIM + r];
+#pragma unroll
+    for (int p = 0; p < DIM; p++) {
+      double s = 0.0;
+#pragma unroll
+      for (int r = 0; r < DIM; r++) s += s_B[q][p][r] * dxi[r];
+      s_g[q][k][p] = s;
+    }
+  }
+  __syncthreads();
+
+  // 4. fields at quadrature points
+  if (tid < NQ) {
+    const int q = tid;
+    double v[DIM], gv[DIM][DIM], Tq = 0.0, gT[DIM];
+#pragma unroll
+    for (int a = 0; a < DIM; a++) {
+      v[a] = 0.0;
+      gT[a] = 0.0;
+#pragma unroll
+      for (int p = 0; p < DIM; p++) gv[p][a] = 0.0;
+    }
+    for (int k = 0; k < NVE; k++) {
+      const double ph = tb[T::PHI + q * NVE + k];
+#pragma unroll
+      for (int a = 0; a < DIM; a++) {
+        const double ua = s_u[a][k];
+        v[a] += ua * ph;
+#pragma unroll
+        for (int p = 0; p < DIM; p++) gv[p][a] += ua * s_g[q][k][p];
+      }
+      if constexpr (EN) {
+        const double tk = s_u[DIM][k];
+        Tq += tk * ph;
+#pragma unroll
+        for (int p = 0; p < DIM; p++) gT[p] += tk * s_g[q][k][p];
+      }
+    }
+    double P = 0.0;
+#pragma unroll
+    for (int k = 0; k < NP; k++) P += s_p[k] * tb[T::PSI + q * NP + k];
+    double mu = A.mu0, dmu = 0.0;
+    if constexpr (VISC == 1) {
+      // calc_shearrate + carreau_viscosity (src/mm_fill_util.c:4564,
+      // src/mm_viscosity.c:1065-1116)
+      double gd = 0.0;
+#pragma unroll
+      for (int a = 0; a < DIM; a++)
+#pragma unroll
+        for (int b = 0; b < DIM; b++) {
+          const double gab = gv[a][b] + gv[b][a];
+          gd += gab * gab;
+        }
+      gd = sqrt(0.5 * fabs(gd));
+      const bool nz = fabs(gd) > 1.0e-298;
+      const double val2 = nz ? pow(A.lam * gd, A.aexp) : 0.0;
+      mu = A.muinf + (A.mu0 - A.muinf) * pow(1.0 + val2, (A.nexp - 1.0) / A.aexp);
+      if (nz) {
+        const double val = pow(A.lam * gd, A.aexp - 1.0);
+        const double val1 = pow(1.0 + val2, (A.nexp - 1.0 - A.aexp) / A.aexp);
+        const double dmu_gd = (A.mu0 - A.muinf) * (A.nexp - 1.0) * A.lam * val * val1;
+        dmu = dmu_gd / gd;
+      }
+    }
+    double div = 0.0;
+#pragma unroll
+    for (int a = 0; a < DIM; a++) {
+      s_v[q][a] = v[a];
+      s_gT[q][a] = gT[a];
+      div += gv[a][a];
+#pragma unroll
+      for (int p = 0; p < DIM; p++) s_gv[q][p][a] = gv[p][a];
+      s_f[q][a] = EN ? A.rho * A.g[a] * (-A.beta * (Tq - A.Tref)) : 0.0;
+    }
+    s_P[q] = P;
+    s_mu[q] = mu;
+    s_dmu[q] = dmu;
+    s_div[q] = div;
+  }
+  __syncthreads();
+
+  const double rho = A.rho, rCp = A.rho * A.Cp, kc = A.kcond;
+  JT *__restrict__ out = (JT *)A.val;
+  const int *__restrict__ map = A.e2nz + (size_t)e * A.E;
+
+  // 5a. residual rows
+  for (int t = tid; t < NV * NVE + NP; t += NT) {
+    int row;
+    double R = 0.0;
+    if (t < NV * NVE) {
+      const int a = t / NVE, i = t - a * NVE;
+      row = s_gun[a][i];
+      if (a < DIM) {
+        for (int q = 0; q < NQ; q++) {
+          const double ph = tb[T::PHI + q * NVE + i];
+          double adv = 0.0, dif = 0.0;
+#pragma unroll
+          for (int p = 0; p < DIM; p++) {
+            adv += s_v[q][p] * s_gv[q][p][a];
+            const double Pi_ap = (a == p ? -s_P[q] : 0.0) + s_mu[q] * (s_gv[q][a][p] + s_gv[q][p][a]);
+            dif += s_g[q][i][p] * Pi_ap;
+          }
+          R += s_dA[q] * (-rho * ph * adv - dif + ph * s_f[q][a]);
+        }
+      } else {
+        for (int q = 0; q < NQ; q++) {
+          const double ph = tb[T::PHI + q * NVE + i];
+          double adv = 0.0, dif = 0.0;
+#pragma unroll
+          for (int p = 0; p < DIM; p++) {
+            adv += s_v[q][p] * s_gT[q][p];
+            dif += s_g[q][i][p] * s_gT[q][p];
+          }
+          R += s_dA[q] * (-rCp * ph * adv - kc * dif + ph * A.hsrc);
+        }
+      }
+    } else {
+      const int k = t - NV * NVE;
+      row = s_gp[k];
+      for (int q = 0; q < NQ; q++) R += s_dA[q] * tb[T::PSI + q * NP + k] * s_div[q];
+    }
+    if (A.dbc[row]) R = A.x[row] - A.dbc_val[row];
+    A.resid[row] += R;
+  }
+
+  // 5b. nodal-variable blocks, one node pair (i,j) per task
+  for (int t = tid; t < NVE2; t += NT) {
+    const int i = t / NVE, j = t - i * NVE;
+    double acc[NV][NV];
+#pragma unroll
+    for (int a = 0; a < NV; a++)
+#pragma unroll
+      for (int b = 0; b < NV; b++) acc[a][b] = 0.0;
+    for (int q = 0; q < NQ; q++) {
+      const double dA = s_dA[q];
+      const double phi_i = tb[T::PHI + q * NVE + i];
+      const double phi_j = tb[T::PHI + q * NVE + j];
+      double Gi[DIM], gj[DIM];
+#pragma unroll
+      for (int p = 0; p < DIM; p++) {
+        Gi[p] = s_g[q][i][p];
+        gj[p] = s_g[q][j][p];
+      }
+      double advj = 0.0, GG = 0.0;
+#pragma unroll
+      for (int p = 0; p < DIM; p++) {
+        advj += s_v[q][p] * gj[p];
+        GG += Gi[p] * gj[p];
+      }
+      const double mu = s_mu[q];
+      const double pij = phi_i * phi_j;
+      double Gg[DIM], dmb[DIM];  // sum_p Gi[p] gamma[a][p], c * sum_p gj[p] gamma[b][p]
+      if constexpr (VISC == 1) {
+        const double c = s_dmu[q];
+#pragma unroll
+        for (int a = 0; a < DIM; a++) {
+          double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+          for (int p = 0; p < DIM; p++) {
+            const double gam = s_gv[q][a][p] + s_gv[q][p][a];
+            s1 += Gi[p] * gam;
+            s2 += gj[p] * gam;
+          }
+          Gg[a] = s1;
+          dmb[a] = c * s2;
+        }
+      }
+      const double diag = rho * phi_i * advj + mu * GG;
+#pragma unroll
+      for (int a = 0; a < DIM; a++) {
+#pragma unroll
+        for (int b = 0; b < DIM; b++) {
+          double v = -rho * pij * s_gv[q][b][a] - mu * gj[a] * Gi[b];
+          if (a == b) v -= diag;
+          if constexpr (VISC == 1) v -= dmb[b] * Gg[a];
+          acc[a][b] += dA * v;
+        }
+      }
+      if constexpr (EN) {
+#pragma unroll
+        for (int a = 0; a < DIM; a++) {
+          acc[a][DIM] += dA * (-rho * A.g[a] * A.beta * pij);
+          acc[DIM][a] += dA * (-rCp * pij * s_gT[q][a]);
+        }
+        acc[DIM][DIM] += dA * (-rCp * phi_i * advj - kc * GG);
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < NV; a++) {
+      const int row = s_gun[a][i];
+      const bool dir = A.dbc[row] != 0;
+#pragma unroll
+      for (int b = 0; b < NV; b++) {
+        double v = acc[a][b];
+        if (dir) {
+          if (a == b && i == j) v = 1.0;
+          else continue;
+        }
+        const int idx = map[(a * NV + b) * NVE2 + t];
+        out[idx] += (JT)v;
+      }
+    }
+  }
+
+  // 5c. momentum x pressure (J_m_P): int grad_a phi_i psi_k dA
+  for (int t = tid; t < NVE * NP; t += NT) {
+    const int i = t / NP, k = t - i * NP;
+    double acc[DIM];
+#pragma unroll
+    for (int a = 0; a < DIM; a++) acc[a] = 0.0;
+    for (int q = 0; q < NQ; q++) {
+      const double w = s_dA[q] * tb[T::PSI + q * NP + k];
+#pragma unroll
+      for (int a = 0; a < DIM; a++) acc[a] += w * s_g[q][i][a];
+    }
+#pragma unroll
+    for (int a = 0; a < DIM; a++) {
+      if (A.dbc[s_gun[a][i]]) continue;
+      const int idx = map[A.off_mp + a * NVE * NP + t];
+      out[idx] += (JT)acc[a];
+    }
+  }
+
+  // 5d. continuity x velocity (J_c_v): int psi_k grad_b phi_j dA
+  for (int t = tid; t < NP * NVE; t += NT) {
+    const int k = t / NVE, j = t - k * NVE;
+    if (A.dbc[s_gp[k]]) continue;
+    double acc[DIM];
+#pragma unroll
+    for (int b = 0; b < DIM; b++) acc[b] = 0.0;
+    for (int q = 0; q < NQ; q++) {
+      const double w = s_dA[q] * tb[T::PSI + q * NP + k];
+#pragma unroll
+      for (int b = 0; b < DIM; b++) acc[b] += w * s_g[q][j][b];
+    }
+#pragma unroll
+    for (int b = 0; b < DIM; b++) {
+      const int idx = map[A.off_cv + b * NP * NVE + t];
+      out[idx] += (JT)acc[b];
+    }
+  }
+
+  // 5e. continuity x pressure: zero for Galerkin Q2-P1, unit diagonal on a
+  // Dirichlet (datum) row
+  if (tid < NP * NP) {
+    const int k = tid / NP, k2 = tid - k * NP;
+    if (k == k2 && A.dbc[s_gp[k]]) {
+      const int idx = map[A.off_cp + tid];
+      out[idx] += (JT)1.0;
+    }
+  }
+}
+
+// Element-to-nonzero map: for every local entry of every element, the MSR
+// index of the global (row, column) pair; the diagonal lives at index row.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_build_e2nz(const int *__restrict__ conn,
+                                                    const int *__restrict__ fu,
+                                                    const int *__restrict__ centroid,
+                                                    const int *__restrict__ bindx, int en,
+                                                    long long E, long long off_mp,
+                                                    long long off_cv, long long off_cp,
+                                                    int *__restrict__ e2nz, int *flag) {
+  constexpr int NVE = El<DIM>::NVE, NP = El<DIM>::NP;
+  const int NV = DIM + en;
+  const int e = blockIdx.x;
+  __shared__ int s_f[NVE];
+  __shared__ int s_pc;
+  if (threadIdx.x < NVE) s_f[threadIdx.x] = fu[conn[(size_t)e * NVE + threadIdx.x]];
+  if (threadIdx.x == 0) s_pc = fu[centroid[e]] + DIM + en;
+  __syncthreads();
+  for (long long idx = threadIdx.x; idx < E; idx += blockDim.x) {
+    int row, col;
+    if (idx < off_mp) {
+      const int blk = (int)(idx / (NVE * NVE)), t = (int)(idx - (long long)blk * NVE * NVE);
+      const int a = blk / NV, b = blk - a * NV;
+      row = s_f[t / NVE] + a;
+      col = s_f[t % NVE] + b;
+    } else if (idx < off_cv) {
+      const int r = (int)(idx - off_mp);
+      const int a = r / (NVE * NP), t = r - a * NVE * NP;
+      row = s_f[t / NP] + a;
+      col = s_pc + t % NP;
+    } else if (idx < off_cp) {
+      const int r = (int)(idx - off_cv);
+      const int b = r / (NP * NVE), t = r - b * NP * NVE;
+      row = s_pc + t / NVE;
+      col = s_f[t % NVE] + b;
+    } else {
+      const int r = (int)(idx - off_cp);
+      row = s_pc + r / NP;
+      col = s_pc + r % NP;
+    }
+    int pos;
+    if (row == col) {
+      pos = row;
+    } else {
+      int lo = bindx[row], hi = bindx[row + 1] - 1;
+      pos = -1;
+      while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = bindx[mid];
+        if (v == col) {
+          pos = mid;
+          break;
+        }
+        if (v < col) lo = mid + 1;
+        else hi = mid - 1;
+      }
+      if (pos < 0) atomicOr(&flag[1], 1);
+    }
+    e2nz[(size_t)e * E + idx] = pos;
+  }
+}
+
+// Reference basis tables on the host.  QUAD9 closed forms of
+// src/rf_shape.c:361-470; HEX27 as products of 1D quadratic Lagrange factors
+// (identical to the closed forms of src/rf_shape.c:1105-1460); Gauss points
+// ordered as find_stu (src/el_elm_info.c:1748-1823).
+static const signed char kHex27[27][3] = {
+    {-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, 1}, {1, -1, 1}, {1, 1, 1},
+    {-1, 1, 1},   {0, -1, -1}, {1, 0, -1}, {0, 1, -1},  {-1, 0, -1}, {-1, -1, 0}, {1, -1, 0},
+    {1, 1, 0},    {-1, 1, 0},  {0, -1, 1}, {1, 0, 1},   {0, 1, 1},   {-1, 0, 1},  {0, 0, 0},
+    {0, 0, -1},   {0, 0, 1},   {-1, 0, 0}, {1, 0, 0},   {0, -1, 0},  {0, 1, 0}};
+static const signed char kQuad9[9][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}, {0, -1},
+                                        {1, 0},   {0, 1},  {-1, 0}, {0, 0}};
+
+static void lagrange1(int node, double s, double &v, double &dv) {
+  if (node < 0) {
+    v = -0.5 * s * (1.0 - s);
+    dv = -0.5 + s;
+  } else if (node > 0) {
+    v = 0.5 * s * (1.0 + s);
+    dv = 0.5 + s;
+  } else {
+    v = 1.0 - s * s;
+    dv = -2.0 * s;
+  }
+}
+
+static std::vector<double> make_tables(int dim) {
+  const int nve = dim == 2 ? 9 : 27, nq = nve, np = dim + 1;
+  std::vector<double> t((size_t)nq * nve + (size_t)nq * nve * dim + (size_t)nq * np + nq);
+  double *phi = t.data(), *dphi = phi + nq * nve, *psi = dphi + nq * nve * dim, *wt = psi + nq * np;
+  const double F2 = 0.77459666924148340428, w1 = 0.55555555555555555556, w2 = 0.88888888888888888888;
+  const double pt[3] = {F2, 0.0, -F2}, ww[3] = {w1, w2, w1};
+  for (int q = 0; q < nq; q++) {
+    const int is = q % 3, it = (q % 9) / 3, iu = q / 9;
+    const double s = pt[is], tt = pt[it], u = dim == 3 ? pt[iu] : 0.0;
+    wt[q] = ww[is] * ww[it] * (dim == 3 ? ww[iu] : 1.0);
+    for (int k = 0; k < nve; k++) {
+      double a, da, b, db, cc = 1.0, dc = 0.0;
+      const signed char *pos = dim == 2 ? kQuad9[k] : kHex27[k];
+      lagrange1(pos[0], s, a, da);
+      lagrange1(pos[1], tt, b, db);
+      if (dim == 3) lagrange1(pos[2], u, cc, dc);
+      phi[q * nve + k] = a * b * cc;
+      dphi[(q * nve + k) * dim + 0] = da * b * cc;
+      dphi[(q * nve + k) * dim + 1] = a * db * cc;
+      if (dim == 3) dphi[(q * nve + k) * dim + 2] = a * b * dc;
+    }
+    // P1 discontinuous basis {1, s, t[, u]} (src/mm_fill_util.c:3802-3858, 4205-4276)
+    psi[q * np + 0] = 1.0;
+    psi[q * np + 1] = s;
+    psi[q * np + 2] = tt;
+    if (dim == 3) psi[q * np + 3] = u;
+  }
+  return t;
+}
+
+int gm_fill_setup_device(gm_ctx *c) {
+  ElemLayout &L = c->lay;
+  L.dim = c->dim;
+  L.nve = c->nve;
+  L.np = c->np;
+  L.nv = c->nv;
+  L.nq = c->nve;
+  const long long nve2 = (long long)L.nve * L.nve;
+  L.off_mp = (long long)L.nv * L.nv * nve2;
+  L.off_cv = L.off_mp + (long long)L.dim * L.nve * L.np;
+  L.off_cp = L.off_cv + (long long)L.dim * L.np * L.nve;
+  L.total = L.off_cp + (long long)L.np * L.np;
+
+  std::vector<double> tab = make_tables(c->dim);
+  GM_HIP(hipMalloc(&c->d_tables, tab.size() * sizeof(double)));
+  GM_HIP(hipMemcpy(c->d_tables, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  const size_t ne = (size_t)c->num_elems;
+  GM_HIP(hipMalloc(&c->d_e2nz, ne * L.total * sizeof(int)));
+  GM_HIP(hipMemsetAsync(c->d_flag, 0, 4 * sizeof(int), c->stream));
+  if (c->dim == 2)
+    k_build_e2nz<2><<<c->num_elems, 256, 0, c->stream>>>(c->d_conn, c->d_fu, c->d_centroid, c->d_bindx,
+                                                        c->energy, L.total, L.off_mp, L.off_cv,
+                                                        L.off_cp, c->d_e2nz, c->d_flag);
+  else
+    k_build_e2nz<3><<<c->num_elems, 256, 0, c->stream>>>(c->d_conn, c->d_fu, c->d_centroid, c->d_bindx,
+                                                        c->energy, L.total, L.off_mp, L.off_cv,
+                                                        L.off_cp, c->d_e2nz, c->d_flag);
+  GM_HIP(hipGetLastError());
+  int flag[4];
+  GM_HIP(hipMemcpyAsync(flag, c->d_flag, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  GM_HIP(hipStreamSynchronize(c->stream));
+  if (flag[1]) {
+    gm_set_error("element-to-nonzero map: entry missing from the MSR graph");
+    return GM_ERR_ARG;
+  }
+  return GM_OK;
+}
+
+template <int DIM, int EN, int VISC, typename JT>
+static void launch_fill(gm_ctx *c, FillArgs &a) {
+  constexpr int NT = DIM == 2 ? 128 : 256;
+  for (int col = 0; col < c->n_ecol; col++) {
+    const int cnt = c->ecol_ptr[col + 1] - c->ecol_ptr[col];
+    if (cnt == 0) continue;
+    k_fill<DIM, EN, VISC, JT, NT><<<cnt, NT, 0, c->stream>>>(a, c->d_ecol_elems + c->ecol_ptr[col]);
+  }
+}
+
+template <int DIM, typename JT>
+static void dispatch_fill(gm_ctx *c, FillArgs &a) {
+  const int en = c->energy, vm = c->phys.visc_model;
+  if (en && vm) launch_fill<DIM, 1, 1, JT>(c, a);
+  else if (en) launch_fill<DIM, 1, 0, JT>(c, a);
+  else if (vm) launch_fill<DIM, 0, 1, JT>(c, a);
+  else launch_fill<DIM, 0, 0, JT>(c, a);
+}
+
+int gm_fill_launch(gm_ctx *c, const double *x, double *resid, double *val, float *val32) {
+  FillArgs a;
+  a.coor = c->d_coor;
+  a.conn = c->d_conn;
+  a.fu = c->d_fu;
+  a.centroid = c->d_centroid;
+  a.dbc = c->d_dbc;
+  a.dbc_val = c->d_dbc_val;
+  a.tables = c->d_tables;
+  a.e2nz = c->d_e2nz;
+  a.x = x;
+  a.resid = resid;
+  a.val = val32 ? (void *)val32 : (void *)val;
+  a.flag = c->d_flag;
+  a.num_nodes = c->num_nodes;
+  a.E = c->lay.total;
+  a.off_mp = c->lay.off_mp;
+  a.off_cv = c->lay.off_cv;
+  a.off_cp = c->lay.off_cp;
+  const gm_physics &p = c->phys;
+  a.rho = p.rho;
+  a.mu0 = p.mu0;
+  a.muinf = p.muinf;
+  a.lam = p.lam;
+  a.nexp = p.nexp;
+  a.aexp = p.aexp;
+  a.Cp = p.Cp;
+  a.kcond = p.kcond;
+  a.hsrc = p.hsrc;
+  for (int k = 0; k < 3; k++) a.g[k] = p.g[k];
+  a.beta = p.beta;
+  a.Tref = p.Tref;
+  GM_HIP(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+  gm_prof_begin(c, GM_K_FILL);
+  if (c->dim == 2) {
+    if (val32) dispatch_fill<2, float>(c, a);
+    else dispatch_fill<2, double>(c, a);
+  } else {
+    if (val32) dispatch_fill<3, float>(c, a);
+    else dispatch_fill<3, double>(c, a);
+  }
+  gm_prof_end(c, GM_K_FILL);
+  GM_HIP(hipGetLastError());
+  int flag = 0;
+  GM_HIP(hipMemcpyAsync(&flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  GM_HIP(hipStreamSynchronize(c->stream));
+  if (flag) {
+    gm_set_error("zero or negative element Jacobian determinant");
+    return GM_ERR_ELEM;
+  }
+  return GM_OK;
+}

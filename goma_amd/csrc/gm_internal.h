@@ -1,0 +1,141 @@
+// Internal definitions shared by the libgm translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/gm_capi.h"
+
+#define GM_MAXROW 1024 /* longest Jacobian row (HEX27 vertex row: 125 nodes x 4 + 8 x 4 P = 532) */
+
+void gm_set_error(const char *fmt, ...);
+
+#define GM_HIP(call)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      gm_set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+      return GM_ERR_HIP;                                                             \
+    }                                                                                \
+  } while (0)
+
+// Element-to-nonzero map entry layout for one element (see gm_fill.hip):
+//   [nv*nv blocks][nve*nve node pairs]   nodal var A x nodal var B
+//   [dim][nve*np]                        momentum a  x pressure dof  (J_m_P)
+//   [dim][np*nve]                        continuity x velocity b     (J_c_v)
+//   [np*np]                              continuity x pressure       (J_c_P)
+struct ElemLayout {
+  int dim, nve, np, nv;  // nv = dim + energy nodal variables
+  int nq;
+  long long off_mp, off_cv, off_cp, total;
+};
+
+struct IluPlan {
+  int bins = -1;
+  int ncol = 0, npatch = 0;
+  std::vector<int> perm;         // perm[new] = old
+  std::vector<int> colour_ptr;   // patches per colour: patch index ranges
+  std::vector<int> patch_row;    // [npatch+1] first new row of each patch
+  long long nnz = 0;
+  int *d_perm = nullptr;      // [N]
+  int *d_rowptr = nullptr;    // [N+1]
+  int *d_col = nullptr;       // [nnz] new column ids, ascending per row
+  int *d_diag = nullptr;      // [N] position of the diagonal
+  int *d_src = nullptr;       // [nnz] MSR index of each entry
+  int *d_patch_row = nullptr; // [npatch+1]
+  double *d_lu = nullptr;     // [nnz]
+  float *d_lu32 = nullptr;    // [nnz] (mixed precision)
+  double *d_y = nullptr;      // [N] work (permuted numbering)
+  float *d_y32 = nullptr;
+};
+
+struct KrylovWork {
+  int m = 0;
+  int prec = -1;
+  double *d_V = nullptr;   // [(m+1) * N]
+  double *d_w = nullptr;   // [N]
+  double *d_z = nullptr;   // [N]
+  double *d_part = nullptr;  // partial dot products [(m+2) * nblk]
+  double *d_h = nullptr;   // [m+2]
+  float *d_V32 = nullptr;
+  float *d_w32 = nullptr, *d_z32 = nullptr;
+};
+
+struct gm_ctx {
+  int dim = 0, nve = 0, np = 0, nv = 0, num_nodes = 0, num_elems = 0, energy = 0;
+  gm_physics phys{};
+  int N = 0;
+  long long nval = 0;  // bindx[N]
+  std::vector<double> coor;
+  std::vector<int> conn;
+  std::vector<int> first_unknown, unk_var, elem_centroid, is_centroid;
+  std::vector<signed char> dbc;
+  std::vector<double> dbc_val;
+  std::vector<int> nn_ptr, nn_list;
+  std::vector<int> bindx;
+  int n_ecol = 0;
+  std::vector<int> ecol_ptr, ecol_elems, elem_colour;
+  ElemLayout lay{};
+
+  hipStream_t stream = nullptr;
+  double *d_coor = nullptr;
+  int *d_conn = nullptr;
+  int *d_fu = nullptr;
+  int *d_centroid = nullptr;
+  signed char *d_dbc = nullptr;
+  double *d_dbc_val = nullptr;
+  int *d_bindx = nullptr;
+  int *d_ecol_elems = nullptr;
+  int *d_e2nz = nullptr;        // [num_elems * lay.total]
+  int *d_flag = nullptr;        // device error flags
+  double *d_tables = nullptr;   // reference basis tables
+
+  // Newton work vectors
+  double *d_val = nullptr, *d_resid = nullptr, *d_dx = nullptr, *d_scale = nullptr;
+  float *d_val32 = nullptr;
+  double *d_red = nullptr;      // reduction scratch
+  double *h_red = nullptr;      // pinned host mirror
+  double *d_xwork = nullptr;
+
+  IluPlan ilu;
+  KrylovWork kry;
+
+  bool prof = false;
+  int prof_count[GM_K_COUNT] = {0};
+  double prof_ms[GM_K_COUNT] = {0};
+  hipEvent_t ev_b[GM_K_COUNT] = {nullptr}, ev_e[GM_K_COUNT] = {nullptr};
+};
+
+// set-up (gm_setup.cpp)
+int gm_build_host(gm_ctx *c);
+int gm_patch_order(const gm_ctx *c, int bins, std::vector<int> &perm, std::vector<int> &colour_ptr,
+                   std::vector<int> &patch_row);
+int gm_default_bins(const gm_ctx *c);
+
+// device set-up and kernels (gm_fill.hip)
+int gm_fill_setup_device(gm_ctx *c);
+int gm_fill_launch(gm_ctx *c, const double *x, double *resid, double *val, float *val32);
+
+// linear algebra (gm_linalg.hip)
+int gm_zero(gm_ctx *c, double *p, long long n);
+int gm_zero32(gm_ctx *c, float *p, long long n);
+int gm_scale_launch(gm_ctx *c, double *val, float *val32, double *b, double *scale);
+int gm_norms(gm_ctx *c, const double *v, long long n, double *l2, double *loo);
+int gm_corr_norms(gm_ctx *c, const double *dx, const double *x, double *l2r, double *loor);
+int gm_axpy_update(gm_ctx *c, double *x, const double *dx, double damp);
+int gm_spmv_launch(gm_ctx *c, const double *val, const double *x, double *y);
+int gm_ilu_plan(gm_ctx *c, int bins);
+int gm_ilu_factor(gm_ctx *c, const double *val, const float *val32, int prec);
+int gm_gmres_run(gm_ctx *c, const double *val, const float *val32, const double *b, double *x,
+                 const gm_solver_opts *o, int *its, double *relres);
+int gm_gmres_run_nofactor(gm_ctx *c, const double *val, const float *val32, const double *b,
+                          double *x, const gm_solver_opts *o, int *its, double *relres);
+void gm_free_linalg(gm_ctx *c);
+
+// timing helpers
+void gm_prof_begin(gm_ctx *c, int id);
+void gm_prof_end(gm_ctx *c, int id);
+int gm_ilu_factor_export(gm_ctx *c, const double *val, int bins, int *rowptr, int *col, double *lu);

@@ -1,0 +1,178 @@
+/*
+ * gm_capi.h -- C ABI of libgm.so, the MI355X-native replacement for Goma's
+ * Newton inner loop (steady incompressible Navier-Stokes + energy on
+ * QUAD9 / HEX27 Q2-P1 elements).
+ *
+ * The library keeps a context (mesh, unknown map, MSR graph, element
+ * colouring, element-to-nonzero map, preconditioner ordering) resident in
+ * HBM and runs the element assembly, the scatter, row-sum scaling, norms,
+ * ILU(0)-preconditioned GMRES and the Newton update as HIP kernels on gfx950.
+ *
+ * Conventions (Goma's, src/mm_sol_nonlinear.c:1108-2913, src/mm_fill.c:158-312):
+ *   - Unknowns are node-major, at each node U1,U2,(U3),(T),P (P1 pressure dofs
+ *     on the element centroid node); variable type ids are Goma's
+ *     (include/rf_fem_const.h:174-200): U1=0 U2=1 U3=2 T=3 P=9.
+ *   - The Jacobian is stored in Goma's MSR layout: val[0..N) diagonal,
+ *     bindx[0..N] row pointers into bindx/val for the off-diagonals,
+ *     bindx[N+1..] column indices; arrays have length bindx[N].
+ *   - matrix_fill accumulates (+=) into resid and val; callers zero them.
+ *   - Newton solves J dx = R (R row-sum scaled) and updates x -= damp*dx.
+ *   - No entry point aborts: every call returns GM_OK (0) or a negative
+ *     status; gm_last_error() gives a message.
+ *   - A context is not re-entrant: one host thread per context.  Every call
+ *     is synchronous: device work is complete when it returns.
+ *
+ * Pointer arguments ending in _dev are device (HBM) pointers; the others are
+ * host pointers.
+ */
+#ifndef GM_CAPI_H
+#define GM_CAPI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GM_OK 0
+#define GM_ERR_ARG -2      /* invalid argument */
+#define GM_ERR_HIP -3      /* HIP runtime error */
+#define GM_ERR_ELEM -1     /* zero or negative element Jacobian (matrix_fill_full's -1) */
+#define GM_ERR_PIVOT -4    /* zero pivot in ILU(0) */
+#define GM_ERR_NAN -5      /* non-finite residual norm (solve_nonlinear_problem's -1) */
+
+#define GM_VAR_U1 0
+#define GM_VAR_U2 1
+#define GM_VAR_U3 2
+#define GM_VAR_T 3
+#define GM_VAR_P 9
+
+#define GM_PREC_FP64 0   /* Jacobian, ILU and Krylov in FP64 */
+#define GM_PREC_MIXED 1  /* Jacobian values, ILU and inner Krylov FP32; R, dots, update FP64 */
+
+#define GM_PC_NONE 0
+#define GM_PC_ILU0 1     /* ILU(0) in patch order (see gm_solver_opts.patch_bins) */
+
+typedef struct gm_mesh {
+  int dim;             /* 2: QUAD9, 3: HEX27 */
+  int num_nodes;
+  int num_elems;
+  const double *coor;  /* [dim][num_nodes] (Goma's Coor) */
+  const int *connect;  /* [num_elems][9|27], Goma local node order (rf_shape.c) */
+} gm_mesh;
+
+typedef struct gm_physics {
+  int energy;          /* 1: energy equation and T unknowns active */
+  int visc_model;      /* 0: Newtonian (mu0); 1: Carreau(-Yasuda) */
+  double rho;
+  double mu0, muinf, lam, nexp, aexp; /* Carreau: mu = muinf + (mu0-muinf)(1+(lam gd)^a)^((n-1)/a) */
+  double Cp, kcond, hsrc;
+  double g[3];         /* momentum_source (gravity vector) */
+  double beta;         /* Volume_Expansion (Boussinesq) */
+  double Tref;         /* reference[TEMPERATURE] */
+} gm_physics;
+
+typedef struct gm_bc {
+  int num_dbc;         /* Dirichlet conditions: (node, variable type, value) */
+  const int *node;     /* later entries override earlier ones on one unknown */
+  const int *var;
+  const double *value;
+  int datum_elem;      /* pressure datum: element whose first P dof is fixed, -1: none */
+  double datum_value;
+} gm_bc;
+
+typedef struct gm_solver_opts {
+  int max_newton;      /* Max_Newton_Steps */
+  double eps_resid;    /* Normalized Residual Tolerance (Epsilon[0]) */
+  double eps_corr;     /* Normalized Correction Tolerance (Epsilon[2]); >1 disables */
+  double damp;         /* full-step damping factor */
+  int kspace;          /* GMRES restart length */
+  int max_its;         /* GMRES iteration limit per Newton step */
+  double lin_tol;      /* ||r||/||r0|| tolerance */
+  int precond;         /* GM_PC_NONE | GM_PC_ILU0 */
+  int patch_bins;      /* ILU(0) patches per axis (0: library default) */
+  int precision;       /* GM_PREC_FP64 | GM_PREC_MIXED */
+  int verbose;         /* 1: one line per Newton step on stderr */
+} gm_solver_opts;
+
+#define GM_MAX_REPORT 64
+typedef struct gm_newton_report {
+  int newton_its;      /* Goma's return value: passes through the Newton loop */
+  int converged;
+  int lin_its_total;
+  int lin_its[GM_MAX_REPORT];
+  double resid_L2[GM_MAX_REPORT];    /* row-sum-scaled ||R||_2 per pass */
+  double resid_Loo[GM_MAX_REPORT];
+  double unscaled_L2[GM_MAX_REPORT];
+  double lin_relres[GM_MAX_REPORT];
+  double t_assemble, t_scale, t_precond, t_solve, t_update, t_total; /* seconds */
+} gm_newton_report;
+
+typedef struct gm_ctx gm_ctx;
+
+/* Builds the unknown map, the MSR graph (find_MSR_problem_graph,
+ * src/mm_fill_util.c:3229), the element colouring and element-to-nonzero
+ * map, and the ILU patch ordering; uploads everything to the current HIP
+ * device.  Replaces the set-up Goma does in setup_problem / solve_problem
+ * (src/rf_setup_problem.c:194, src/rf_solve.c:729-769). */
+int gm_ctx_create(const gm_mesh *mesh, const gm_physics *phys, const gm_bc *bc, gm_ctx **out);
+void gm_ctx_destroy(gm_ctx *ctx);
+const char *gm_last_error(void);
+
+int gm_num_unknowns(const gm_ctx *ctx);
+int gm_msr_size(const gm_ctx *ctx);                /* bindx[N] = length of bindx and val */
+int gm_copy_msr_graph(const gm_ctx *ctx, int *bindx);        /* host copy, length bindx[N] */
+int gm_copy_first_unknown(const gm_ctx *ctx, int *fu);       /* [num_nodes+1] */
+int gm_copy_dirichlet(const gm_ctx *ctx, signed char *mask, double *value); /* [N] each */
+int gm_num_elem_colours(const gm_ctx *ctx);
+int gm_copy_elem_colours(const gm_ctx *ctx, int *colour);    /* [num_elems] */
+/* ILU patch ordering: perm[new] = old; colour_ptr[ncol+1]; returns ncol. */
+int gm_copy_patch_permutation(gm_ctx *ctx, int patch_bins, int *perm, int *colour_ptr);
+
+/* Replaces the material constants (continuation parameter update,
+ * update_parameterC, src/ac_update_parameter.c:48). */
+int gm_set_physics(gm_ctx *ctx, const gm_physics *phys);
+
+/* matrix_fill_full (src/mm_fill.c:158-312): accumulates the residual and the
+ * analytic Jacobian of every element, with Dirichlet rows replaced per element
+ * (put_dirichlet_in_matrix, src/bc_dirich.c:44).  val is MSR (gm_msr_size
+ * doubles).  Returns GM_OK or GM_ERR_ELEM. */
+int gm_matrix_fill_full_dev(gm_ctx *ctx, const double *x_dev, double *resid_dev, double *val_dev);
+int gm_matrix_fill_full(gm_ctx *ctx, const double *x, double *resid, double *val);
+
+/* row_sum_scale_MSR (src/sl_matrix_util.c:506): scales val and b in place. */
+int gm_row_sum_scale_dev(gm_ctx *ctx, double *val_dev, double *b_dev, double *scale_dev);
+/* MV_MSR (src/sl_auxutil.c:373): y = A x. */
+int gm_msr_matvec_dev(gm_ctx *ctx, const double *val_dev, const double *x_dev, double *y_dev);
+/* Preconditioned GMRES on an MSR matrix (the AZ_solve call,
+ * src/mm_sol_nonlinear.c:1736): x_dev holds the initial guess. */
+int gm_gmres_dev(gm_ctx *ctx, const double *val_dev, const double *b_dev, double *x_dev,
+                 const gm_solver_opts *opt, int *its, double *relres);
+/* ILU(0) factors of the patch-ordered matrix, as CSR in permuted numbering
+ * (for tests): rowptr[N+1], col[nnz], lu[nnz]; returns nnz or <0. */
+int gm_ilu0_factor_dev(gm_ctx *ctx, const double *val_dev, int patch_bins, int *rowptr, int *col,
+                       double *lu);
+
+/* solve_nonlinear_problem (src/mm_sol_nonlinear.c:596): steady Newton to
+ * ||D^-1 R||_2 < eps_resid and ||D^-1 R||_oo < eps_resid.  x is the initial
+ * guess on entry and the solution on exit.  Returns the number of passes
+ * through the Newton loop, or a negative status. */
+int gm_solve_nonlinear_dev(gm_ctx *ctx, double *x_dev, const gm_solver_opts *opt,
+                           gm_newton_report *rep);
+int gm_solve_nonlinear(gm_ctx *ctx, double *x, const gm_solver_opts *opt, gm_newton_report *rep);
+
+void gm_default_solver_opts(gm_solver_opts *opt);
+
+/* Kernel timing: when enabled, every launch of the named kernel family is
+ * bracketed by HIP events on the context's stream.  ids: */
+#define GM_K_FILL 0      /* element assembly + scatter, all colours of one fill */
+#define GM_K_SPMV 1      /* MSR matvec */
+#define GM_K_ILU_APPLY 2 /* both triangular sweeps of one ILU(0) application */
+#define GM_K_ILU_FACT 3  /* one ILU(0) factorisation */
+#define GM_K_ORTHO 4     /* one Gram-Schmidt step */
+#define GM_K_COUNT 5
+int gm_prof_enable(gm_ctx *ctx, int on);
+int gm_prof_get(gm_ctx *ctx, int kernel_id, int *count, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

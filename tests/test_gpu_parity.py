@@ -1,0 +1,186 @@
+"""GPU (libgm) vs FP64 CPU oracle on the same seeded inputs.
+
+Bars: integer set-up (unknown map, MSR graph, Dirichlet map, ILU ordering)
+bit-exact; FP64 residual and Jacobian within 1e-12 of the largest entry (the
+kernels and the oracle sum the same terms in different orders, with FMA
+contraction on the GPU); Newton solutions within 1e-9 of max|x|.
+"""
+import numpy as np
+import pytest
+
+from goma_amd import problems
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _cases():
+    return {
+        "cavity2d": lambda: problems.lid_cavity(n=4, Re=100.0),
+        "natconv2d": lambda: problems.natural_convection(n=4, Ra=1e4),
+        "carreau2d_jitter": lambda: problems.natural_convection(n=4, Ra=1e4, carreau=True, jitter=0.15),
+        "cavity3d_energy": lambda: problems.lid_cavity(n=2, Re=400.0, dim=3, energy=True, Ra=1e4),
+    }
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from goma_amd import capi as M
+
+    M.lib()
+    assert torch.cuda.is_available()
+    return M
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("name", list(_cases()))
+def test_setup_matches_oracle(name, capi, oracle_mod):
+    prob = _cases()[name]()
+    print(f"[case] setup {name}", flush=True)
+    ctx = capi.Context(prob)
+    orc = oracle_mod.Oracle(prob)
+    assert ctx.N == orc.N
+    assert ctx.nval == orc.nval
+    np.testing.assert_array_equal(ctx.bindx(), orc.ija)
+    np.testing.assert_array_equal(ctx.first_unknown(), orc.first_unknown)
+    m, v = ctx.dirichlet()
+    np.testing.assert_array_equal(m, orc.dbc_mask)
+    np.testing.assert_array_equal(v, orc.dbc_val)
+    bins = 2
+    perm, cptr = ctx.patch_permutation(bins)
+    import ctypes as C
+
+    L = oracle_mod.lib()
+    L.om_patch_permutation.argtypes = [C.c_void_p, C.c_int, oracle_mod._ip, oracle_mod._ip,
+                                       C.c_void_p, C.c_void_p]
+    operm = np.zeros(orc.N, np.int32)
+    ocp = np.zeros(orc.N + 2, np.int32)
+    nc = L.om_patch_permutation(orc.h, bins, operm, ocp, None, None)
+    np.testing.assert_array_equal(perm, operm)
+    np.testing.assert_array_equal(cptr, ocp[: nc + 1])
+    col = ctx.elem_colours()
+    conn = prob.mesh.connect
+    for c in np.unique(col):
+        nodes = conn[col == c].ravel()
+        assert len(nodes) == len(np.unique(nodes)), "elements of one colour share a node"
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", list(_cases()))
+def test_fill_matches_oracle(name, capi, oracle_mod):
+    prob = _cases()[name]()
+    print(f"[case] fill {name}", flush=True)
+    ctx = capi.Context(prob)
+    orc = oracle_mod.Oracle(prob)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(orc.N) * 0.5
+    rc, r_ref, v_ref = orc.fill(x)
+    assert rc == 0
+    xd, rd, vd = _dev(x), torch.zeros(ctx.N, dtype=torch.float64, device="cuda"), \
+        torch.zeros(ctx.nval, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    assert ctx.fill_dev(xd.data_ptr(), rd.data_ptr(), vd.data_ptr()) == 0
+    r, v = rd.cpu().numpy(), vd.cpu().numpy()
+    assert np.abs(r - r_ref).max() <= 1e-12 * np.abs(r_ref).max()
+    assert np.abs(v - v_ref).max() <= 1e-12 * np.abs(v_ref).max()
+    # accumulate semantics: a second fill doubles both arrays
+    assert ctx.fill_dev(xd.data_ptr(), rd.data_ptr(), vd.data_ptr()) == 0
+    assert np.abs(vd.cpu().numpy() - 2 * v_ref).max() <= 4e-12 * np.abs(v_ref).max()
+    # host-pointer drop-in
+    r2, v2 = np.zeros(ctx.N), np.zeros(ctx.nval)
+    assert ctx.fill_host(x, r2, v2) == 0
+    np.testing.assert_array_equal(r2, r)
+    np.testing.assert_array_equal(v2, v)
+    # row-sum scaling and matvec
+    s_ref = np.zeros(orc.N)
+    oracle_mod.lib().om_row_sum_scale(orc.N, v_ref, orc.ija, r_ref, s_ref)
+    sd = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    rd2, vd2 = _dev(r), _dev(v)
+    ctx.row_sum_scale_dev(vd2.data_ptr(), rd2.data_ptr(), sd.data_ptr())
+    np.testing.assert_allclose(sd.cpu().numpy(), s_ref, rtol=1e-12)
+    np.testing.assert_allclose(rd2.cpu().numpy(), r_ref, rtol=1e-11, atol=1e-14)
+    y_ref = np.zeros(orc.N)
+    oracle_mod.lib().om_msr_matvec(orc.N, v_ref, orc.ija, x, y_ref)
+    yd = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    ctx.matvec_dev(vd2.data_ptr(), xd.data_ptr(), yd.data_ptr())
+    assert np.abs(yd.cpu().numpy() - y_ref).max() <= 1e-12 * np.abs(y_ref).max()
+    ctx.close()
+
+
+def test_fill_determinism(capi):
+    prob = problems.natural_convection(n=8, Ra=1e4)
+    ctx = capi.Context(prob)
+    x = _dev(np.random.default_rng(3).standard_normal(ctx.N))
+    outs = []
+    for _ in range(2):
+        rd = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+        vd = torch.zeros(ctx.nval, dtype=torch.float64, device="cuda")
+        assert ctx.fill_dev(x.data_ptr(), rd.data_ptr(), vd.data_ptr()) == 0
+        outs.append((rd.cpu().numpy(), vd.cpu().numpy()))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", ["natconv2d", "cavity3d_energy"])
+def test_ilu_factors_match_oracle(name, capi, oracle_mod):
+    prob = _cases()[name]()
+    print(f"[case] ilu {name}", flush=True)
+    ctx = capi.Context(prob)
+    orc = oracle_mod.Oracle(prob)
+    x = np.random.default_rng(11).standard_normal(orc.N) * 0.3
+    rc, r, v = orc.fill(x)
+    s = np.zeros(orc.N)
+    oracle_mod.lib().om_row_sum_scale(orc.N, v, orc.ija, r, s)
+    bins = 2
+    perm, _ = ctx.patch_permutation(bins)
+    rc, orow, ocol, olu = orc.ilu0(v, perm)
+    assert rc > 0
+    vd = _dev(v)
+    rowptr, col, lu = ctx.ilu0_factor_dev(vd.data_ptr(), bins)
+    np.testing.assert_array_equal(rowptr, orow)
+    np.testing.assert_array_equal(col, ocol)
+    assert np.abs(lu - olu).max() <= 1e-10 * np.abs(olu).max()
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", ["cavity2d", "natconv2d"])
+def test_gmres_true_residual(name, capi, oracle_mod):
+    prob = _cases()[name]()
+    ctx = capi.Context(prob)
+    orc = oracle_mod.Oracle(prob)
+    x = np.random.default_rng(5).standard_normal(orc.N) * 0.3
+    rc, r, v = orc.fill(x)
+    s = np.zeros(orc.N)
+    oracle_mod.lib().om_row_sum_scale(orc.N, v, orc.ija, r, s)
+    vd, bd = _dev(v), _dev(r)
+    xd = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    its, rr = ctx.gmres_dev(vd.data_ptr(), bd.data_ptr(), xd.data_ptr(), kspace=50, max_its=500,
+                            lin_tol=1e-8, patch_bins=2)
+    xs = xd.cpu().numpy()
+    A = oracle_mod.msr_to_scipy(orc.ija, v, orc.N)
+    assert np.linalg.norm(r - A @ xs) <= 1.01e-8 * np.linalg.norm(r)
+    assert rr < 1e-8
+    ctx.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_newton_matches_oracle(prec, capi, oracle_mod):
+    prob = problems.lid_cavity(n=8, Re=100.0)
+    print(f"[case] newton cavity n=8 prec={prec}", flush=True)
+    ctx = capi.Context(prob)
+    orc = oracle_mod.Oracle(prob)
+    rc, x_ref, rep_ref = orc.newton(precond=1, kspace=50, max_its=1000)
+    assert rep_ref.converged
+    xd = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    rc, rep = ctx.solve_dev(xd.data_ptr(), kspace=50, max_its=1000, patch_bins=2, precision=prec)
+    assert rep.converged
+    assert rep.resid_L2[rep.newton_its - 1] < 1e-10
+    x = xd.cpu().numpy()
+    assert np.abs(x - x_ref).max() <= 1e-9 * np.abs(x_ref).max()
+    ctx.close()

@@ -129,6 +129,21 @@ static void gauss_point(int dim, int iq, double *s, double *t, double *u, double
   *w = wt[is] * wt[it] * ((dim == 3) ? wt[iu] : 1.0);
 }
 
+
+/* Basis/quadrature accessors for tests. */
+void om_shape(int dim, double s, double t, double u, double *phi, double *dphi) {
+  double d[MDE][3];
+  if (dim == 2) shape_quad9(s, t, phi, d);
+  else shape_hex27(s, t, u, phi, d);
+  int n = dim == 2 ? 9 : 27;
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) dphi[i * 3 + k] = d[i][k];
+}
+
+void om_gauss(int dim, int iq, double *stuw) {
+  gauss_point(dim, iq, &stuw[0], &stuw[1], &stuw[2], &stuw[3]);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Problem set-up: unknown map, node graph, MSR graph, Dirichlet map         */
 /* ------------------------------------------------------------------------ */

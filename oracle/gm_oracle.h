@@ -82,7 +82,7 @@ typedef struct om_solver_opts {
   int max_its;        /* GMRES max iterations per Newton step */
   double lin_tol;     /* ||r||/||r0|| tolerance (AZ_r0) */
   int precond;        /* 0: none, 1: ILU(0) natural order, 2: ILU(0) node-colour order,
-                         3+k: ILU(0) in patch order with k = precond-2 bins per axis */
+                         b+2 (b >= 1): ILU(0) in patch order with b patch bins per axis */
 } om_solver_opts;
 
 typedef struct om_report {
@@ -97,6 +97,11 @@ typedef struct om_report {
 } om_report;
 
 typedef struct om_problem om_problem;
+
+/* Q2 shape functions phi[nve] and dphi[nve][3] at (s,t,u); Gauss point iq
+ * of the 3-point tensor rule: stuw = {s, t, u, weight}. */
+void om_shape(int dim, double s, double t, double u, double *phi, double *dphi);
+void om_gauss(int dim, int iq, double *stuw);
 
 om_problem *om_create(const om_mesh *mesh, const om_physics *phys, const om_bc *bc);
 void om_destroy(om_problem *p);

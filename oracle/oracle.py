@@ -90,6 +90,11 @@ def lib():
         L.om_gmres_msr.argtypes = [C.c_int, _dp, _ip, _dp, _dp, C.c_void_p, C.c_int, C.c_int,
                                    C.c_int, C.c_double, C.POINTER(C.c_double)]
         L.om_ilu0_csr.argtypes = [C.c_int, _dp, _ip, C.c_void_p, _ip, _ip, _dp]
+        L.om_shape.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp]
+        L.om_gauss.argtypes = [C.c_int, C.c_int, _dp]
+        L.om_patch_permutation.argtypes = [C.c_void_p, C.c_int, _ip, _ip, C.c_void_p, C.c_void_p]
+        L.om_gmres_perm.argtypes = [C.c_int, _dp, _ip, _dp, _dp, C.c_void_p, C.c_void_p, C.c_int,
+                                    C.c_int, C.c_double, C.POINTER(C.c_double)]
         L.om_newton.argtypes = [C.c_void_p, _dp, C.POINTER(OmSolverOpts), C.POINTER(OmReport)]
         _lib = L
     return _lib
@@ -160,6 +165,12 @@ class Oracle:
         lib().om_colour_permutation(self.h, perm, cptr, C.byref(nc))
         return perm, cptr[: nc.value + 1].copy()
 
+    def patch_permutation(self, bins):
+        perm = np.zeros(self.N, np.int32)
+        cptr = np.zeros(self.N + 2, np.int32)
+        nc = lib().om_patch_permutation(self.h, bins, perm, cptr, None, None)
+        return perm, cptr[: nc + 1].copy()
+
     def gmres(self, val, b, precond=1, kspace=30, max_its=500, tol=1e-6, x0=None):
         x = np.zeros(self.N) if x0 is None else np.array(x0, dtype=np.float64)
         rr = C.c_double(0.0)
@@ -201,3 +212,21 @@ def msr_to_scipy(ija, val, N):
     vals.append(val[N + 1: ija[N]])
     return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
                          shape=(N, N))
+
+
+def shape(dim, s, t, u=0.0):
+    n = 9 if dim == 2 else 27
+    phi = np.zeros(n)
+    dphi = np.zeros(n * 3)
+    lib().om_shape(dim, s, t, u, phi, dphi)
+    return phi, dphi.reshape(n, 3)[:, :dim].copy()
+
+
+def gauss(dim):
+    nq = 9 if dim == 2 else 27
+    out = np.zeros((nq, 4))
+    for q in range(nq):
+        v = np.zeros(4)
+        lib().om_gauss(dim, q, v)
+        out[q] = v
+    return out

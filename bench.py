@@ -1,0 +1,271 @@
+#!/usr/bin/env python3
+"""Benchmark: seconds per converged steady Newton solve on one MI355X.
+
+A step is one converged Newton solve (||D^-1 R||_2 < 1e-10 and
+||D^-1 R||_oo < 1e-10, Goma's test) from the zero initial guess, with the
+mesh, matrix graph and all work arrays already resident in HBM.  The default
+workload is 2D Boussinesq natural convection on a 256x256 QUAD9 mesh
+(Q2/P1 velocity-pressure plus Q2 temperature, Pr = 0.71, Ra = 1e5), FP64.
+
+Prints one JSON line.  Besides the headline value it reports, for the kernel
+that takes the most time, its algorithmic bytes per launch over its average
+launch duration (HIP events around every launch on the library's stream) as
+a fraction of the MI355X HBM peak, and the oracle's FP64 CPU path timed on a
+bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = ("s per converged Newton solve (‖R‖<1e-10) vs plain FP64 GPU; "
+          "assembly HBM GB/s")
+
+
+def make_problem(name: str, n: int | None):
+    from goma_amd import problems
+
+    if name == "natconv2d":
+        return problems.natural_convection(n=n or 256, Ra=1e5, Pr=0.71), dict(Ra=1e5, Pr=0.71)
+    if name == "cavity2d":
+        return problems.lid_cavity(n=n or 32, Re=100.0), dict(Re=100.0)
+    if name == "cavity3d":
+        return (problems.lid_cavity(n=n or 48, Re=400.0, dim=3, energy=True, Ra=1e4, Pr=0.71),
+                dict(Re=400.0, Ra=1e4, Pr=0.71))
+    if name == "carreau2d":
+        return (problems.natural_convection(n=n or 128, Ra=1e3, carreau=True, jitter=0.15),
+                dict(Ra=1e3, jitter=0.15))
+    raise SystemExit(f"unknown workload {name}")
+
+
+def algorithmic_bytes(ctx, prob, prec: int):
+    """Compulsory HBM bytes per launch family (counted once)."""
+    m = prob.mesh
+    N, nval = ctx.N, ctx.nval
+    nnz = nval - 1  # diagonal + off-diagonals
+    sv = 4 if prec == 1 else 8
+    d = m.dim
+    nve = m.nodes_per_elem
+    # assembly: write J once, write R and read x in FP64, coordinates, connectivity
+    b_asm = nnz * sv + 16 * N + 8 * d * m.num_nodes + 4 * nve * m.num_elems
+    # SpMV y = A x: values + column ids, row pointers, x and y
+    b_spmv = nnz * (sv + 4) + 4 * (N + 1) + N * (sv + sv)
+    # one ILU sweep (L or U half): its values and column ids, row pointers,
+    # diagonal pointers, permutation, input and output vectors
+    half = (nnz - N) / 2
+    b_sweep_L = half * (sv + 4) + 4 * (N + 1) + 4 * N + 4 * N + N * (sv + sv)
+    b_sweep_U = (half + N) * (sv + 4) + 4 * (N + 1) + 4 * N + 4 * N + N * (sv + sv)
+    return dict(asm=b_asm, spmv=b_spmv, ilu_lower=b_sweep_L, ilu_upper=b_sweep_U)
+
+
+def cpu_baseline(prob, gpu_counts: dict, budget_s: float):
+    """Oracle (single-threaded FP64 restatement of Goma's path) on a bounded sample.
+
+    Times: one assembly over a prefix of the elements (scaled to the mesh),
+    one ILU(0) factorisation and a few GMRES iterations on the full matrix;
+    the per-solve figure scales each by the counts the GPU solve needed.
+    """
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ctypes as C
+
+    import oracle as O
+
+    O.build()
+    o = O.Oracle(prob)
+    ne = prob.mesh.num_elems
+    x = np.zeros(o.N)
+    # assembly sample
+    e1 = min(ne, 4096)
+    t0 = time.perf_counter()
+    rc, r, v = o.fill(x, 0, e1)
+    t_asm_elem = (time.perf_counter() - t0) / e1
+    # full matrix for the solver sample
+    t0 = time.perf_counter()
+    rc, r, v = o.fill(x)
+    t_full = time.perf_counter() - t0
+    s = np.zeros(o.N)
+    O.lib().om_row_sum_scale(o.N, v, o.ija, r, s)
+    bins = gpu_counts["patch_bins"]
+    its_sample = 10
+    t0 = time.perf_counter()
+    xx = np.zeros(o.N)
+    rr = C.c_double()
+    its = O.lib().om_gmres_msr(o.N, v, o.ija, r, xx, o.h, 2 + bins, 50, its_sample, 1e-30,
+                               C.byref(rr))
+    t_gm = time.perf_counter() - t0
+    # separate the factorisation from the iterations
+    t0 = time.perf_counter()
+    its0 = O.lib().om_gmres_msr(o.N, v, o.ija, r, np.zeros(o.N), o.h, 2 + bins, 50, 1, 1e-30,
+                                C.byref(rr))
+    t_one = time.perf_counter() - t0
+    t_it = max((t_gm - t_one) / max(its - its0, 1), 1e-9)
+    t_fact = max(t_one - t_it, 0.0)
+    est = (gpu_counts["assemblies"] * t_asm_elem * ne + gpu_counts["factorisations"] * t_fact
+           + gpu_counts["gmres_its"] * t_it)
+    return dict(value=est, unit="s per converged solve (extrapolated)", cores=1, kind="port",
+                sample=(f"oracle FP64 C, 1 thread: assembly of {e1} of {ne} elements "
+                        f"({t_asm_elem * 1e6:.1f} us/element; full assembly {t_full:.2f} s), one "
+                        f"ILU(0) factorisation ({t_fact:.2f} s) and {its} GMRES(50) iterations "
+                        f"({t_it * 1e3:.1f} ms/iteration) on the full {o.N}-unknown system at the "
+                        f"zero guess; scaled by the GPU solve's {gpu_counts['assemblies']} "
+                        f"assemblies, {gpu_counts['factorisations']} factorisations and "
+                        f"{gpu_counts['gmres_its']} GMRES iterations"),
+                assembly_elems_per_s=1.0 / t_asm_elem, gmres_it_s=t_it, ilu_factor_s=t_fact)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="natconv2d")
+    ap.add_argument("--n", type=int, default=None, help="elements per side (default: workload's)")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--kspace", type=int, default=50)
+    ap.add_argument("--max-its", type=int, default=500)
+    ap.add_argument("--patch-bins", type=int, default=0)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--verbose", action="store_true")
+    args = ap.parse_args(argv)
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
+              "the multi-GPU decomposition is a design, not a build, in this project.",
+              file=sys.stderr)
+        return 2
+
+    import torch
+
+    from goma_amd import capi
+
+    prec = 1 if args.precision == "mixed" else 0
+    prob, params = make_problem(args.workload, args.n)
+    t0 = time.perf_counter()
+    ctx = capi.Context(prob)
+    t_setup = time.perf_counter() - t0
+    bins = args.patch_bins or 0
+    print(f"[bench] {prob.name}: N={ctx.N} nnz={ctx.nval - 1} setup {t_setup:.2f}s", file=sys.stderr,
+          flush=True)
+    x = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    opts = dict(kspace=args.kspace, max_its=args.max_its, patch_bins=bins, precision=prec,
+                max_newton=40, verbose=1 if args.verbose else 0)
+    for w in range(args.warmup):
+        x.zero_()
+        torch.cuda.synchronize()
+        rc, rep = ctx.solve_dev(x.data_ptr(), **opts)
+        print(f"[bench] warmup {w}: newton {rep.newton_its} gmres {rep.lin_its_total} "
+              f"{rep.t_total:.2f}s", file=sys.stderr, flush=True)
+    ctx.prof_enable(True)
+    reps = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        x.zero_()
+        rc, rep = ctx.solve_dev(x.data_ptr(), **opts)
+        reps.append(rep.summary())
+        print(f"[bench] step {k}: newton {rep.newton_its} gmres {rep.lin_its_total} "
+              f"converged {bool(rep.converged)} {rep.t_total:.2f}s", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = {nm: ctx.prof_get(i) for i, nm in enumerate(
+        ["fill", "spmv", "ilu_lower", "ilu_upper", "ilu_factor", "mdot", "orth"])}
+    ctx.prof_enable(False)
+    if not all(r["converged"] for r in reps):
+        print("[bench] WARNING: a step did not converge", file=sys.stderr, flush=True)
+
+    byt = algorithmic_bytes(ctx, prob, prec)
+    ms_step = elapsed / args.steps * 1e3
+    fills = prof["fill"][0]
+    n_ecol = capi.lib().gm_num_elem_colours(ctx.h)
+    assemblies = fills // max(n_ecol, 1)
+    kern = {}
+    for nm, (cnt, ms) in prof.items():
+        if cnt:
+            kern[nm] = dict(launches=cnt, total_ms=ms, avg_ms=ms / cnt)
+    # dominant kernel and its roofline
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    total_bytes = {"fill": byt["asm"] * assemblies, "spmv": byt["spmv"] * kern.get("spmv", {}).get("launches", 0),
+                   "ilu_lower": byt["ilu_lower"] * (kern.get("ilu_lower", {}).get("launches", 0) / max(1, _ncol(ctx))),
+                   "ilu_upper": byt["ilu_upper"] * (kern.get("ilu_upper", {}).get("launches", 0) / max(1, _ncol(ctx)))}
+    roof = None
+    for nm in kern:
+        if nm in total_bytes and kern[nm]["total_ms"] > 0:
+            gbs = total_bytes[nm] / (kern[nm]["total_ms"] * 1e-3) / 1e9
+            kern[nm]["GBps"] = gbs
+            kern[nm]["bytes_per_launch"] = total_bytes[nm] / kern[nm]["launches"]
+    if dom in total_bytes:
+        a = kern[dom]["GBps"]
+        roof = dict(kernel=dom, bound="hbm", achieved=round(a, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(a / HBM_PEAK_GBS, 4), traffic=None,
+                    bytes_per_launch=kern[dom]["bytes_per_launch"], avg_launch_ms=kern[dom]["avg_ms"])
+    gm_its = sum(r["lin_its_total"] for r in reps) // len(reps)
+    newton = reps[0]["newton_its"]
+    asm_s = kern["fill"]["total_ms"] * 1e-3 / max(assemblies, 1)
+    out = {
+        "metric": METRIC,
+        "value": round(elapsed / args.steps, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == 0 else "f32/f64",
+        "data": "synthetic",
+        "config": {"workload": f"{prob.name} ({args.workload}) " + ", ".join(f"{k}={v}" for k, v in params.items()),
+                   "unknowns": ctx.N, "nnz": ctx.nval - 1, "elements": prob.mesh.num_elems,
+                   "precision": args.precision, "solver": f"GMRES({args.kspace}) + ILU(0) patch order",
+                   "initial_guess": "zero"},
+        "newton_its": newton,
+        "gmres_its": gm_its,
+        "final_resid_L2": reps[0]["resid_L2"][-1] if reps[0]["resid_L2"] else None,
+        "assembly_s": round(asm_s, 6),
+        "assembly_GBps": round(byt["asm"] / asm_s / 1e9, 1) if asm_s > 0 else None,
+        "element_assemblies_per_s": round(prob.mesh.num_elems / asm_s, 1) if asm_s > 0 else None,
+        "spmv_GBps": round(kern["spmv"]["GBps"], 1) if "spmv" in kern else None,
+        "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in kern.items()},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        counts = dict(assemblies=assemblies // max(args.steps, 1), gmres_its=gm_its,
+                      factorisations=max(newton - 1, 1), patch_bins=_bins(ctx, args))
+        out["cpu_baseline"] = cpu_baseline(prob, counts, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+def _ncol(ctx):
+    return ctx.ilu_colours if hasattr(ctx, "ilu_colours") else _ilu_colours(ctx)
+
+
+def _ilu_colours(ctx):
+    perm, cptr = ctx.patch_permutation(getattr(ctx, "_bins", 0))
+    ctx.ilu_colours = len(cptr) - 1
+    return ctx.ilu_colours
+
+
+def _bins(ctx, args):
+    if args.patch_bins:
+        return args.patch_bins
+    import math
+
+    per = ctx.prob.mesh.n
+    return max(1, round(per / (8 if ctx.prob.mesh.dim == 2 else 4)))
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -27,8 +27,8 @@ GM_PREC_FP64 = 0
 GM_PREC_MIXED = 1
 GM_PC_NONE = 0
 GM_PC_ILU0 = 1
-K_FILL, K_SPMV, K_ILU_APPLY, K_ILU_FACT, K_ORTHO = range(5)
-KERNEL_NAMES = ["fill", "spmv", "ilu_apply", "ilu_factor", "ortho"]
+K_FILL, K_SPMV, K_ILU_LOWER, K_ILU_UPPER, K_ILU_FACT, K_MDOT, K_ORTH = range(7)
+KERNEL_NAMES = ["fill", "spmv", "ilu_lower", "ilu_upper", "ilu_factor", "mdot", "orth"]
 
 
 class GmMesh(C.Structure):

@@ -23,19 +23,39 @@ static double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+static constexpr int kProfPool = 2048;
+
+// fold recorded event pairs of one kernel id into the totals
+static void prof_fold(gm_ctx *c, int id) {
+  if (!c->ev_used[id]) return;
+  hipEventSynchronize(c->ev_e[id][c->ev_used[id] - 1]);
+  for (int i = 0; i < c->ev_used[id]; i++) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev_b[id][i], c->ev_e[id][i]);
+    c->prof_ms[id] += ms;
+  }
+  c->prof_count[id] += c->ev_used[id];
+  c->ev_used[id] = 0;
+}
+
 void gm_prof_begin(gm_ctx *c, int id) {
   if (!c->prof) return;
-  hipEventRecord(c->ev_b[id], c->stream);
+  if (c->ev_used[id] == kProfPool) prof_fold(c, id);
+  const int k = c->ev_used[id];
+  if ((int)c->ev_b[id].size() <= k) {
+    hipEvent_t b, e;
+    hipEventCreate(&b);
+    hipEventCreate(&e);
+    c->ev_b[id].push_back(b);
+    c->ev_e[id].push_back(e);
+  }
+  hipEventRecord(c->ev_b[id][k], c->stream);
 }
 
 void gm_prof_end(gm_ctx *c, int id) {
   if (!c->prof) return;
-  hipEventRecord(c->ev_e[id], c->stream);
-  hipEventSynchronize(c->ev_e[id]);
-  float ms = 0.f;
-  hipEventElapsedTime(&ms, c->ev_b[id], c->ev_e[id]);
-  c->prof_ms[id] += ms;
-  c->prof_count[id]++;
+  hipEventRecord(c->ev_e[id][c->ev_used[id]], c->stream);
+  c->ev_used[id]++;
 }
 
 extern "C" void gm_default_solver_opts(gm_solver_opts *o) {
@@ -44,7 +64,7 @@ extern "C" void gm_default_solver_opts(gm_solver_opts *o) {
   o->eps_corr = 1e10;
   o->damp = 1.0;
   o->kspace = 50;
-  o->max_its = 2000;
+  o->max_its = 500;
   o->lin_tol = 1e-6;
   o->precond = GM_PC_ILU0;
   o->patch_bins = 0;
@@ -62,8 +82,8 @@ static void free_ctx(gm_ctx *c) {
     if (p) hipFree(p);
   if (c->h_red) hipHostFree(c->h_red);
   for (int i = 0; i < GM_K_COUNT; i++) {
-    if (c->ev_b[i]) hipEventDestroy(c->ev_b[i]);
-    if (c->ev_e[i]) hipEventDestroy(c->ev_e[i]);
+    for (hipEvent_t e : c->ev_b[i]) hipEventDestroy(e);
+    for (hipEvent_t e : c->ev_e[i]) hipEventDestroy(e);
   }
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -131,10 +151,6 @@ static int create_impl(const gm_mesh *m, const gm_physics *ph, const gm_bc *bc, 
   }
 
   GM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  for (int i = 0; i < GM_K_COUNT; i++) {
-    GM_HIP(hipEventCreate(&c->ev_b[i]));
-    GM_HIP(hipEventCreate(&c->ev_e[i]));
-  }
   if ((rc = upload(&c->d_coor, c->coor.data(), c->coor.size()))) return rc;
   if ((rc = upload(&c->d_conn, c->conn.data(), c->conn.size()))) return rc;
   if ((rc = upload(&c->d_fu, c->first_unknown.data(), c->first_unknown.size()))) return rc;
@@ -397,17 +413,20 @@ extern "C" int gm_solve_nonlinear(gm_ctx *c, double *x, const gm_solver_opts *o,
 
 extern "C" int gm_prof_enable(gm_ctx *c, int on) {
   if (!c) return GM_ERR_ARG;
+  if (c->stream) hipStreamSynchronize(c->stream);
   c->prof = on != 0;
   for (int i = 0; i < GM_K_COUNT; i++) {
     c->prof_count[i] = 0;
     c->prof_ms[i] = 0.0;
+    c->ev_used[i] = 0;
   }
   return GM_OK;
 }
 
 extern "C" int gm_prof_get(gm_ctx *c, int id, int *count, double *ms) {
   if (!c || id < 0 || id >= GM_K_COUNT || !count || !ms) return GM_ERR_ARG;
-  *count = c->prof_count[id];
+  prof_fold(c, id);
+  *count = (int)c->prof_count[id];
   *ms = c->prof_ms[id];
   return GM_OK;
 }

@@ -571,7 +571,9 @@ static void launch_fill(gm_ctx *c, FillArgs &a) {
   for (int col = 0; col < c->n_ecol; col++) {
     const int cnt = c->ecol_ptr[col + 1] - c->ecol_ptr[col];
     if (cnt == 0) continue;
+    gm_prof_begin(c, GM_K_FILL);
     k_fill<DIM, EN, VISC, JT, NT><<<cnt, NT, 0, c->stream>>>(a, c->d_ecol_elems + c->ecol_ptr[col]);
+    gm_prof_end(c, GM_K_FILL);
   }
 }
 
@@ -617,7 +619,6 @@ int gm_fill_launch(gm_ctx *c, const double *x, double *resid, double *val, float
   a.beta = p.beta;
   a.Tref = p.Tref;
   GM_HIP(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
-  gm_prof_begin(c, GM_K_FILL);
   if (c->dim == 2) {
     if (val32) dispatch_fill<2, float>(c, a);
     else dispatch_fill<2, double>(c, a);
@@ -625,7 +626,6 @@ int gm_fill_launch(gm_ctx *c, const double *x, double *resid, double *val, float
     if (val32) dispatch_fill<3, float>(c, a);
     else dispatch_fill<3, double>(c, a);
   }
-  gm_prof_end(c, GM_K_FILL);
   GM_HIP(hipGetLastError());
   int flag = 0;
   GM_HIP(hipMemcpyAsync(&flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));

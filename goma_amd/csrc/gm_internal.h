@@ -104,9 +104,10 @@ struct gm_ctx {
   KrylovWork kry;
 
   bool prof = false;
-  int prof_count[GM_K_COUNT] = {0};
+  long long prof_count[GM_K_COUNT] = {0};
   double prof_ms[GM_K_COUNT] = {0};
-  hipEvent_t ev_b[GM_K_COUNT] = {nullptr}, ev_e[GM_K_COUNT] = {nullptr};
+  std::vector<hipEvent_t> ev_b[GM_K_COUNT], ev_e[GM_K_COUNT];
+  int ev_used[GM_K_COUNT] = {0};
 };
 
 // set-up (gm_setup.cpp)

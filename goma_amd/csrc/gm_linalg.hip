@@ -439,7 +439,6 @@ int gm_ilu_factor(gm_ctx *c, const double *val, const float *val32, int prec) {
     GM_HIP(hipMalloc(&P.d_y32, c->N * sizeof(float)));
   }
   GM_HIP(hipMemsetAsync(c->d_flag + 2, 0, sizeof(int), c->stream));
-  gm_prof_begin(c, GM_K_ILU_FACT);
   if (prec == GM_PREC_MIXED) {
     if (val32) k_gather<float, float><<<4096, 256, 0, c->stream>>>(P.nnz, P.d_src, val32, P.d_lu32);
     else k_gather<double, float><<<4096, 256, 0, c->stream>>>(P.nnz, P.d_src, val, P.d_lu32);
@@ -449,14 +448,15 @@ int gm_ilu_factor(gm_ctx *c, const double *val, const float *val32, int prec) {
   for (int col = 0; col < P.ncol; col++) {
     const int p0 = P.colour_ptr[col], cnt = P.colour_ptr[col + 1] - p0;
     if (!cnt) continue;
+    gm_prof_begin(c, GM_K_ILU_FACT);
     if (prec == GM_PREC_MIXED)
       k_ilu_factor<float><<<cnt, 64, 0, c->stream>>>(P.d_rowptr, P.d_col, P.d_diag, P.d_lu32,
                                                      P.d_patch_row, p0, c->d_flag);
     else
       k_ilu_factor<double><<<cnt, 64, 0, c->stream>>>(P.d_rowptr, P.d_col, P.d_diag, P.d_lu,
                                                       P.d_patch_row, p0, c->d_flag);
+    gm_prof_end(c, GM_K_ILU_FACT);
   }
-  gm_prof_end(c, GM_K_ILU_FACT);
   GM_HIP(hipGetLastError());
   int flag = 0;
   GM_HIP(hipMemcpyAsync(&flag, c->d_flag + 2, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -472,20 +472,22 @@ int gm_ilu_factor(gm_ctx *c, const double *val, const float *val32, int prec) {
 template <typename VT, typename RT, typename ZT>
 static void ilu_apply(gm_ctx *c, const VT *lu, VT *ywork, const RT *r, ZT *z) {
   IluPlan &P = c->ilu;
-  gm_prof_begin(c, GM_K_ILU_APPLY);
   for (int col = 0; col < P.ncol; col++) {
     const int p0 = P.colour_ptr[col], cnt = P.colour_ptr[col + 1] - p0;
-    if (cnt)
-      k_ilu_lower<VT, RT, VT><<<cnt, 64, 0, c->stream>>>(P.d_rowptr, P.d_col, P.d_diag, lu,
-                                                         P.d_patch_row, p0, P.d_perm, r, ywork);
+    if (!cnt) continue;
+    gm_prof_begin(c, GM_K_ILU_LOWER);
+    k_ilu_lower<VT, RT, VT><<<cnt, 64, 0, c->stream>>>(P.d_rowptr, P.d_col, P.d_diag, lu,
+                                                       P.d_patch_row, p0, P.d_perm, r, ywork);
+    gm_prof_end(c, GM_K_ILU_LOWER);
   }
   for (int col = P.ncol - 1; col >= 0; col--) {
     const int p0 = P.colour_ptr[col], cnt = P.colour_ptr[col + 1] - p0;
-    if (cnt)
-      k_ilu_upper<VT, VT, ZT><<<cnt, 64, 0, c->stream>>>(P.d_rowptr, P.d_col, P.d_diag, lu,
-                                                         P.d_patch_row, p0, P.d_perm, ywork, z);
+    if (!cnt) continue;
+    gm_prof_begin(c, GM_K_ILU_UPPER);
+    k_ilu_upper<VT, VT, ZT><<<cnt, 64, 0, c->stream>>>(P.d_rowptr, P.d_col, P.d_diag, lu,
+                                                       P.d_patch_row, p0, P.d_perm, ywork, z);
+    gm_prof_end(c, GM_K_ILU_UPPER);
   }
-  gm_prof_end(c, GM_K_ILU_APPLY);
 }
 
 // ---------------------------------------------------------------- Krylov kernels
@@ -667,13 +669,16 @@ static int gmres_t(gm_ctx *c, const VT *a, const VT *lu, VT *ywork, const double
       BT *vj = V + (size_t)j * N, *vn = V + (size_t)(j + 1) * N;
       precond(vj, z);
       spmv<VT, BT, BT>(c, a, z, w);
-      gm_prof_begin(c, GM_K_ORTHO);
+      gm_prof_begin(c, GM_K_MDOT);
       k_mdot<BT><<<KB, RED_THREADS, 0, c->stream>>>(N, j + 1, V, w, K.d_part);
+      gm_prof_end(c, GM_K_MDOT);
       GM_HIP(hipGetLastError());
       if ((rc = sum_parts(c, K.d_part, j + 2, KB, h.data()))) return rc;
       const double ww0 = h[j + 1];
       GM_HIP(hipMemcpyAsync(K.d_h, h.data(), (j + 1) * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      gm_prof_begin(c, GM_K_ORTH);
       k_orth_update<BT><<<KB, RED_THREADS, 0, c->stream>>>(N, j + 1, V, K.d_h, w, K.d_part);
+      gm_prof_end(c, GM_K_ORTH);
       double ww;
       if ((rc = sum_parts(c, K.d_part, 1, KB, &ww))) return rc;
       if (ww < eta2 * ww0) {
@@ -684,7 +689,6 @@ static int gmres_t(gm_ctx *c, const VT *a, const VT *lu, VT *ywork, const double
         if ((rc = sum_parts(c, K.d_part, 1, KB, &ww))) return rc;
         for (int k = 0; k <= j; k++) h[k] += h2[k];
       }
-      gm_prof_end(c, GM_K_ORTHO);
       const double hn = std::sqrt(ww);
       double *hj = H.data() + (size_t)j * (m + 1);
       for (int k = 0; k <= j; k++) hj[k] = h[k];

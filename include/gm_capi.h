@@ -161,14 +161,17 @@ int gm_solve_nonlinear(gm_ctx *ctx, double *x, const gm_solver_opts *opt, gm_new
 
 void gm_default_solver_opts(gm_solver_opts *opt);
 
-/* Kernel timing: when enabled, every launch of the named kernel family is
- * bracketed by HIP events on the context's stream.  ids: */
-#define GM_K_FILL 0      /* element assembly + scatter, all colours of one fill */
-#define GM_K_SPMV 1      /* MSR matvec */
-#define GM_K_ILU_APPLY 2 /* both triangular sweeps of one ILU(0) application */
-#define GM_K_ILU_FACT 3  /* one ILU(0) factorisation */
-#define GM_K_ORTHO 4     /* one Gram-Schmidt step */
-#define GM_K_COUNT 5
+/* Kernel timing: when enabled, every launch of the kernels below is
+ * bracketed by HIP events on the context's stream (the stream the kernels
+ * run on); gm_prof_get returns the launch count and the summed duration. */
+#define GM_K_FILL 0       /* k_fill: assembly + scatter of one element colour */
+#define GM_K_SPMV 1       /* k_spmv_msr: MSR matvec */
+#define GM_K_ILU_LOWER 2  /* k_ilu_lower: forward sweep of one patch colour */
+#define GM_K_ILU_UPPER 3  /* k_ilu_upper: backward sweep of one patch colour */
+#define GM_K_ILU_FACT 4   /* k_ilu_factor: ILU(0) of one patch colour */
+#define GM_K_MDOT 5       /* k_mdot: Gram-Schmidt projections */
+#define GM_K_ORTH 6       /* k_orth_update: Gram-Schmidt update */
+#define GM_K_COUNT 7
 int gm_prof_enable(gm_ctx *ctx, int on);
 int gm_prof_get(gm_ctx *ctx, int kernel_id, int *count, double *total_ms);
 

@@ -50,6 +50,12 @@ struct IluPlan {
   float *d_lu32 = nullptr;    // [nnz] (mixed precision)
   double *d_y = nullptr;      // [N] work (permuted numbering)
   float *d_y32 = nullptr;
+  int *d_node_first = nullptr, *d_node_last = nullptr;  // [N] node row range of each row
+  int levels = 0;             // total forward levels over all patches
+  // per-patch level schedules, [0] forward / factorisation, [1] backward
+  int *d_lv_ptr[2] = {nullptr, nullptr}, *d_lv_nstart[2] = {nullptr, nullptr};
+  int *d_lv_node[2] = {nullptr, nullptr}, *d_lv_rstart[2] = {nullptr, nullptr};
+  int *d_lv_row[2] = {nullptr, nullptr};
 };
 
 struct KrylovWork {

@@ -226,117 +226,189 @@ int gm_spmv_launch(gm_ctx *c, const double *val, const double *x, double *y) {
 
 // ---------------------------------------------------------------- ILU(0)
 // ILU(0) of P A P^T where P is the patch ordering.  Patches of one colour
-// share no couplings, so each colour is one launch with one wavefront per
-// patch; inside a patch rows are processed in order (IKJ variant).
+// share no couplings, so each colour is one launch with one workgroup per
+// patch.  Inside a patch the rows keep their sequential (IKJ) order of
+// dependence; the host groups the patch's nodes into levels (a node depends
+// on every earlier node of the patch its rows reference in L, or later node in
+// U), and the workgroup walks the levels: all rows of a level first take the
+// contributions from outside their own node in parallel, then one thread per
+// node resolves the node's small diagonal block.  The arithmetic per entry is
+// the sequential algorithm's; only the summation order inside a row differs.
 
-static constexpr int ILU_MAXPATCH = 4096;  // rows of one patch held in LDS by the sweeps
+static constexpr int ILU_MAXPATCH = 8192;  // rows of one patch held in LDS by the sweeps
+static constexpr int ILU_NT = 256;         // threads per patch workgroup (sweeps)
+static constexpr int ILU_FT = 512;         // threads per patch workgroup (factorisation)
 
-template <typename VT>
-__global__ __launch_bounds__(64) void k_ilu_factor(const int *__restrict__ rowptr,
-                                                   const int *__restrict__ col,
-                                                   const int *__restrict__ diag, VT *lu,
-                                                   const int *__restrict__ patch_row, int patch0,
-                                                   int *flag) {
-  __shared__ int s_col[GM_MAXROW];
-  __shared__ VT s_val[GM_MAXROW];
-  const int P = patch0 + blockIdx.x;
-  const int r0 = patch_row[P], r1 = patch_row[P + 1];
-  const int lane = threadIdx.x;
-  for (int r = r0; r < r1; r++) {
-    const int b = rowptr[r], len = rowptr[r + 1] - b, dpos = diag[r] - b;
-    for (int t = lane; t < len; t += 64) {
-      s_col[t] = col[b + t];
-      s_val[t] = lu[b + t];
-    }
-    __syncthreads();
-    for (int kk = 0; kk < dpos; kk++) {
-      const int k = s_col[kk];
-      const int dk = diag[k];
-      const VT piv = lu[dk];
-      const VT l = s_val[kk] / piv;
-      __syncthreads();
-      if (lane == 0) s_val[kk] = l;
-      const int ke = rowptr[k + 1];
-      for (int jj = dk + 1 + lane; jj < ke; jj += 64) {
-        const int j = col[jj];
-        int lo = kk + 1, hi = len - 1, pos = -1;
-        while (lo <= hi) {
-          const int mid = (lo + hi) >> 1;
-          const int v = s_col[mid];
-          if (v == j) {
-            pos = mid;
-            break;
-          }
-          if (v < j) lo = mid + 1;
-          else hi = mid - 1;
-        }
-        if (pos >= 0) s_val[pos] -= l * lu[jj];
-      }
-      __syncthreads();
-    }
-    if (lane == 0 && s_val[dpos] == (VT)0) atomicOr(flag + 2, 1);
-    for (int t = lane; t < len; t += 64) lu[b + t] = s_val[t];
-    __syncthreads();
-  }
+struct IluLv {
+  const int *rowptr, *col, *diag, *node_first, *node_last, *patch_row;
+  const int *lv_ptr, *lv_nstart, *lv_node, *lv_rstart, *lv_row;
+};
+
+template <int G, typename T>
+__device__ inline T group_sum(T v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+  return v;
 }
 
-// forward sweep  y = L^-1 (P b),  L unit lower
-template <typename VT, typename BT, typename WT>
-__global__ __launch_bounds__(64) void k_ilu_lower(const int *__restrict__ rowptr,
-                                                  const int *__restrict__ col,
-                                                  const int *__restrict__ diag,
-                                                  const VT *__restrict__ lu,
-                                                  const int *__restrict__ patch_row, int patch0,
-                                                  const int *__restrict__ perm,
-                                                  const BT *__restrict__ b, WT *y) {
+// forward sweep  y = L^-1 (P b),  L unit lower; y in permuted numbering
+template <typename VT, typename BT, typename WT, int G>
+__global__ __launch_bounds__(ILU_NT) void k_ilu_lower(IluLv S, const VT *__restrict__ lu,
+                                                      int patch0, const int *__restrict__ perm,
+                                                      const BT *__restrict__ b, WT *y) {
   __shared__ WT s_y[ILU_MAXPATCH];
+  constexpr int NG = ILU_NT / G;
   const int P = patch0 + blockIdx.x;
-  const int r0 = patch_row[P], r1 = patch_row[P + 1];
-  const int lane = threadIdx.x;
-  for (int r = r0; r < r1; r++) {
-    WT s = 0;
-    const int e = diag[r];
-    for (int t = rowptr[r] + lane; t < e; t += 64) {
-      const int cc = col[t];
-      const WT yc = cc >= r0 ? s_y[cc - r0] : y[cc];
-      s += (WT)lu[t] * yc;
+  const int r0 = S.patch_row[P], r1 = S.patch_row[P + 1];
+  const int tid = threadIdx.x, lane = tid % G, grp = tid / G;
+  for (int l = S.lv_ptr[P]; l < S.lv_ptr[P + 1]; l++) {
+    const int ra = S.lv_rstart[l], rb = S.lv_rstart[l + 1];
+    for (int k = ra + grp; k < rb; k += NG) {
+      const int r = S.lv_row[k];
+      const int nf = S.node_first[r];
+      WT s = 0;
+      const int e = S.diag[r];
+      for (int t = S.rowptr[r] + lane; t < e; t += G) {
+        const int cc = S.col[t];
+        if (cc < nf) s += (WT)lu[t] * (cc >= r0 ? s_y[cc - r0] : y[cc]);
+      }
+      s = group_sum<G>(s);
+      if (lane == 0) s_y[r - r0] = (WT)b[perm[r]] - s;
     }
-    s = wave_sum(s);
-    if (lane == 0) s_y[r - r0] = (WT)b[perm[r]] - s;
+    __syncthreads();
+    for (int k = S.lv_nstart[l] + tid; k < S.lv_nstart[l + 1]; k += ILU_NT) {
+      const int nf = S.lv_node[k];
+      const int nl = S.node_last[nf];
+      for (int r = nf + 1; r <= nl; r++) {
+        WT s = s_y[r - r0];
+        for (int t = S.diag[r] - 1; t >= S.rowptr[r]; t--) {
+          const int cc = S.col[t];
+          if (cc < nf) break;
+          s -= (WT)lu[t] * s_y[cc - r0];
+        }
+        s_y[r - r0] = s;
+      }
+    }
     __syncthreads();
   }
-  for (int r = r0 + lane; r < r1; r += 64) y[r] = s_y[r - r0];
+  for (int r = r0 + tid; r < r1; r += ILU_NT) y[r] = s_y[r - r0];
 }
 
 // backward sweep x = U^-1 y in place on y (permuted), plus the unpermuted copy
-template <typename VT, typename WT, typename OT>
-__global__ __launch_bounds__(64) void k_ilu_upper(const int *__restrict__ rowptr,
-                                                  const int *__restrict__ col,
-                                                  const int *__restrict__ diag,
-                                                  const VT *__restrict__ lu,
-                                                  const int *__restrict__ patch_row, int patch0,
-                                                  const int *__restrict__ perm, WT *y,
-                                                  OT *__restrict__ out) {
+template <typename VT, typename WT, typename OT, int G>
+__global__ __launch_bounds__(ILU_NT) void k_ilu_upper(IluLv S, const VT *__restrict__ lu,
+                                                      int patch0, const int *__restrict__ perm,
+                                                      WT *y, OT *__restrict__ out) {
   __shared__ WT s_x[ILU_MAXPATCH];
+  constexpr int NG = ILU_NT / G;
   const int P = patch0 + blockIdx.x;
-  const int r0 = patch_row[P], r1 = patch_row[P + 1];
-  const int lane = threadIdx.x;
-  for (int r = r1 - 1; r >= r0; r--) {
-    WT s = 0;
-    const int d = diag[r];
-    for (int t = d + 1 + lane; t < rowptr[r + 1]; t += 64) {
-      const int cc = col[t];
-      const WT xc = cc < r1 ? s_x[cc - r0] : y[cc];
-      s += (WT)lu[t] * xc;
+  const int r0 = S.patch_row[P], r1 = S.patch_row[P + 1];
+  const int tid = threadIdx.x, lane = tid % G, grp = tid / G;
+  for (int l = S.lv_ptr[P]; l < S.lv_ptr[P + 1]; l++) {
+    const int ra = S.lv_rstart[l], rb = S.lv_rstart[l + 1];
+    for (int k = ra + grp; k < rb; k += NG) {
+      const int r = S.lv_row[k];
+      const int nl = S.node_last[S.node_first[r]];
+      WT s = 0;
+      const int e = S.rowptr[r + 1];
+      for (int t = S.diag[r] + 1 + lane; t < e; t += G) {
+        const int cc = S.col[t];
+        if (cc > nl) s += (WT)lu[t] * (cc < r1 ? s_x[cc - r0] : y[cc]);
+      }
+      s = group_sum<G>(s);
+      if (lane == 0) s_x[r - r0] = y[r] - s;
     }
-    s = wave_sum(s);
-    if (lane == 0) s_x[r - r0] = (y[r] - s) / (WT)lu[d];
+    __syncthreads();
+    for (int k = S.lv_nstart[l] + tid; k < S.lv_nstart[l + 1]; k += ILU_NT) {
+      const int nf = S.lv_node[k];
+      const int nl = S.node_last[nf];
+      for (int r = nl; r >= nf; r--) {
+        WT s = s_x[r - r0];
+        const int d = S.diag[r];
+        for (int t = d + 1; t < S.rowptr[r + 1]; t++) {
+          const int cc = S.col[t];
+          if (cc > nl) break;
+          s -= (WT)lu[t] * s_x[cc - r0];
+        }
+        s_x[r - r0] = s / (WT)lu[d];
+      }
+    }
     __syncthreads();
   }
-  for (int r = r0 + lane; r < r1; r += 64) {
+  for (int r = r0 + tid; r < r1; r += ILU_NT) {
     const WT v = s_x[r - r0];
     y[r] = v;
     out[perm[r]] = (OT)v;
+  }
+}
+
+// IKJ elimination of row r by the pivot rows whose column lies in [kb, ke),
+// one wavefront per row, the row staged in LDS.
+template <typename VT>
+__device__ inline void ilu_row_eliminate(const IluLv &S, VT *lu, int r, int kb, int ke, int *s_col,
+                                         VT *s_val, int *flag) {
+  const int lane = threadIdx.x & 63;
+  const int b = S.rowptr[r], len = S.rowptr[r + 1] - b, dpos = S.diag[r] - b;
+  for (int t = lane; t < len; t += 64) {
+    s_col[t] = S.col[b + t];
+    s_val[t] = lu[b + t];
+  }
+  __builtin_amdgcn_wave_barrier();
+  __threadfence_block();
+  for (int kk = 0; kk < dpos; kk++) {
+    const int k = s_col[kk];
+    if (k < kb) continue;
+    if (k >= ke) break;
+    const int dk = S.diag[k];
+    const VT l = s_val[kk] / lu[dk];
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) s_val[kk] = l;
+    const int kend = S.rowptr[k + 1];
+    for (int jj = dk + 1 + lane; jj < kend; jj += 64) {
+      const int j = S.col[jj];
+      int lo = kk + 1, hi = len - 1, pos = -1;
+      while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = s_col[mid];
+        if (v == j) {
+          pos = mid;
+          break;
+        }
+        if (v < j) lo = mid + 1;
+        else hi = mid - 1;
+      }
+      if (pos >= 0) s_val[pos] -= l * lu[jj];
+    }
+    __builtin_amdgcn_wave_barrier();
+    __threadfence_block();
+  }
+  for (int t = lane; t < len; t += 64) lu[b + t] = s_val[t];
+  if (lane == 0 && ke > r && s_val[dpos] == (VT)0) atomicOr(flag + 2, 1);
+  __threadfence_block();
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <typename VT>
+__global__ __launch_bounds__(ILU_FT) void k_ilu_factor(IluLv S, VT *lu, int patch0, int *flag) {
+  constexpr int NW = ILU_FT / 64;
+  __shared__ int s_col[NW][GM_MAXROW];
+  __shared__ VT s_val[NW][GM_MAXROW];
+  const int P = patch0 + blockIdx.x;
+  const int w = threadIdx.x >> 6;
+  for (int l = S.lv_ptr[P]; l < S.lv_ptr[P + 1]; l++) {
+    // rows of the level against pivot rows outside their own node
+    for (int k = S.lv_rstart[l] + w; k < S.lv_rstart[l + 1]; k += NW) {
+      const int r = S.lv_row[k];
+      ilu_row_eliminate<VT>(S, lu, r, 0, S.node_first[r], s_col[w], s_val[w], flag);
+    }
+    __syncthreads();
+    // each node's rows against the node's earlier rows, in order
+    for (int k = S.lv_nstart[l] + w; k < S.lv_nstart[l + 1]; k += NW) {
+      const int nf = S.lv_node[k];
+      const int nl = S.node_last[nf];
+      for (int r = nf; r <= nl; r++) ilu_row_eliminate<VT>(S, lu, r, nf, r + 1, s_col[w], s_val[w], flag);
+    }
+    __syncthreads();
   }
 }
 
@@ -349,22 +421,47 @@ __global__ void k_gather(long long n, const int *__restrict__ src, const VT *__r
     lu[i] = (DT)a[src[i]];
 }
 
+static IluLv lv_args(const IluPlan &P, bool upper) {
+  IluLv S;
+  S.rowptr = P.d_rowptr;
+  S.col = P.d_col;
+  S.diag = P.d_diag;
+  S.node_first = P.d_node_first;
+  S.node_last = P.d_node_last;
+  S.patch_row = P.d_patch_row;
+  const int u = upper ? 1 : 0;
+  S.lv_ptr = P.d_lv_ptr[u];
+  S.lv_nstart = P.d_lv_nstart[u];
+  S.lv_node = P.d_lv_node[u];
+  S.lv_rstart = P.d_lv_rstart[u];
+  S.lv_row = P.d_lv_row[u];
+  return S;
+}
+
+static void free_plan(IluPlan &P) {
+  void *ptrs[] = {P.d_perm, P.d_rowptr, P.d_col, P.d_diag, P.d_src, P.d_patch_row, P.d_lu,
+                  P.d_lu32, P.d_y, P.d_y32, P.d_node_first, P.d_node_last};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  for (int u = 0; u < 2; u++) {
+    void *q[] = {P.d_lv_ptr[u], P.d_lv_nstart[u], P.d_lv_node[u], P.d_lv_rstart[u], P.d_lv_row[u]};
+    for (void *p : q)
+      if (p) (void)hipFree(p);
+  }
+  P = IluPlan();
+}
+
+template <typename T>
+static int up(T **d, const std::vector<T> &h) {
+  GM_HIP(hipMalloc(d, std::max<size_t>(h.size(), 1) * sizeof(T)));
+  if (!h.empty()) GM_HIP(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return GM_OK;
+}
+
 int gm_ilu_plan(gm_ctx *c, int bins) {
   IluPlan &P = c->ilu;
   if (P.bins == bins) return GM_OK;
-  if (P.d_perm) {
-    hipFree(P.d_perm);
-    hipFree(P.d_rowptr);
-    hipFree(P.d_col);
-    hipFree(P.d_diag);
-    hipFree(P.d_src);
-    hipFree(P.d_patch_row);
-    hipFree(P.d_lu);
-    hipFree(P.d_y);
-    if (P.d_lu32) hipFree(P.d_lu32);
-    if (P.d_y32) hipFree(P.d_y32);
-    P = IluPlan();
-  }
+  if (P.d_perm) free_plan(P);
   std::vector<int> cptr;
   int ncol = gm_patch_order(c, bins, P.perm, cptr, P.patch_row);
   if (ncol < 0) return ncol;
@@ -412,21 +509,87 @@ int gm_ilu_plan(gm_ctx *c, int bins) {
       gm_set_error("Jacobian row longer than %d", GM_MAXROW);
       return GM_ERR_ARG;
     }
-  GM_HIP(hipMalloc(&P.d_perm, N * sizeof(int)));
-  GM_HIP(hipMalloc(&P.d_rowptr, (N + 1) * sizeof(int)));
-  GM_HIP(hipMalloc(&P.d_col, nnz * sizeof(int)));
-  GM_HIP(hipMalloc(&P.d_diag, N * sizeof(int)));
-  GM_HIP(hipMalloc(&P.d_src, nnz * sizeof(int)));
-  GM_HIP(hipMalloc(&P.d_patch_row, (P.npatch + 1) * sizeof(int)));
+  // nodes in permuted numbering: the unknowns of a mesh node stay contiguous
+  std::vector<int> node_first(N), node_last(N);
+  {
+    std::vector<int> node_of(N);
+    for (int n = 0; n < c->num_nodes; n++)
+      for (int u = c->first_unknown[n]; u < c->first_unknown[n + 1]; u++) node_of[u] = n;
+    int r = 0;
+    while (r < N) {
+      const int n = node_of[P.perm[r]];
+      const int cnt = c->first_unknown[n + 1] - c->first_unknown[n];
+      for (int k = 0; k < cnt; k++) {
+        node_first[r + k] = r;
+        node_last[r + k] = r + cnt - 1;
+      }
+      r += cnt;
+    }
+  }
+  // level schedules per patch (0: forward / factorisation, 1: backward)
+  std::vector<int> lv_ptr[2], lv_nstart[2], lv_node[2], lv_rstart[2], lv_row[2];
+  {
+    std::vector<int> level(N, 0);
+    for (int u = 0; u < 2; u++) {
+      lv_ptr[u].assign(1, 0);
+      lv_nstart[u].assign(1, 0);
+      lv_rstart[u].assign(1, 0);
+      for (int p = 0; p < P.npatch; p++) {
+        const int r0 = P.patch_row[p], r1 = P.patch_row[p + 1];
+        std::vector<int> nodes;
+        for (int r = r0; r < r1; r = node_last[r] + 1) nodes.push_back(r);
+        int maxlev = -1;
+        if (u == 0) {
+          for (int nf : nodes) {
+            int lv = 0;
+            for (int r = nf; r <= node_last[nf]; r++)
+              for (int t = rowptr[r]; t < diag[r]; t++) {
+                const int cc = col[t];
+                if (cc >= r0 && cc < nf) lv = std::max(lv, level[node_first[cc]] + 1);
+              }
+            level[nf] = lv;
+            maxlev = std::max(maxlev, lv);
+          }
+        } else {
+          for (auto it = nodes.rbegin(); it != nodes.rend(); ++it) {
+            const int nf = *it, nl = node_last[nf];
+            int lv = 0;
+            for (int r = nf; r <= nl; r++)
+              for (int t = diag[r] + 1; t < rowptr[r + 1]; t++) {
+                const int cc = col[t];
+                if (cc > nl && cc < r1) lv = std::max(lv, level[node_first[cc]] + 1);
+              }
+            level[nf] = lv;
+            maxlev = std::max(maxlev, lv);
+          }
+        }
+        std::vector<std::vector<int>> byl(maxlev + 1);
+        for (int nf : nodes) byl[level[nf]].push_back(nf);
+        for (int lv = 0; lv <= maxlev; lv++) {
+          for (int nf : byl[lv]) {
+            lv_node[u].push_back(nf);
+            for (int r = nf; r <= node_last[nf]; r++) lv_row[u].push_back(r);
+          }
+          lv_nstart[u].push_back((int)lv_node[u].size());
+          lv_rstart[u].push_back((int)lv_row[u].size());
+        }
+        lv_ptr[u].push_back((int)lv_nstart[u].size() - 1);
+      }
+    }
+  }
+  P.levels = (int)lv_ptr[0].back();
+  int rc;
+  if ((rc = up(&P.d_perm, P.perm)) || (rc = up(&P.d_rowptr, rowptr)) || (rc = up(&P.d_col, col)) ||
+      (rc = up(&P.d_diag, diag)) || (rc = up(&P.d_src, src)) || (rc = up(&P.d_patch_row, P.patch_row)) ||
+      (rc = up(&P.d_node_first, node_first)) || (rc = up(&P.d_node_last, node_last)))
+    return rc;
+  for (int u = 0; u < 2; u++)
+    if ((rc = up(&P.d_lv_ptr[u], lv_ptr[u])) || (rc = up(&P.d_lv_nstart[u], lv_nstart[u])) ||
+        (rc = up(&P.d_lv_node[u], lv_node[u])) || (rc = up(&P.d_lv_rstart[u], lv_rstart[u])) ||
+        (rc = up(&P.d_lv_row[u], lv_row[u])))
+      return rc;
   GM_HIP(hipMalloc(&P.d_lu, nnz * sizeof(double)));
   GM_HIP(hipMalloc(&P.d_y, N * sizeof(double)));
-  GM_HIP(hipMemcpy(P.d_perm, P.perm.data(), N * sizeof(int), hipMemcpyHostToDevice));
-  GM_HIP(hipMemcpy(P.d_rowptr, rowptr.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice));
-  GM_HIP(hipMemcpy(P.d_col, col.data(), nnz * sizeof(int), hipMemcpyHostToDevice));
-  GM_HIP(hipMemcpy(P.d_diag, diag.data(), N * sizeof(int), hipMemcpyHostToDevice));
-  GM_HIP(hipMemcpy(P.d_src, src.data(), nnz * sizeof(int), hipMemcpyHostToDevice));
-  GM_HIP(hipMemcpy(P.d_patch_row, P.patch_row.data(), (P.npatch + 1) * sizeof(int),
-                   hipMemcpyHostToDevice));
   P.bins = bins;
   return GM_OK;
 }
@@ -445,16 +608,15 @@ int gm_ilu_factor(gm_ctx *c, const double *val, const float *val32, int prec) {
   } else {
     k_gather<double, double><<<4096, 256, 0, c->stream>>>(P.nnz, P.d_src, val, P.d_lu);
   }
+  const IluLv S = lv_args(P, false);
   for (int col = 0; col < P.ncol; col++) {
     const int p0 = P.colour_ptr[col], cnt = P.colour_ptr[col + 1] - p0;
     if (!cnt) continue;
     gm_prof_begin(c, GM_K_ILU_FACT);
     if (prec == GM_PREC_MIXED)
-      k_ilu_factor<float><<<cnt, 64, 0, c->stream>>>(P.d_rowptr, P.d_col, P.d_diag, P.d_lu32,
-                                                     P.d_patch_row, p0, c->d_flag);
+      k_ilu_factor<float><<<cnt, ILU_FT, 0, c->stream>>>(S, P.d_lu32, p0, c->d_flag);
     else
-      k_ilu_factor<double><<<cnt, 64, 0, c->stream>>>(P.d_rowptr, P.d_col, P.d_diag, P.d_lu,
-                                                      P.d_patch_row, p0, c->d_flag);
+      k_ilu_factor<double><<<cnt, ILU_FT, 0, c->stream>>>(S, P.d_lu, p0, c->d_flag);
     gm_prof_end(c, GM_K_ILU_FACT);
   }
   GM_HIP(hipGetLastError());
@@ -472,20 +634,25 @@ int gm_ilu_factor(gm_ctx *c, const double *val, const float *val32, int prec) {
 template <typename VT, typename RT, typename ZT>
 static void ilu_apply(gm_ctx *c, const VT *lu, VT *ywork, const RT *r, ZT *z) {
   IluPlan &P = c->ilu;
+  const IluLv SL = lv_args(P, false), SU = lv_args(P, true);
   for (int col = 0; col < P.ncol; col++) {
     const int p0 = P.colour_ptr[col], cnt = P.colour_ptr[col + 1] - p0;
     if (!cnt) continue;
     gm_prof_begin(c, GM_K_ILU_LOWER);
-    k_ilu_lower<VT, RT, VT><<<cnt, 64, 0, c->stream>>>(P.d_rowptr, P.d_col, P.d_diag, lu,
-                                                       P.d_patch_row, p0, P.d_perm, r, ywork);
+    if (c->dim == 2)
+      k_ilu_lower<VT, RT, VT, 16><<<cnt, ILU_NT, 0, c->stream>>>(SL, lu, p0, P.d_perm, r, ywork);
+    else
+      k_ilu_lower<VT, RT, VT, 64><<<cnt, ILU_NT, 0, c->stream>>>(SL, lu, p0, P.d_perm, r, ywork);
     gm_prof_end(c, GM_K_ILU_LOWER);
   }
   for (int col = P.ncol - 1; col >= 0; col--) {
     const int p0 = P.colour_ptr[col], cnt = P.colour_ptr[col + 1] - p0;
     if (!cnt) continue;
     gm_prof_begin(c, GM_K_ILU_UPPER);
-    k_ilu_upper<VT, VT, ZT><<<cnt, 64, 0, c->stream>>>(P.d_rowptr, P.d_col, P.d_diag, lu,
-                                                       P.d_patch_row, p0, P.d_perm, ywork, z);
+    if (c->dim == 2)
+      k_ilu_upper<VT, VT, ZT, 16><<<cnt, ILU_NT, 0, c->stream>>>(SU, lu, p0, P.d_perm, ywork, z);
+    else
+      k_ilu_upper<VT, VT, ZT, 64><<<cnt, ILU_NT, 0, c->stream>>>(SU, lu, p0, P.d_perm, ywork, z);
     gm_prof_end(c, GM_K_ILU_UPPER);
   }
 }
@@ -766,13 +933,11 @@ int gm_gmres_run_nofactor(gm_ctx *c, const double *val, const float *val32, cons
 }
 
 void gm_free_linalg(gm_ctx *c) {
-  IluPlan &P = c->ilu;
-  void *ptrs[] = {P.d_perm, P.d_rowptr, P.d_col, P.d_diag, P.d_src, P.d_patch_row, P.d_lu,
-                  P.d_lu32, P.d_y, P.d_y32, c->kry.d_V, c->kry.d_w, c->kry.d_z, c->kry.d_part,
-                  c->kry.d_h, c->kry.d_V32, c->kry.d_w32, c->kry.d_z32};
+  free_plan(c->ilu);
+  KrylovWork &K = c->kry;
+  void *ptrs[] = {K.d_V, K.d_w, K.d_z, K.d_part, K.d_h, K.d_V32, K.d_w32, K.d_z32};
   for (void *p : ptrs)
-    if (p) hipFree(p);
-  P = IluPlan();
+    if (p) (void)hipFree(p);
   c->kry = KrylovWork();
 }
 

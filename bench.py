@@ -133,6 +133,10 @@ def main(argv=None):
     ap.add_argument("--kspace", type=int, default=50)
     ap.add_argument("--max-its", type=int, default=500)
     ap.add_argument("--patch-bins", type=int, default=0)
+    ap.add_argument("--precond", choices=["mg", "ilu"], default="mg")
+    ap.add_argument("--mg-nu", type=int, default=1)
+    ap.add_argument("--mg-omega", type=float, default=1.0)
+    ap.add_argument("--mg-coarse", type=int, default=2, help="coarsest elements per side")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
@@ -154,11 +158,15 @@ def main(argv=None):
     ctx = capi.Context(prob)
     t_setup = time.perf_counter() - t0
     bins = args.patch_bins or 0
-    print(f"[bench] {prob.name}: N={ctx.N} nnz={ctx.nval - 1} setup {t_setup:.2f}s", file=sys.stderr,
-          flush=True)
+    pc = 2 if args.precond == "mg" else 1
+    nlev = ctx.attach_multigrid(args.mg_coarse) if pc == 2 else 0
+    t_setup = time.perf_counter() - t0
+    print(f"[bench] {prob.name}: N={ctx.N} nnz={ctx.nval - 1} levels {nlev} setup {t_setup:.2f}s",
+          file=sys.stderr, flush=True)
     x = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
     opts = dict(kspace=args.kspace, max_its=args.max_its, patch_bins=bins, precision=prec,
-                max_newton=40, verbose=1 if args.verbose else 0)
+                max_newton=40, verbose=1 if args.verbose else 0, precond=pc, mg_nu=args.mg_nu,
+                mg_omega=args.mg_omega)
     for w in range(args.warmup):
         x.zero_()
         torch.cuda.synchronize()
@@ -178,7 +186,7 @@ def main(argv=None):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     prof = {nm: ctx.prof_get(i) for i, nm in enumerate(
-        ["fill", "spmv", "ilu_lower", "ilu_upper", "ilu_factor", "mdot", "orth"])}
+        ["fill", "spmv", "ilu_lower", "ilu_upper", "ilu_factor", "mdot", "orth", "mg_vcycle"])}
     ctx.prof_enable(False)
     if not all(r["converged"] for r in reps):
         print("[bench] WARNING: a step did not converge", file=sys.stderr, flush=True)
@@ -226,7 +234,7 @@ def main(argv=None):
         "data": "synthetic",
         "config": {"workload": f"{prob.name} ({args.workload}) " + ", ".join(f"{k}={v}" for k, v in params.items()),
                    "unknowns": ctx.N, "nnz": ctx.nval - 1, "elements": prob.mesh.num_elems,
-                   "precision": args.precision, "solver": f"GMRES({args.kspace}) + ILU(0) patch order",
+                   "precision": args.precision, "solver": f"GMRES({args.kspace}) + " + ("multigrid V(%d,%d) element-block smoothing, %d levels" % (args.mg_nu, args.mg_nu, nlev) if pc == 2 else "ILU(0) patch order"),
                    "initial_guess": "zero"},
         "newton_its": newton,
         "gmres_its": gm_its,

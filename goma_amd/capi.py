@@ -27,8 +27,9 @@ GM_PREC_FP64 = 0
 GM_PREC_MIXED = 1
 GM_PC_NONE = 0
 GM_PC_ILU0 = 1
-K_FILL, K_SPMV, K_ILU_LOWER, K_ILU_UPPER, K_ILU_FACT, K_MDOT, K_ORTH = range(7)
-KERNEL_NAMES = ["fill", "spmv", "ilu_lower", "ilu_upper", "ilu_factor", "mdot", "orth"]
+GM_PC_MG = 2
+K_FILL, K_SPMV, K_ILU_LOWER, K_ILU_UPPER, K_ILU_FACT, K_MDOT, K_ORTH, K_MG = range(8)
+KERNEL_NAMES = ["fill", "spmv", "ilu_lower", "ilu_upper", "ilu_factor", "mdot", "orth", "mg_vcycle"]
 
 
 class GmMesh(C.Structure):
@@ -53,7 +54,8 @@ class GmSolverOpts(C.Structure):
     _fields_ = [("max_newton", C.c_int), ("eps_resid", C.c_double), ("eps_corr", C.c_double),
                 ("damp", C.c_double), ("kspace", C.c_int), ("max_its", C.c_int),
                 ("lin_tol", C.c_double), ("precond", C.c_int), ("patch_bins", C.c_int),
-                ("precision", C.c_int), ("verbose", C.c_int)]
+                ("precision", C.c_int), ("verbose", C.c_int), ("mg_nu", C.c_int),
+                ("mg_omega", C.c_double), ("mg_coarse_sweeps", C.c_int)]
 
 
 MAX_REPORT = 64
@@ -84,7 +86,8 @@ EXPORTS = [
     "gm_copy_elem_colours", "gm_copy_patch_permutation", "gm_set_physics",
     "gm_matrix_fill_full_dev", "gm_matrix_fill_full", "gm_row_sum_scale_dev", "gm_msr_matvec_dev",
     "gm_gmres_dev", "gm_ilu0_factor_dev", "gm_solve_nonlinear_dev", "gm_solve_nonlinear",
-    "gm_default_solver_opts", "gm_prof_enable", "gm_prof_get",
+    "gm_default_solver_opts", "gm_prof_enable", "gm_prof_get", "gm_mg_add_level",
+    "gm_mg_num_levels", "gm_mg_setup_dev",
 ]
 
 _lib = None
@@ -123,6 +126,9 @@ def lib():
     L.gm_solve_nonlinear.argtypes = [vp, vp, C.POINTER(GmSolverOpts), C.POINTER(GmNewtonReport)]
     L.gm_default_solver_opts.argtypes = [C.POINTER(GmSolverOpts)]
     L.gm_default_solver_opts.restype = None
+    L.gm_mg_add_level.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.gm_mg_num_levels.argtypes = [vp]
+    L.gm_mg_setup_dev.argtypes = [vp, vp, vp, vp]
     L.gm_prof_enable.argtypes = [vp, ip]
     L.gm_prof_get.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(C.c_double)]
     _lib = L
@@ -182,6 +188,9 @@ class Context:
         if getattr(self, "h", None):
             lib().gm_ctx_destroy(self.h)
             self.h = None
+        for cc in getattr(self, "_coarse", []):
+            cc.close()
+        self._coarse = []
 
     def __del__(self):
         try:
@@ -267,6 +276,28 @@ class Context:
         rc = lib().gm_solve_nonlinear(self.h, x.ctypes.data, C.byref(o), C.byref(rep))
         _check(rc, "gm_solve_nonlinear")
         return rc, rep
+
+    # ---- multigrid hierarchy
+    def attach_multigrid(self, min_elems_per_side: int = 2):
+        """Builds coarse contexts down to ``min_elems_per_side`` elements per
+        side and attaches them with their Q2/P1 prolongations."""
+        from . import multigrid
+
+        self._coarse = []
+        above = self
+        for cp in multigrid.hierarchy(self.prob, min_elems_per_side):
+            cc = Context(cp)
+            rp, ci, cv, inj = multigrid.prolongation(above.first_unknown(), cc.first_unknown(),
+                                                     above.prob.mesh, cp.mesh, cp.phys.energy)
+            self._mg_arrays = getattr(self, "_mg_arrays", []) + [(rp, ci, cv, inj)]
+            _check(lib().gm_mg_add_level(self.h, cc.h, rp.ctypes.data, ci.ctypes.data,
+                                         cv.ctypes.data, inj.ctypes.data), "gm_mg_add_level")
+            self._coarse.append(cc)
+            above = cc
+        return lib().gm_mg_num_levels(self.h)
+
+    def mg_setup_dev(self, x_ptr, val_ptr, scale_ptr=None):
+        _check(lib().gm_mg_setup_dev(self.h, x_ptr, val_ptr, scale_ptr), "gm_mg_setup_dev")
 
     # ---- kernel timing
     def prof_enable(self, on: bool = True):

@@ -70,11 +70,15 @@ extern "C" void gm_default_solver_opts(gm_solver_opts *o) {
   o->patch_bins = 0;
   o->precision = GM_PREC_FP64;
   o->verbose = 0;
+  o->mg_nu = 1;
+  o->mg_omega = 1.0;
+  o->mg_coarse_sweeps = 20;
 }
 
 static void free_ctx(gm_ctx *c) {
   if (!c) return;
   gm_free_linalg(c);
+  gm_mg_free(c);
   void *ptrs[] = {c->d_coor, c->d_conn,   c->d_fu,    c->d_centroid, c->d_dbc,   c->d_dbc_val,
                   c->d_bindx, c->d_ecol_elems, c->d_e2nz, c->d_flag, c->d_tables, c->d_val,
                   c->d_resid, c->d_dx,    c->d_scale, c->d_val32,    c->d_red,   c->d_xwork};
@@ -361,14 +365,15 @@ extern "C" int gm_solve_nonlinear_dev(gm_ctx *c, double *x, const gm_solver_opts
     if (!converged) {
       int its = 0;
       double rr = 0.0;
-      if (o->precond != GM_PC_NONE) {
+      gm_solver_opts oo = *o;
+      // preconditioner set-up timed separately, then GMRES
+      if (o->precond == GM_PC_ILU0) {
         const int bins = o->patch_bins > 0 ? o->patch_bins : gm_default_bins(c);
         if ((rc = gm_ilu_plan(c, bins))) return rc;
-      }
-      gm_solver_opts oo = *o;
-      // factorisation timed separately, then GMRES without re-factoring
-      if (o->precond != GM_PC_NONE) {
         if ((rc = gm_ilu_factor(c, c->d_val, c->d_val32, prec))) return rc;
+      } else if (o->precond == GM_PC_MG) {
+        if ((rc = gm_mg_setup(c, x, c->d_val, c->d_val32, prec, c->d_scale))) return rc;
+        GM_HIP(hipStreamSynchronize(c->stream));
       }
       double td = now_s();
       rep->t_precond += td - tc;

@@ -70,6 +70,21 @@ struct KrylovWork {
   float *d_w32 = nullptr, *d_z32 = nullptr;
 };
 
+struct gm_ctx;
+struct MgLevel {
+  gm_ctx *ctx = nullptr;  // the level's context (not owned)
+  // transfer between this level and the next finer one: P (fine x coarse), R = P^T
+  int *d_Prow = nullptr, *d_Pcol = nullptr, *d_Rrow = nullptr, *d_Rcol = nullptr;
+  double *d_Pval = nullptr, *d_Rval = nullptr;
+  int *d_inject = nullptr;    // coarse node -> finer node at the same position
+  double *d_state = nullptr;  // injected state for re-assembly
+  double *d_rdummy = nullptr; // residual written by the coarse assembly (unused)
+  void *d_x = nullptr, *d_b = nullptr, *d_r = nullptr;  // level vectors
+  void *d_val = nullptr;      // coarse Jacobian (MSR)
+  void *d_ainv = nullptr;     // element-block inverses
+  int vec_prec = -1, ainv_prec = -1, val_prec = -1;
+};
+
 struct gm_ctx {
   int dim = 0, nve = 0, np = 0, nv = 0, num_nodes = 0, num_elems = 0, energy = 0;
   gm_physics phys{};
@@ -108,6 +123,9 @@ struct gm_ctx {
 
   IluPlan ilu;
   KrylovWork kry;
+  std::vector<MgLevel> mg;  // coarse levels, finest first
+  MgLevel mg0;              // level-0 vectors and element inverses
+  const double *mg_scale = nullptr;  // row-sum scale of the level-0 matrix
 
   bool prof = false;
   long long prof_count[GM_K_COUNT] = {0};
@@ -141,6 +159,13 @@ int gm_gmres_run(gm_ctx *c, const double *val, const float *val32, const double 
 int gm_gmres_run_nofactor(gm_ctx *c, const double *val, const float *val32, const double *b,
                           double *x, const gm_solver_opts *o, int *its, double *relres);
 void gm_free_linalg(gm_ctx *c);
+
+// multigrid (gm_mg.hip)
+int gm_mg_setup(gm_ctx *c, const double *x, const double *val, const float *val32, int prec,
+                const double *scale);
+int gm_mg_apply_d(gm_ctx *c, const double *a0, const double *v, double *z, const gm_solver_opts *o);
+int gm_mg_apply_f(gm_ctx *c, const float *a0, const float *v, float *z, const gm_solver_opts *o);
+void gm_mg_free(gm_ctx *c);
 
 // timing helpers
 void gm_prof_begin(gm_ctx *c, int id);

@@ -805,8 +805,14 @@ static int gmres_t(gm_ctx *c, const VT *a, const VT *lu, VT *ywork, const double
   const int grid = 2048;
 
   auto precond = [&](const BT *in, BT *out) {
-    if (pc) ilu_apply<VT, BT, BT>(c, lu, ywork, in, out);
-    else k_scale_into<BT><<<grid, 256, 0, c->stream>>>(N, in, 1.0, out);
+    if (o->precond == GM_PC_MG) {
+      if constexpr (sizeof(BT) == 8) gm_mg_apply_d(c, (const double *)a, (const double *)in, (double *)out, o);
+      else gm_mg_apply_f(c, (const float *)a, (const float *)in, (float *)out, o);
+    } else if (pc) {
+      ilu_apply<VT, BT, BT>(c, lu, ywork, in, out);
+    } else {
+      k_scale_into<BT><<<grid, 256, 0, c->stream>>>(N, in, 1.0, out);
+    }
   };
 
   // r0 = b - A x
@@ -909,10 +915,13 @@ int gm_gmres_run(gm_ctx *c, const double *val, const float *val32, const double 
   const int prec = o->precision;
   int rc = ensure_krylov(c, o->kspace, prec);
   if (rc) return rc;
-  if (o->precond != GM_PC_NONE) {
+  if (o->precond == GM_PC_ILU0) {
     int bins = o->patch_bins > 0 ? o->patch_bins : gm_default_bins(c);
     if ((rc = gm_ilu_plan(c, bins))) return rc;
     if ((rc = gm_ilu_factor(c, val, val32, prec))) return rc;
+  } else if (o->precond == GM_PC_MG && c->mg0.ainv_prec != prec) {
+    gm_set_error("multigrid preconditioner not set up (gm_mg_setup_dev)");
+    return GM_ERR_ARG;
   }
   return gm_gmres_run_nofactor(c, val, val32, b, x, o, its, relres);
 }

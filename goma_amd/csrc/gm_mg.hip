@@ -1,0 +1,494 @@
+// Geometric multigrid preconditioner with element-block (Vanka) smoothing.
+//
+// Levels: level 0 is the context's own mesh and (row-sum-scaled) Jacobian;
+// coarser levels are contexts on coarsened meshes whose Jacobians are
+// re-assembled by the same fill kernels at the injected state.  Transfers are
+// CSR matrices P (coarse -> fine) and R = P^T supplied by the caller.
+//
+// Smoother: for each element colour, r = b - A x on every row, then every
+// element of the colour solves its local system (all velocity, temperature
+// and pressure unknowns of the element, entries taken from the assembled
+// matrix through the element-to-nonzero map) and adds omega * A_e^-1 r_e to x.
+// Elements of one colour share no unknowns, and the residual is formed before
+// the colour's updates, so the sweep is deterministic (block Gauss-Seidel
+// across colours, block Jacobi within a colour).  The local inverses are
+// formed once per Newton step by Gauss-Jordan elimination with partial
+// pivoting in LDS.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "gm_internal.h"
+
+template <int DIM, int EN>
+struct Loc {
+  static constexpr int NVE = DIM == 2 ? 9 : 27;
+  static constexpr int NP = DIM + 1;
+  static constexpr int NV = DIM + EN;
+  static constexpr int N = NV * NVE + NP;
+};
+
+struct LevelDev {
+  const int *conn, *fu, *centroid, *e2nz, *bindx;
+  const signed char *dbc;
+  long long E, off_mp, off_cv, off_cp;
+};
+
+template <int DIM, int EN>
+__device__ inline int local_dof(const LevelDev &L, int e, int li) {
+  using Lc = Loc<DIM, EN>;
+  if (li < Lc::NV * Lc::NVE) {
+    const int a = li / Lc::NVE, i = li - a * Lc::NVE;
+    return L.fu[L.conn[(size_t)e * Lc::NVE + i]] + a;
+  }
+  return L.fu[L.centroid[e]] + DIM + EN + (li - Lc::NV * Lc::NVE);
+}
+
+// e2nz position of local entry (li, lj), -1 when the pair is not in the graph
+template <int DIM, int EN>
+__device__ inline long long local_pos(const LevelDev &L, int li, int lj) {
+  using Lc = Loc<DIM, EN>;
+  constexpr int NVE = Lc::NVE, NP = Lc::NP, NV = Lc::NV, NN = NV * NVE;
+  if (li < NN) {
+    const int a = li / NVE, i = li - a * NVE;
+    if (lj < NN) {
+      const int b = lj / NVE, j = lj - b * NVE;
+      return (long long)(a * NV + b) * NVE * NVE + i * NVE + j;
+    }
+    if (a >= DIM) return -1;  // energy row x pressure column: not in the graph
+    return L.off_mp + (long long)a * NVE * NP + i * NP + (lj - NN);
+  }
+  const int k = li - NN;
+  if (lj < NN) {
+    const int b = lj / NVE, j = lj - b * NVE;
+    if (b >= DIM) return -1;  // continuity x temperature: identically zero
+    return L.off_cv + (long long)b * NP * NVE + k * NVE + j;
+  }
+  return L.off_cp + (long long)k * NP + (lj - NN);
+}
+
+// local matrix -> inverse (stored transposed: ainv[e][j][i] = (A_e^-1)[i][j])
+template <int DIM, int EN, typename VT, typename AT, int NT>
+__global__ __launch_bounds__(NT) void k_vanka_invert(LevelDev L, const VT *__restrict__ val,
+                                                     AT *__restrict__ ainv) {
+  using Lc = Loc<DIM, EN>;
+  constexpr int n = Lc::N;
+  __shared__ double A[n][n + 1];
+  __shared__ int s_piv[n];
+  __shared__ double s_f[n];
+  __shared__ double s_d;
+  __shared__ int s_p;
+  const int e = blockIdx.x, tid = threadIdx.x;
+  const int *map = L.e2nz + (size_t)e * L.E;
+  for (int idx = tid; idx < n * n; idx += NT) {
+    const int i = idx / n, j = idx - i * n;
+    const long long p = local_pos<DIM, EN>(L, i, j);
+    A[i][j] = p >= 0 ? (double)val[map[p]] : 0.0;
+  }
+  __syncthreads();
+  for (int k = 0; k < n; k++) {
+    if (tid == 0) {
+      int p = k;
+      double m = fabs(A[k][k]);
+      for (int i = k + 1; i < n; i++)
+        if (fabs(A[i][k]) > m) {
+          m = fabs(A[i][k]);
+          p = i;
+        }
+      s_p = p;
+      s_piv[k] = p;
+    }
+    __syncthreads();
+    const int p = s_p;
+    if (p != k)
+      for (int j = tid; j < n; j += NT) {
+        const double t = A[k][j];
+        A[k][j] = A[p][j];
+        A[p][j] = t;
+      }
+    __syncthreads();
+    if (tid == 0) {
+      const double piv = A[k][k];
+      // a structurally empty column (a local pressure mode with no
+      // velocity coupling inside the element) leaves that unknown untouched
+      s_d = fabs(piv) > 1e-300 ? 1.0 / piv : 0.0;
+    }
+    __syncthreads();
+    const double d = s_d;
+    for (int j = tid; j < n; j += NT) A[k][j] = (j == k) ? d : A[k][j] * d;
+    for (int i = tid; i < n; i += NT) s_f[i] = A[i][k];
+    __syncthreads();
+    for (int idx = tid; idx < n * n; idx += NT) {
+      const int i = idx / n, j = idx - i * n;
+      if (i == k) continue;
+      const double f = s_f[i];
+      if (j == k) A[i][j] = -f * d;
+      else A[i][j] -= f * A[k][j];
+    }
+    __syncthreads();
+  }
+  // undo the row interchanges as column interchanges, in reverse order
+  for (int k = n - 1; k >= 0; k--) {
+    const int p = s_piv[k];
+    if (p != k)
+      for (int i = tid; i < n; i += NT) {
+        const double t = A[i][k];
+        A[i][k] = A[i][p];
+        A[i][p] = t;
+      }
+    __syncthreads();
+  }
+  AT *out = ainv + (size_t)e * n * n;
+  for (int idx = tid; idx < n * n; idx += NT) {
+    const int j = idx / n, i = idx - j * n;
+    out[idx] = (AT)A[i][j];
+  }
+}
+
+// r = b - A x over all rows (MSR), G lanes per row
+template <typename VT, typename BT, int G>
+__global__ __launch_bounds__(256) void k_mg_resid(int N, const VT *__restrict__ a,
+                                                  const int *__restrict__ bindx,
+                                                  const BT *__restrict__ x,
+                                                  const BT *__restrict__ b, BT *__restrict__ r) {
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const int lane = threadIdx.x & (G - 1);
+  if (row >= N) return;
+  using AT = typename std::conditional<sizeof(BT) == 8, double, float>::type;
+  const int k0 = bindx[row], k1 = bindx[row + 1];
+  AT s = 0;
+  for (int k = k0 + lane; k < k1; k += G) s += (AT)a[k] * (AT)x[bindx[k]];
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
+  if (lane == 0) r[row] = (BT)((AT)b[row] - s - (AT)a[row] * (AT)x[row]);
+}
+
+// x_e += omega * A_e^-1 r_e for the elements of one colour
+template <int DIM, int EN, typename AT, typename BT, int NT>
+__global__ __launch_bounds__(NT) void k_vanka_update(LevelDev L, const int *__restrict__ elems,
+                                                     const AT *__restrict__ ainv,
+                                                     const BT *__restrict__ r, BT *__restrict__ x,
+                                                     double omega) {
+  using Lc = Loc<DIM, EN>;
+  constexpr int n = Lc::N;
+  __shared__ double s_r[n];
+  __shared__ int s_g[n];
+  const int e = elems[blockIdx.x], tid = threadIdx.x;
+  for (int i = tid; i < n; i += NT) {
+    const int g = local_dof<DIM, EN>(L, e, i);
+    s_g[i] = g;
+    s_r[i] = (double)r[g];
+  }
+  __syncthreads();
+  const AT *M = ainv + (size_t)e * n * n;
+  for (int i = tid; i < n; i += NT) {
+    double s = 0.0;
+    for (int j = 0; j < n; j++) s += (double)M[j * n + i] * s_r[j];
+    x[s_g[i]] += (BT)(omega * s);
+  }
+}
+
+// y = M v (CSR), optionally scaling v by sc and masking Dirichlet rows
+template <typename BT>
+__global__ void k_csr_apply(int nrows, const int *__restrict__ rp, const int *__restrict__ ci,
+                            const double *__restrict__ cv, const BT *__restrict__ v,
+                            const double *__restrict__ sc, const signed char *__restrict__ mask,
+                            BT *__restrict__ y, int accumulate) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nrows; i += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; k++) {
+      const int j = ci[k];
+      s += cv[k] * (double)v[j] * (sc ? sc[j] : 1.0);
+    }
+    if (mask && mask[i]) s = 0.0;
+    if (accumulate) y[i] += (BT)s;
+    else y[i] = (BT)s;
+  }
+}
+
+// coarse state: nodal values injected from the finer level, pressure 0
+__global__ void k_inject(int ncn, const int *__restrict__ inj, const int *__restrict__ fu_f,
+                         const int *__restrict__ fu_c, int nv, const double *__restrict__ xf,
+                         double *__restrict__ xc) {
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < ncn; n += gridDim.x * blockDim.x) {
+    const int f = fu_f[inj[n]], cc = fu_c[n], cnt = fu_c[n + 1] - cc;
+    for (int a = 0; a < cnt; a++) xc[cc + a] = a < nv ? xf[f + a] : 0.0;
+  }
+}
+
+template <typename T>
+__global__ void k_set(T *p, long long n, T v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// ---------------------------------------------------------------- host side
+
+static LevelDev level_dev(const gm_ctx *c) {
+  LevelDev L;
+  L.conn = c->d_conn;
+  L.fu = c->d_fu;
+  L.centroid = c->d_centroid;
+  L.e2nz = c->d_e2nz;
+  L.bindx = c->d_bindx;
+  L.dbc = c->d_dbc;
+  L.E = c->lay.total;
+  L.off_mp = c->lay.off_mp;
+  L.off_cv = c->lay.off_cv;
+  L.off_cp = c->lay.off_cp;
+  return L;
+}
+
+static int nloc_of(const gm_ctx *c) { return c->nv * c->nve + c->np; }
+
+template <typename T>
+static int upv(T **d, const T *h, size_t n) {
+  GM_HIP(hipMalloc(d, std::max<size_t>(n, 1) * sizeof(T)));
+  if (n) GM_HIP(hipMemcpy(*d, h, n * sizeof(T), hipMemcpyHostToDevice));
+  return GM_OK;
+}
+
+extern "C" int gm_mg_add_level(gm_ctx *fine, gm_ctx *coarse, const int *P_rowptr, const int *P_col,
+                               const double *P_val, const int *inject) {
+  if (!fine || !coarse || !P_rowptr || !P_col || !P_val || !inject) return GM_ERR_ARG;
+  gm_ctx *above = fine->mg.empty() ? fine : fine->mg.back().ctx;
+  if (coarse->dim != fine->dim || coarse->energy != fine->energy) {
+    gm_set_error("coarse level has a different dimension or variable set");
+    return GM_ERR_ARG;
+  }
+  MgLevel L;
+  L.ctx = coarse;
+  const int Nf = above->N, Nc = coarse->N;
+  const int nnz = P_rowptr[Nf];
+  for (int k = 0; k < nnz; k++)
+    if (P_col[k] < 0 || P_col[k] >= Nc) {
+      gm_set_error("prolongation column %d outside the coarse level", P_col[k]);
+      return GM_ERR_ARG;
+    }
+  for (int n = 0; n < coarse->num_nodes; n++)
+    if (inject[n] < 0 || inject[n] >= above->num_nodes) {
+      gm_set_error("injection map entry outside the finer level");
+      return GM_ERR_ARG;
+    }
+  // R = P^T
+  std::vector<int> rrow(Nc + 1, 0), rcol(nnz);
+  std::vector<double> rval(nnz);
+  for (int k = 0; k < nnz; k++) rrow[P_col[k] + 1]++;
+  for (int i = 0; i < Nc; i++) rrow[i + 1] += rrow[i];
+  std::vector<int> fp(rrow.begin(), rrow.end() - 1);
+  for (int i = 0; i < Nf; i++)
+    for (int k = P_rowptr[i]; k < P_rowptr[i + 1]; k++) {
+      const int q = fp[P_col[k]]++;
+      rcol[q] = i;
+      rval[q] = P_val[k];
+    }
+  int rc;
+  if ((rc = upv(&L.d_Prow, P_rowptr, Nf + 1)) || (rc = upv(&L.d_Pcol, P_col, nnz)) ||
+      (rc = upv(&L.d_Pval, P_val, nnz)) || (rc = upv(&L.d_Rrow, rrow.data(), Nc + 1)) ||
+      (rc = upv(&L.d_Rcol, rcol.data(), nnz)) || (rc = upv(&L.d_Rval, rval.data(), nnz)) ||
+      (rc = upv(&L.d_inject, inject, coarse->num_nodes)))
+    return rc;
+  GM_HIP(hipMalloc(&L.d_state, Nc * sizeof(double)));
+  GM_HIP(hipMalloc(&L.d_rdummy, Nc * sizeof(double)));
+  fine->mg.push_back(L);
+  return GM_OK;
+}
+
+extern "C" int gm_mg_num_levels(const gm_ctx *c) { return c ? (int)c->mg.size() + 1 : GM_ERR_ARG; }
+
+static int alloc_level_vectors(MgLevel &L, const gm_ctx *lc, int prec, bool coarse) {
+  const size_t N = lc->N;
+  const size_t es = prec == GM_PREC_MIXED ? sizeof(float) : sizeof(double);
+  if (L.vec_prec != prec) {
+    if (L.d_x) (void)hipFree(L.d_x);
+    if (L.d_b) (void)hipFree(L.d_b);
+    if (L.d_r) (void)hipFree(L.d_r);
+    GM_HIP(hipMalloc(&L.d_x, N * es));
+    GM_HIP(hipMalloc(&L.d_b, N * es));
+    GM_HIP(hipMalloc(&L.d_r, N * es));
+    L.vec_prec = prec;
+  }
+  const size_t nl = nloc_of(lc);
+  if (L.ainv_prec != prec) {
+    if (L.d_ainv) (void)hipFree(L.d_ainv);
+    GM_HIP(hipMalloc(&L.d_ainv, (size_t)lc->num_elems * nl * nl * es));
+    L.ainv_prec = prec;
+  }
+  if (coarse && L.val_prec != prec) {
+    if (L.d_val) (void)hipFree(L.d_val);
+    GM_HIP(hipMalloc(&L.d_val, (size_t)lc->nval * es));
+    L.val_prec = prec;
+  }
+  return GM_OK;
+}
+
+template <int DIM, int EN, typename VT, typename AT>
+static void launch_invert(gm_ctx *lc, hipStream_t s, const VT *val, AT *ainv) {
+  constexpr int NT = DIM == 2 ? 128 : 256;
+  k_vanka_invert<DIM, EN, VT, AT, NT><<<lc->num_elems, NT, 0, s>>>(level_dev(lc), val, ainv);
+}
+
+template <typename VT>
+static void invert(gm_ctx *lc, hipStream_t s, const VT *val, VT *ainv) {
+  if (lc->dim == 2) {
+    if (lc->energy) launch_invert<2, 1, VT, VT>(lc, s, val, ainv);
+    else launch_invert<2, 0, VT, VT>(lc, s, val, ainv);
+  } else {
+    if (lc->energy) launch_invert<3, 1, VT, VT>(lc, s, val, ainv);
+    else launch_invert<3, 0, VT, VT>(lc, s, val, ainv);
+  }
+}
+
+// Per Newton step: coarse Jacobians at the injected state, local inverses on
+// every level.  val / val32 is the level-0 (scaled) matrix, x the state.
+int gm_mg_setup(gm_ctx *c, const double *x, const double *val, const float *val32, int prec,
+                const double *scale) {
+  c->mg_scale = scale;
+  if (c->mg.empty()) {
+    gm_set_error("multigrid preconditioner requested but no coarse level attached");
+    return GM_ERR_ARG;
+  }
+  int rc;
+  if ((rc = alloc_level_vectors(c->mg0, c, prec, false))) return rc;
+  const double *xs = x;
+  gm_ctx *above = c;
+  for (MgLevel &L : c->mg) {
+    gm_ctx *lc = L.ctx;
+    if ((rc = alloc_level_vectors(L, lc, prec, true))) return rc;
+    k_inject<<<1024, 256, 0, c->stream>>>(lc->num_nodes, L.d_inject, above->d_fu, lc->d_fu, lc->nv,
+                                          xs, L.d_state);
+    GM_HIP(hipGetLastError());
+    GM_HIP(hipStreamSynchronize(c->stream));
+    k_set<double><<<256, 256, 0, lc->stream>>>(L.d_rdummy, lc->N, 0.0);
+    if (prec == GM_PREC_MIXED) {
+      k_set<float><<<2048, 256, 0, lc->stream>>>((float *)L.d_val, lc->nval, 0.0f);
+      rc = gm_fill_launch(lc, L.d_state, L.d_rdummy, nullptr, (float *)L.d_val);
+    } else {
+      k_set<double><<<2048, 256, 0, lc->stream>>>((double *)L.d_val, lc->nval, 0.0);
+      rc = gm_fill_launch(lc, L.d_state, L.d_rdummy, (double *)L.d_val, nullptr);
+    }
+    if (rc) return rc;
+    xs = L.d_state;
+    above = lc;
+  }
+  if (prec == GM_PREC_MIXED) {
+    invert<float>(c, c->stream, val32, (float *)c->mg0.d_ainv);
+    for (MgLevel &L : c->mg) invert<float>(L.ctx, c->stream, (const float *)L.d_val, (float *)L.d_ainv);
+  } else {
+    invert<double>(c, c->stream, val, (double *)c->mg0.d_ainv);
+    for (MgLevel &L : c->mg) invert<double>(L.ctx, c->stream, (const double *)L.d_val, (double *)L.d_ainv);
+  }
+  GM_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+template <typename VT, typename BT>
+static void resid(gm_ctx *lc, hipStream_t s, const VT *a, const BT *x, const BT *b, BT *r) {
+  constexpr int G = 16;
+  const int blocks = (int)(((long long)lc->N * G + 255) / 256);
+  k_mg_resid<VT, BT, G><<<blocks, 256, 0, s>>>(lc->N, a, lc->d_bindx, x, b, r);
+}
+
+template <int DIM, int EN, typename AT, typename BT>
+static void vanka_colour(gm_ctx *lc, hipStream_t s, int col, const AT *ainv, const BT *r, BT *x,
+                         double omega) {
+  constexpr int NT = DIM == 2 ? 64 : 128;
+  const int cnt = lc->ecol_ptr[col + 1] - lc->ecol_ptr[col];
+  if (cnt)
+    k_vanka_update<DIM, EN, AT, BT, NT><<<cnt, NT, 0, s>>>(level_dev(lc), lc->d_ecol_elems + lc->ecol_ptr[col],
+                                                           ainv, r, x, omega);
+}
+
+template <typename VT, typename BT>
+static void smooth(gm_ctx *lc, hipStream_t s, const VT *a, const VT *ainv, const BT *b, BT *x, BT *r,
+                   int sweeps, double omega) {
+  for (int it = 0; it < sweeps; it++)
+    for (int col = 0; col < lc->n_ecol; col++) {
+      resid<VT, BT>(lc, s, a, x, b, r);
+      if (lc->dim == 2) {
+        if (lc->energy) vanka_colour<2, 1, VT, BT>(lc, s, col, ainv, r, x, omega);
+        else vanka_colour<2, 0, VT, BT>(lc, s, col, ainv, r, x, omega);
+      } else {
+        if (lc->energy) vanka_colour<3, 1, VT, BT>(lc, s, col, ainv, r, x, omega);
+        else vanka_colour<3, 0, VT, BT>(lc, s, col, ainv, r, x, omega);
+      }
+    }
+}
+
+// z = M^-1 v: one V-cycle.  VT: matrix / inverse type, BT: vector type.
+template <typename VT, typename BT>
+static void vcycle(gm_ctx *c, const VT *a0, const BT *v, BT *z, const gm_solver_opts *o) {
+  hipStream_t s = c->stream;
+  const int nl = (int)c->mg.size();
+  const int nu = o->mg_nu > 0 ? o->mg_nu : 1;
+  const double om = o->mg_omega > 0 ? o->mg_omega : 1.0;
+  const int cs = o->mg_coarse_sweeps > 0 ? o->mg_coarse_sweeps : 20;
+  auto lv_ctx = [&](int l) { return l == 0 ? c : c->mg[l - 1].ctx; };
+  auto lv = [&](int l) -> MgLevel & { return l == 0 ? c->mg0 : c->mg[l - 1]; };
+  auto amat = [&](int l) -> const VT * { return l == 0 ? a0 : (const VT *)c->mg[l - 1].d_val; };
+  // level 0 works on the caller's vectors
+  BT *x0 = z;
+  const BT *b0 = v;
+  for (int l = 0; l <= nl; l++) {
+    gm_ctx *lc = lv_ctx(l);
+    MgLevel &L = lv(l);
+    BT *x = l == 0 ? x0 : (BT *)L.d_x;
+    const BT *b = l == 0 ? b0 : (const BT *)L.d_b;
+    k_set<BT><<<1024, 256, 0, s>>>(x, lc->N, (BT)0);
+    smooth<VT, BT>(lc, s, amat(l), (const VT *)L.d_ainv, b, x, (BT *)L.d_r, l == nl ? cs : nu, om);
+    if (l == nl) break;
+    resid<VT, BT>(lc, s, amat(l), x, b, (BT *)L.d_r);
+    MgLevel &C = c->mg[l];
+    // restrict the (unscaled) residual: the level-0 matrix is row-sum scaled
+    k_csr_apply<BT><<<1024, 256, 0, s>>>(C.ctx->N, C.d_Rrow, C.d_Rcol, C.d_Rval, (const BT *)L.d_r,
+                                         l == 0 ? c->mg_scale : nullptr, C.ctx->d_dbc, (BT *)C.d_b, 0);
+  }
+  for (int l = nl - 1; l >= 0; l--) {
+    gm_ctx *lc = lv_ctx(l);
+    MgLevel &L = lv(l);
+    MgLevel &C = c->mg[l];
+    BT *x = l == 0 ? x0 : (BT *)L.d_x;
+    const BT *b = l == 0 ? b0 : (const BT *)L.d_b;
+    k_csr_apply<BT><<<1024, 256, 0, s>>>(lc->N, C.d_Prow, C.d_Pcol, C.d_Pval, (const BT *)C.d_x,
+                                         nullptr, lc->d_dbc, x, 1);
+    smooth<VT, BT>(lc, s, amat(l), (const VT *)L.d_ainv, b, x, (BT *)L.d_r, nu, om);
+  }
+}
+
+int gm_mg_apply_d(gm_ctx *c, const double *a0, const double *v, double *z, const gm_solver_opts *o) {
+  gm_prof_begin(c, GM_K_MG);
+  vcycle<double, double>(c, a0, v, z, o);
+  gm_prof_end(c, GM_K_MG);
+  GM_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+int gm_mg_apply_f(gm_ctx *c, const float *a0, const float *v, float *z, const gm_solver_opts *o) {
+  gm_prof_begin(c, GM_K_MG);
+  vcycle<float, float>(c, a0, v, z, o);
+  gm_prof_end(c, GM_K_MG);
+  GM_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+void gm_mg_free(gm_ctx *c) {
+  auto fr = [](MgLevel &L) {
+    void *p[] = {L.d_Prow, L.d_Pcol, L.d_Pval, L.d_Rrow, L.d_Rcol, L.d_Rval, L.d_inject,
+                 L.d_state, L.d_rdummy, L.d_x, L.d_b, L.d_r, L.d_val, L.d_ainv};
+    for (void *q : p)
+      if (q) (void)hipFree(q);
+  };
+  for (MgLevel &L : c->mg) fr(L);
+  fr(c->mg0);
+  c->mg.clear();
+  c->mg0 = MgLevel();
+}
+
+extern "C" int gm_mg_setup_dev(gm_ctx *c, const double *x, const double *val, const double *scale) {
+  if (!c || !x || !val) return GM_ERR_ARG;
+  int rc = gm_mg_setup(c, x, val, nullptr, GM_PREC_FP64, scale);
+  if (rc) return rc;
+  GM_HIP(hipStreamSynchronize(c->stream));
+  return GM_OK;
+}

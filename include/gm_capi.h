@@ -50,6 +50,7 @@ extern "C" {
 
 #define GM_PC_NONE 0
 #define GM_PC_ILU0 1     /* ILU(0) in patch order (see gm_solver_opts.patch_bins) */
+#define GM_PC_MG 2       /* geometric multigrid V-cycle, element-block (Vanka) smoothing */
 
 typedef struct gm_mesh {
   int dim;             /* 2: QUAD9, 3: HEX27 */
@@ -91,6 +92,9 @@ typedef struct gm_solver_opts {
   int patch_bins;      /* ILU(0) patches per axis (0: library default) */
   int precision;       /* GM_PREC_FP64 | GM_PREC_MIXED */
   int verbose;         /* 1: one line per Newton step on stderr */
+  int mg_nu;           /* multigrid: smoothing sweeps before and after the coarse correction */
+  double mg_omega;     /* multigrid: damping of the element-block updates */
+  int mg_coarse_sweeps;/* multigrid: sweeps on the coarsest level */
 } gm_solver_opts;
 
 #define GM_MAX_REPORT 64
@@ -161,6 +165,20 @@ int gm_solve_nonlinear(gm_ctx *ctx, double *x, const gm_solver_opts *opt, gm_new
 
 void gm_default_solver_opts(gm_solver_opts *opt);
 
+/* Multigrid hierarchy for GM_PC_MG (the reference has no counterpart: Goma's
+ * AztecOO path offers one-level preconditioners only).  Attaches `coarse` as
+ * the next level below the current coarsest one.  `coarse` is a context on the
+ * coarsened mesh with the same physics and the Dirichlet masks of its nodes;
+ * P (CSR: P_rowptr[N_fine+1], P_col, P_val) interpolates coarse unknowns to
+ * the finer level; inject[coarse node] is the finer node at the same place.
+ * The fine context keeps a pointer to `coarse`, which must outlive it. */
+int gm_mg_add_level(gm_ctx *fine, gm_ctx *coarse, const int *P_rowptr, const int *P_col,
+                    const double *P_val, const int *inject);
+int gm_mg_num_levels(const gm_ctx *ctx);
+/* Coarse Jacobians at the injected state x_dev and element-block inverses for
+ * the level-0 matrix val_dev (row-sum scaled by scale_dev, or NULL). */
+int gm_mg_setup_dev(gm_ctx *ctx, const double *x_dev, const double *val_dev, const double *scale_dev);
+
 /* Kernel timing: when enabled, every launch of the kernels below is
  * bracketed by HIP events on the context's stream (the stream the kernels
  * run on); gm_prof_get returns the launch count and the summed duration. */
@@ -171,7 +189,8 @@ void gm_default_solver_opts(gm_solver_opts *opt);
 #define GM_K_ILU_FACT 4   /* k_ilu_factor: ILU(0) of one patch colour */
 #define GM_K_MDOT 5       /* k_mdot: Gram-Schmidt projections */
 #define GM_K_ORTH 6       /* k_orth_update: Gram-Schmidt update */
-#define GM_K_COUNT 7
+#define GM_K_MG 7         /* one multigrid V-cycle (all its kernels) */
+#define GM_K_COUNT 8
 int gm_prof_enable(gm_ctx *ctx, int on);
 int gm_prof_get(gm_ctx *ctx, int kernel_id, int *count, double *total_ms);
 

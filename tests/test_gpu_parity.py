@@ -184,3 +184,51 @@ def test_newton_matches_oracle(prec, capi, oracle_mod):
     x = xd.cpu().numpy()
     assert np.abs(x - x_ref).max() <= 1e-9 * np.abs(x_ref).max()
     ctx.close()
+
+
+@pytest.mark.parametrize("name", ["natconv2d", "cavity3d_energy"])
+def test_multigrid_gmres_true_residual(name, capi, oracle_mod):
+    prob = (problems.natural_convection(n=8, Ra=1e4) if name == "natconv2d"
+            else problems.lid_cavity(n=4, Re=400.0, dim=3, energy=True, Ra=1e4))
+    print(f"[case] multigrid gmres {name}", flush=True)
+    ctx = capi.Context(prob)
+    nl = ctx.attach_multigrid(2)
+    assert nl >= 2
+    orc = oracle_mod.Oracle(prob)
+    x = np.random.default_rng(5).standard_normal(orc.N) * 0.3
+    rc, r, v = orc.fill(x)
+    s = np.zeros(orc.N)
+    oracle_mod.lib().om_row_sum_scale(orc.N, v, orc.ija, r, s)
+    vd, bd, xd, sd = _dev(v), _dev(r), _dev(x), _dev(s)
+    ctx.mg_setup_dev(xd.data_ptr(), vd.data_ptr(), sd.data_ptr())
+    zd = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    its, rr = ctx.gmres_dev(vd.data_ptr(), bd.data_ptr(), zd.data_ptr(), kspace=50, max_its=200,
+                            lin_tol=1e-8, precond=2)
+    A = oracle_mod.msr_to_scipy(orc.ija, v, orc.N)
+    z = zd.cpu().numpy()
+    assert np.linalg.norm(r - A @ z) <= 1.01e-8 * np.linalg.norm(r)
+    # ILU(0) on the same system needs more iterations than the multigrid cycle
+    z2 = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    its_ilu, _ = ctx.gmres_dev(vd.data_ptr(), bd.data_ptr(), z2.data_ptr(), kspace=50, max_its=500,
+                               lin_tol=1e-8, precond=1, patch_bins=2)
+    print(f"multigrid its {its}, ilu its {its_ilu}")
+    assert its < its_ilu
+    ctx.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_newton_multigrid_matches_oracle(prec, capi, oracle_mod):
+    prob = problems.natural_convection(n=8, Ra=1e4)
+    print(f"[case] newton multigrid natconv n=8 prec={prec}", flush=True)
+    ctx = capi.Context(prob)
+    ctx.attach_multigrid(2)
+    orc = oracle_mod.Oracle(prob)
+    rc, x_ref, rep_ref = orc.newton(precond=1, kspace=50, max_its=1000)
+    assert rep_ref.converged
+    xd = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    rc, rep = ctx.solve_dev(xd.data_ptr(), kspace=50, max_its=200, precond=2, precision=prec)
+    assert rep.converged
+    x = xd.cpu().numpy()
+    assert np.abs(x - x_ref).max() <= 1e-8 * np.abs(x_ref).max()
+    ctx.close()

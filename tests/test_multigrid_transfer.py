@@ -1,0 +1,60 @@
+"""Q2/P1 transfer operators of the multigrid preconditioner (CPU).
+
+Exactness pin: prolongating the coarse nodal interpolant of a field that is
+quadratic per coordinate, with a pressure linear in x, reproduces the fine
+nodal values and fine P1 coefficients exactly on an affine mesh.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from goma_amd import multigrid, problems
+
+
+def _field(o_fu, mesh, dim, energy, f, pfun):
+    x = np.zeros(int(o_fu[-1]))
+    X = mesh.coor
+    for a in range(dim + energy):
+        x[o_fu[:-1] + a] = f(X, a)
+    cl = 8 if dim == 2 else 20
+    h = 1.0 / mesh.n
+    for e in range(mesh.num_elems):
+        c = mesh.connect[e, cl]
+        b = o_fu[c] + dim + energy
+        xc = X[:, c]
+        x[b] = pfun(xc)
+        for k in range(dim):
+            dk = h / 2 * np.eye(dim)[k]
+            x[b + 1 + k] = (pfun(xc + dk) - pfun(xc - dk)) / 2
+    return x
+
+
+@pytest.mark.parametrize("dim,n,energy", [(2, 8, 1), (2, 4, 0), (3, 4, 1)])
+def test_prolongation_exact(oracle_mod, dim, n, energy):
+    p = (problems.natural_convection(n=n, Ra=1e4, dim=dim) if energy
+         else problems.lid_cavity(n=n, dim=dim))
+    hs = multigrid.hierarchy(p)
+    of, oc = oracle_mod.Oracle(p), oracle_mod.Oracle(hs[0])
+    rp, ci, cv, inj = multigrid.prolongation(of.first_unknown, oc.first_unknown, p.mesh,
+                                             hs[0].mesh, energy)
+    P = sp.csr_matrix((cv, ci, rp), shape=(of.N, oc.N))
+    f = lambda X, a: (a + 1) * X[0] ** 2 - X[1] * X[0] + 0.3 * X[-1] ** 2 + a
+    pf = lambda xc: 1.0 + 2 * xc[0] - 3 * xc[1]
+    xc = _field(oc.first_unknown, hs[0].mesh, dim, energy, f, pf)
+    xf = _field(of.first_unknown, p.mesh, dim, energy, f, pf)
+    assert np.abs(P @ xc - xf).max() < 1e-13
+    np.testing.assert_allclose(p.mesh.coor[:, inj], hs[0].mesh.coor)
+    # partition of unity of the nodal rows
+    rows = np.asarray(P.sum(axis=1)).ravel()
+    nodal = np.isin(of.unk_var, [0, 1, 2, 3])
+    np.testing.assert_allclose(rows[nodal], 1.0, atol=1e-14)
+
+
+def test_coarse_dirichlet_masks():
+    p = problems.lid_cavity(n=8)
+    hs = multigrid.hierarchy(p)
+    assert [h.mesh.n for h in hs] == [4, 2]
+    c = hs[0]
+    lat = c.mesh.lattice()
+    bnd = np.nonzero(np.any((lat == 0) | (lat == 2 * c.mesh.n), axis=1))[0]
+    assert set(np.unique(c.dbc_node)) == set(bnd)

@@ -131,11 +131,12 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=None, help="elements per side (default: workload's)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--kspace", type=int, default=50)
-    ap.add_argument("--max-its", type=int, default=500)
+    ap.add_argument("--max-its", type=int, default=200)
     ap.add_argument("--patch-bins", type=int, default=0)
     ap.add_argument("--precond", choices=["mg", "ilu"], default="mg")
     ap.add_argument("--mg-nu", type=int, default=1)
-    ap.add_argument("--mg-omega", type=float, default=1.0)
+    ap.add_argument("--mg-omega", type=float, default=0.7)
+    ap.add_argument("--line-search", type=int, default=1, help="1: Goma's backtracking line search")
     ap.add_argument("--mg-coarse", type=int, default=2, help="coarsest elements per side")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -166,7 +167,7 @@ def main(argv=None):
     x = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
     opts = dict(kspace=args.kspace, max_its=args.max_its, patch_bins=bins, precision=prec,
                 max_newton=40, verbose=1 if args.verbose else 0, precond=pc, mg_nu=args.mg_nu,
-                mg_omega=args.mg_omega)
+                mg_omega=args.mg_omega, line_search=args.line_search)
     for w in range(args.warmup):
         x.zero_()
         torch.cuda.synchronize()

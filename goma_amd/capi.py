@@ -55,7 +55,8 @@ class GmSolverOpts(C.Structure):
                 ("damp", C.c_double), ("kspace", C.c_int), ("max_its", C.c_int),
                 ("lin_tol", C.c_double), ("precond", C.c_int), ("patch_bins", C.c_int),
                 ("precision", C.c_int), ("verbose", C.c_int), ("mg_nu", C.c_int),
-                ("mg_omega", C.c_double), ("mg_coarse_sweeps", C.c_int)]
+                ("mg_omega", C.c_double), ("mg_coarse_sweeps", C.c_int), ("line_search", C.c_int),
+                ("min_damp", C.c_double)]
 
 
 MAX_REPORT = 64
@@ -65,7 +66,8 @@ class GmNewtonReport(C.Structure):
     _fields_ = [("newton_its", C.c_int), ("converged", C.c_int), ("lin_its_total", C.c_int),
                 ("lin_its", C.c_int * MAX_REPORT), ("resid_L2", C.c_double * MAX_REPORT),
                 ("resid_Loo", C.c_double * MAX_REPORT), ("unscaled_L2", C.c_double * MAX_REPORT),
-                ("lin_relres", C.c_double * MAX_REPORT), ("t_assemble", C.c_double),
+                ("lin_relres", C.c_double * MAX_REPORT), ("damp", C.c_double * MAX_REPORT),
+                ("t_assemble", C.c_double),
                 ("t_scale", C.c_double), ("t_precond", C.c_double), ("t_solve", C.c_double),
                 ("t_update", C.c_double), ("t_total", C.c_double)]
 
@@ -74,7 +76,8 @@ class GmNewtonReport(C.Structure):
         return dict(newton_its=self.newton_its, converged=bool(self.converged),
                     lin_its_total=self.lin_its_total, lin_its=list(self.lin_its[:n]),
                     resid_L2=list(self.resid_L2[:n]), resid_Loo=list(self.resid_Loo[:n]),
-                    unscaled_L2=list(self.unscaled_L2[:n]), t_assemble=self.t_assemble,
+                    unscaled_L2=list(self.unscaled_L2[:n]), damp=list(self.damp[:n]),
+                    t_assemble=self.t_assemble,
                     t_scale=self.t_scale, t_precond=self.t_precond, t_solve=self.t_solve,
                     t_update=self.t_update, t_total=self.t_total)
 

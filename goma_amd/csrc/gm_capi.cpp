@@ -71,8 +71,10 @@ extern "C" void gm_default_solver_opts(gm_solver_opts *o) {
   o->precision = GM_PREC_FP64;
   o->verbose = 0;
   o->mg_nu = 1;
-  o->mg_omega = 1.0;
+  o->mg_omega = 0.7;
   o->mg_coarse_sweeps = 20;
+  o->line_search = 0;
+  o->min_damp = 0.05;
 }
 
 static void free_ctx(gm_ctx *c) {
@@ -81,7 +83,8 @@ static void free_ctx(gm_ctx *c) {
   gm_mg_free(c);
   void *ptrs[] = {c->d_coor, c->d_conn,   c->d_fu,    c->d_centroid, c->d_dbc,   c->d_dbc_val,
                   c->d_bindx, c->d_ecol_elems, c->d_e2nz, c->d_flag, c->d_tables, c->d_val,
-                  c->d_resid, c->d_dx,    c->d_scale, c->d_val32,    c->d_red,   c->d_xwork};
+                  c->d_resid, c->d_dx,    c->d_scale, c->d_val32,    c->d_red,   c->d_xwork,
+                  c->d_xsave, c->d_rtrial, c->d_w};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->h_red) hipHostFree(c->h_red);
@@ -312,6 +315,11 @@ static int ensure_newton_buffers(gm_ctx *c, int prec) {
     GM_HIP(hipMalloc(&c->d_dx, c->N * sizeof(double)));
     GM_HIP(hipMalloc(&c->d_scale, c->N * sizeof(double)));
   }
+  if (!c->d_xsave) {
+    GM_HIP(hipMalloc(&c->d_xsave, c->N * sizeof(double)));
+    GM_HIP(hipMalloc(&c->d_rtrial, c->N * sizeof(double)));
+    GM_HIP(hipMalloc(&c->d_w, c->N * sizeof(double)));
+  }
   if (prec == GM_PREC_MIXED) {
     if (!c->d_val32) GM_HIP(hipMalloc(&c->d_val32, c->nval * sizeof(float)));
   } else {
@@ -320,7 +328,55 @@ static int ensure_newton_buffers(gm_ctx *c, int prec) {
   return GM_OK;
 }
 
-// solve_nonlinear_problem, steady, NLS_FULL_STEP (src/mm_sol_nonlinear.c:1108-2913)
+// perform_backtrack_line_search (src/mm_sol_nonlinear.c:344-590): try the
+// full step, then halve the step while the scaled residual fails
+// 0.5|R|^2 <= 0.5|R0|^2 + 0.5 slope damp, slope = -|(J dx).R(x - dx)|, down to
+// the minimum damping; keep the step with the smallest residual norm.
+// Residual-only assemblies, scaled with the current row sums.
+static int backtrack(gm_ctx *c, double *x, const gm_solver_opts *o, double L2, double *best_out) {
+  int rc;
+  const double eps0 = o->eps_resid;
+  GM_HIP(hipMemcpyAsync(c->d_xsave, x, c->N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  auto trial = [&](double damp, double *g) -> int {
+    int r;
+    if ((r = gm_lincomb(c, x, c->d_xsave, damp, c->d_dx))) return r;
+    if ((r = gm_zero(c, c->d_rtrial, c->N))) return r;
+    if ((r = gm_fill_launch_ex(c, x, c->d_rtrial, nullptr, nullptr, 1))) return r;
+    if ((r = gm_div_scale(c, c->d_rtrial, c->d_scale))) return r;
+    double loo;
+    return gm_norms(c, c->d_rtrial, c->N, g, &loo);
+  };
+  double damp = 1.0, g;
+  if ((rc = trial(damp, &g))) return rc;
+  double best_damp = damp, best_norm = g;
+  const double r_check = 0.5 * L2 * L2;
+  if (o->precision == GM_PREC_MIXED) rc = gm_spmv32_launch(c, c->d_val32, c->d_dx, c->d_w);
+  else rc = gm_spmv_launch(c, c->d_val, c->d_dx, c->d_w);
+  if (rc) return rc;
+  double slope;
+  if ((rc = gm_dot(c, c->d_w, c->d_rtrial, &slope))) return rc;
+  if (slope > 0) slope = -slope;
+  else if (slope == 0) slope = -1;
+  bool skip = best_norm < eps0;
+  while (!skip) {
+    damp *= 0.5;
+    if (damp < o->min_damp) break;
+    if ((rc = trial(damp, &g))) return rc;
+    if (std::isnan(g)) break;
+    if (g < best_norm) {
+      best_damp = damp;
+      best_norm = g;
+    }
+    if (best_norm < eps0) break;
+    if (0.5 * g * g <= r_check + 0.5 * slope * damp) break;
+  }
+  if ((rc = gm_lincomb(c, x, c->d_xsave, best_damp, c->d_dx))) return rc;
+  *best_out = best_damp;
+  return GM_OK;
+}
+
+// solve_nonlinear_problem, steady, NLS_FULL_STEP or NLS_BACKTRACK
+// (src/mm_sol_nonlinear.c:1108-2913)
 extern "C" int gm_solve_nonlinear_dev(gm_ctx *c, double *x, const gm_solver_opts *o,
                                       gm_newton_report *rep) {
   if (!c || !x || !o || !rep) return GM_ERR_ARG;
@@ -389,15 +445,22 @@ extern "C" int gm_solve_nonlinear_dev(gm_ctx *c, double *x, const gm_solver_opts
       rep->lin_its_total += its;
       double cL2r, cLoor;
       if ((rc = gm_corr_norms(c, c->d_dx, x, &cL2r, &cLoor))) return rc;
-      if ((rc = gm_axpy_update(c, x, c->d_dx, o->damp))) return rc;
+      double taken = o->damp;
+      if (o->line_search) {
+        if ((rc = backtrack(c, x, o, L2, &taken))) return rc;
+      } else if ((rc = gm_axpy_update(c, x, c->d_dx, o->damp))) {
+        return rc;
+      }
+      if (inewton < GM_MAX_REPORT) rep->damp[inewton] = taken;
       GM_HIP(hipStreamSynchronize(c->stream));
       rep->t_update += now_s() - te;
       converged = (L2 < o->eps_resid) && (Loo < o->eps_resid) && (cL2r < o->eps_corr) &&
                   (cLoor < o->eps_corr);
     }
     if (o->verbose)
-      fprintf(stderr, "[gm] newton %2d |R|2 %.3e |R|oo %.3e unscaled %.3e lin_its %d\n", inewton, L2,
-              Loo, un, inewton < GM_MAX_REPORT ? rep->lin_its[inewton] : -1);
+      fprintf(stderr, "[gm] newton %2d |R|2 %.3e |R|oo %.3e unscaled %.3e lin_its %d damp %.4f\n",
+              inewton, L2, Loo, un, inewton < GM_MAX_REPORT ? rep->lin_its[inewton] : -1,
+              inewton < GM_MAX_REPORT ? rep->damp[inewton] : 0.0);
     inewton++;
   }
   rep->newton_its = inewton;

@@ -52,6 +52,7 @@ struct FillArgs {
   void *val;
   int *flag;
   int num_nodes;
+  int resid_only;  // 1: residual only (Assemble_Jacobian = FALSE)
   long long E;
   long long off_mp, off_cv, off_cp;
   double rho, mu0, muinf, lam, nexp, aexp, Cp, kcond, hsrc, g[3], beta, Tref;
@@ -281,6 +282,8 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
     if (A.dbc[row]) R = A.x[row] - A.dbc_val[row];
     A.resid[row] += R;
   }
+
+  if (A.resid_only) return;
 
   // 5b. nodal-variable blocks, one node pair (i,j) per task
   for (int t = tid; t < NVE2; t += NT) {
@@ -587,7 +590,13 @@ static void dispatch_fill(gm_ctx *c, FillArgs &a) {
 }
 
 int gm_fill_launch(gm_ctx *c, const double *x, double *resid, double *val, float *val32) {
+  return gm_fill_launch_ex(c, x, resid, val, val32, 0);
+}
+
+int gm_fill_launch_ex(gm_ctx *c, const double *x, double *resid, double *val, float *val32,
+                      int resid_only) {
   FillArgs a;
+  a.resid_only = resid_only;
   a.coor = c->d_coor;
   a.conn = c->d_conn;
   a.fu = c->d_fu;

@@ -120,6 +120,7 @@ struct gm_ctx {
   double *d_red = nullptr;      // reduction scratch
   double *h_red = nullptr;      // pinned host mirror
   double *d_xwork = nullptr;
+  double *d_xsave = nullptr, *d_rtrial = nullptr, *d_w = nullptr;  // line search work
 
   IluPlan ilu;
   KrylovWork kry;
@@ -143,6 +144,8 @@ int gm_default_bins(const gm_ctx *c);
 // device set-up and kernels (gm_fill.hip)
 int gm_fill_setup_device(gm_ctx *c);
 int gm_fill_launch(gm_ctx *c, const double *x, double *resid, double *val, float *val32);
+int gm_fill_launch_ex(gm_ctx *c, const double *x, double *resid, double *val, float *val32,
+                      int resid_only);
 
 // linear algebra (gm_linalg.hip)
 int gm_zero(gm_ctx *c, double *p, long long n);
@@ -152,6 +155,10 @@ int gm_norms(gm_ctx *c, const double *v, long long n, double *l2, double *loo);
 int gm_corr_norms(gm_ctx *c, const double *dx, const double *x, double *l2r, double *loor);
 int gm_axpy_update(gm_ctx *c, double *x, const double *dx, double damp);
 int gm_spmv_launch(gm_ctx *c, const double *val, const double *x, double *y);
+int gm_spmv32_launch(gm_ctx *c, const float *val, const double *x, double *y);
+int gm_lincomb(gm_ctx *c, double *out, const double *a, double s, const double *b);  // out = a - s b
+int gm_div_scale(gm_ctx *c, double *v, const double *scale);                        // v /= scale
+int gm_dot(gm_ctx *c, const double *a, const double *b, double *out);
 int gm_ilu_plan(gm_ctx *c, int bins);
 int gm_ilu_factor(gm_ctx *c, const double *val, const float *val32, int prec);
 int gm_gmres_run(gm_ctx *c, const double *val, const float *val32, const double *b, double *x,

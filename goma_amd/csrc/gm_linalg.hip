@@ -224,6 +224,58 @@ int gm_spmv_launch(gm_ctx *c, const double *val, const double *x, double *y) {
   return GM_OK;
 }
 
+int gm_spmv32_launch(gm_ctx *c, const float *val, const double *x, double *y) {
+  spmv<float, double, double>(c, val, x, y);
+  GM_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+__global__ void k_lincomb(int n, double *__restrict__ out, const double *__restrict__ a, double s,
+                          const double *__restrict__ b) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[i] = a[i] - s * b[i];
+}
+
+__global__ void k_div_scale(int n, double *__restrict__ v, const double *__restrict__ sc) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    v[i] = v[i] / sc[i];
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_dot(int n, const double *__restrict__ a,
+                                                     const double *__restrict__ b,
+                                                     double *__restrict__ part) {
+  __shared__ double sh[RED_THREADS / 64];
+  double s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    s += a[i] * b[i];
+  double t = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+int gm_lincomb(gm_ctx *c, double *out, const double *a, double s, const double *b) {
+  k_lincomb<<<2048, 256, 0, c->stream>>>(c->N, out, a, s, b);
+  GM_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+int gm_div_scale(gm_ctx *c, double *v, const double *scale) {
+  k_div_scale<<<2048, 256, 0, c->stream>>>(c->N, v, scale);
+  GM_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+int gm_dot(gm_ctx *c, const double *a, const double *b, double *out) {
+  k_dot<<<RED_BLOCKS, RED_THREADS, 0, c->stream>>>(c->N, a, b, c->d_red);
+  GM_HIP(hipGetLastError());
+  GM_HIP(hipMemcpyAsync(c->h_red, c->d_red, RED_BLOCKS * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+  GM_HIP(hipStreamSynchronize(c->stream));
+  double s = 0.0;
+  for (int i = 0; i < RED_BLOCKS; i++) s += c->h_red[i];
+  *out = s;
+  return GM_OK;
+}
+
 // ---------------------------------------------------------------- ILU(0)
 // ILU(0) of P A P^T where P is the patch ordering.  Patches of one colour
 // share no couplings, so each colour is one launch with one workgroup per

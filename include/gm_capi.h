@@ -95,6 +95,8 @@ typedef struct gm_solver_opts {
   int mg_nu;           /* multigrid: smoothing sweeps before and after the coarse correction */
   double mg_omega;     /* multigrid: damping of the element-block updates */
   int mg_coarse_sweeps;/* multigrid: sweeps on the coarsest level */
+  int line_search;     /* 0: full step (NLS_FULL_STEP); 1: backtracking (NLS_BACKTRACK) */
+  double min_damp;     /* Line_Search_Minimum_Damping */
 } gm_solver_opts;
 
 #define GM_MAX_REPORT 64
@@ -107,6 +109,7 @@ typedef struct gm_newton_report {
   double resid_Loo[GM_MAX_REPORT];
   double unscaled_L2[GM_MAX_REPORT];
   double lin_relres[GM_MAX_REPORT];
+  double damp[GM_MAX_REPORT];        /* step length taken at each pass */
   double t_assemble, t_scale, t_precond, t_solve, t_update, t_total; /* seconds */
 } gm_newton_report;
 

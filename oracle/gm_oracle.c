@@ -1185,7 +1185,55 @@ int om_gmres_perm(int N, const double *val, const int *ija, const double *b, dou
 /* Newton loop                                                               */
 /* ------------------------------------------------------------------------ */
 
-/* solve_nonlinear_problem, steady, NLS_FULL_STEP
+/* perform_backtrack_line_search (src/mm_sol_nonlinear.c:344-590). */
+static double backtrack(const om_problem *p, double *x, const double *dx, const double *val,
+                        const double *scale, double L2, double eps0, double min_damp) {
+  const int N = p->N;
+  const int nv = p->ija[N];
+  double *xs = (double *)malloc(N * sizeof(double)), *R = (double *)malloc(N * sizeof(double));
+  double *w = (double *)malloc(N * sizeof(double)), *vd = (double *)malloc((size_t)nv * sizeof(double));
+  memcpy(xs, x, N * sizeof(double));
+  double damp = 1.0;
+  for (int j = 0; j < N; j++) x[j] = xs[j] - damp * dx[j];
+  memset(R, 0, N * sizeof(double));
+  om_fill(p, x, R, vd); /* vd receives a Jacobian this restatement ignores */
+  for (int j = 0; j < N; j++) R[j] = R[j] / scale[j];
+  double g = om_norm_L2(R, N);
+  double best_damp = damp, best_norm = g;
+  double r_check = 0.5 * (L2 * L2);
+  om_msr_matvec(N, val, p->ija, dx, w);
+  double slope = 0.0;
+  for (int j = 0; j < N; j++) slope += w[j] * R[j];
+  if (slope > 0) slope = -slope;
+  else if (slope == 0) slope = -1;
+  int skip = best_norm < eps0;
+  while (!skip) {
+    damp *= 0.5;
+    if (damp < min_damp) break;
+    for (int j = 0; j < N; j++) x[j] = xs[j] - damp * dx[j];
+    memset(R, 0, N * sizeof(double));
+    memset(vd, 0, (size_t)nv * sizeof(double));
+    om_fill(p, x, R, vd);
+    for (int j = 0; j < N; j++) R[j] = R[j] / scale[j];
+    g = om_norm_L2(R, N);
+    if (isnan(g)) break;
+    if (g < best_norm) {
+      best_damp = damp;
+      best_norm = g;
+    }
+    if (best_norm < eps0) break;
+    g = 0.5 * (g * g);
+    if (g <= r_check + 0.5 * slope * damp) break;
+  }
+  for (int j = 0; j < N; j++) x[j] = xs[j] - best_damp * dx[j];
+  free(xs);
+  free(R);
+  free(w);
+  free(vd);
+  return best_damp;
+}
+
+/* solve_nonlinear_problem, steady, NLS_FULL_STEP or NLS_BACKTRACK
  * (src/mm_sol_nonlinear.c:1108-2913). */
 int om_newton(const om_problem *p, double *x, const om_solver_opts *o, om_report *rep) {
   const int N = p->N;
@@ -1241,7 +1289,12 @@ int om_newton(const om_problem *p, double *x, const om_solver_opts *o, om_report
       if (inewton < 64) rep->lin_its[inewton] = its;
       rep->lin_its_total += its;
       double cL2r = om_norm_L2_r(dx, x, N), cLoor = om_norm_Loo_r(dx, x, N);
-      for (int i = 0; i < N; i++) x[i] -= o->damp * dx[i];
+      double taken = o->damp;
+      if (o->line_search)
+        taken = backtrack(p, x, dx, val, scale, L2, o->eps_resid, o->min_damp);
+      else
+        for (int i = 0; i < N; i++) x[i] -= o->damp * dx[i];
+      if (inewton < 64) rep->damp[inewton] = taken;
       converged = (L2 < o->eps_resid) && (Loo < o->eps_resid) && (cL2r < o->eps_corr) &&
                   (cLoor < o->eps_corr);
     }

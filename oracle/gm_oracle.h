@@ -83,6 +83,8 @@ typedef struct om_solver_opts {
   double lin_tol;     /* ||r||/||r0|| tolerance (AZ_r0) */
   int precond;        /* 0: none, 1: ILU(0) natural order, 2: ILU(0) node-colour order,
                          b+2 (b >= 1): ILU(0) in patch order with b patch bins per axis */
+  int line_search;    /* 0: NLS_FULL_STEP, 1: NLS_BACKTRACK */
+  double min_damp;    /* Line_Search_Minimum_Damping */
 } om_solver_opts;
 
 typedef struct om_report {
@@ -94,6 +96,7 @@ typedef struct om_report {
   double resid_Loo[64];
   double unscaled_L2[64];
   double t_assemble, t_scale, t_precond, t_solve, t_total; /* seconds */
+  double damp[64];
 } om_report;
 
 typedef struct om_problem om_problem;

@@ -40,7 +40,8 @@ class OmBC(C.Structure):
 class OmSolverOpts(C.Structure):
     _fields_ = [("max_newton", C.c_int), ("eps_resid", C.c_double), ("eps_corr", C.c_double),
                 ("damp", C.c_double), ("kspace", C.c_int), ("max_its", C.c_int),
-                ("lin_tol", C.c_double), ("precond", C.c_int)]
+                ("lin_tol", C.c_double), ("precond", C.c_int), ("line_search", C.c_int),
+                ("min_damp", C.c_double)]
 
 
 class OmReport(C.Structure):
@@ -48,7 +49,7 @@ class OmReport(C.Structure):
                 ("lin_its", C.c_int * 64), ("resid_L2", C.c_double * 64),
                 ("resid_Loo", C.c_double * 64), ("unscaled_L2", C.c_double * 64),
                 ("t_assemble", C.c_double), ("t_scale", C.c_double), ("t_precond", C.c_double),
-                ("t_solve", C.c_double), ("t_total", C.c_double)]
+                ("t_solve", C.c_double), ("t_total", C.c_double), ("damp", C.c_double * 64)]
 
 
 def build(force: bool = False) -> str:
@@ -191,9 +192,10 @@ class Oracle:
         return rc, rowptr, col, lu
 
     def newton(self, x0=None, max_newton=20, eps_resid=1e-10, eps_corr=1e10, damp=1.0,
-               kspace=30, max_its=500, lin_tol=1e-6, precond=1):
+               kspace=30, max_its=500, lin_tol=1e-6, precond=1, line_search=0, min_damp=0.05):
         x = np.zeros(self.N) if x0 is None else np.array(x0, dtype=np.float64)
-        o = OmSolverOpts(max_newton, eps_resid, eps_corr, damp, kspace, max_its, lin_tol, precond)
+        o = OmSolverOpts(max_newton, eps_resid, eps_corr, damp, kspace, max_its, lin_tol, precond,
+                         line_search, min_damp)
         rep = OmReport()
         rc = lib().om_newton(self.h, x, C.byref(o), C.byref(rep))
         return rc, x, rep

@@ -227,8 +227,32 @@ def test_newton_multigrid_matches_oracle(prec, capi, oracle_mod):
     assert rep_ref.converged
     xd = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
-    rc, rep = ctx.solve_dev(xd.data_ptr(), kspace=50, max_its=200, precond=2, precision=prec)
+    rc, rep = ctx.solve_dev(xd.data_ptr(), kspace=50, max_its=200, precond=2, precision=prec,
+                            line_search=1)
     assert rep.converged
+    x = xd.cpu().numpy()
+    assert np.abs(x - x_ref).max() <= 1e-8 * np.abs(x_ref).max()
+    ctx.close()
+
+
+def test_newton_line_search_matches_oracle(capi, oracle_mod):
+    """Backtracking Newton (NLS_BACKTRACK) at Ra = 1e5 from rest: the damping
+    sequence and the solution follow the oracle."""
+    prob = problems.natural_convection(n=8, Ra=1e5)
+    print("[case] newton line search natconv n=8 Ra=1e5", flush=True)
+    ctx = capi.Context(prob)
+    orc = oracle_mod.Oracle(prob)
+    rc, x_ref, rep_ref = orc.newton(precond=1, kspace=100, max_its=2000, lin_tol=1e-10,
+                                    line_search=1, max_newton=60)
+    assert rep_ref.converged
+    xd = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    rc, rep = ctx.solve_dev(xd.data_ptr(), kspace=100, max_its=2000, lin_tol=1e-10, patch_bins=1,
+                            line_search=1, max_newton=60)
+    assert rep.converged
+    n = rep_ref.newton_its
+    assert rep.newton_its == n
+    np.testing.assert_allclose(list(rep.damp[: n - 1]), list(rep_ref.damp[: n - 1]))
     x = xd.cpu().numpy()
     assert np.abs(x - x_ref).max() <= 1e-8 * np.abs(x_ref).max()
     ctx.close()

@@ -1,0 +1,30 @@
+"""The C ABI library loads on the CPU and exports every symbol that
+include/gm_capi.h declares (no compute calls: there is no GPU here)."""
+import os
+import re
+
+from goma_amd import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "gm_capi.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(gm_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_and_binding_agree():
+    assert _declared() == sorted(capi.EXPORTS)
+
+
+def test_library_exports_every_symbol():
+    L = capi.lib()
+    missing = [s for s in _declared() if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_default_options_are_gomas():
+    o = capi.solver_opts()
+    assert o.eps_resid == 1e-10 and o.eps_corr == 1e10 and o.min_damp == 0.05
+    assert o.lin_tol == 1e-6 and o.max_its == 500 and o.line_search == 0

@@ -28,8 +28,9 @@ GM_PREC_MIXED = 1
 GM_PC_NONE = 0
 GM_PC_ILU0 = 1
 GM_PC_MG = 2
-K_FILL, K_SPMV, K_ILU_LOWER, K_ILU_UPPER, K_ILU_FACT, K_MDOT, K_ORTH, K_MG = range(8)
-KERNEL_NAMES = ["fill", "spmv", "ilu_lower", "ilu_upper", "ilu_factor", "mdot", "orth", "mg_vcycle"]
+KERNEL_NAMES = ["k_fill", "k_spmv_msr", "k_ilu_lower", "k_ilu_upper", "k_ilu_factor", "k_mdot",
+                "k_orth_update", "k_mg_resid", "k_vanka_update", "k_csr_apply", "k_vanka_invert",
+                "k_fill(residual only)"]
 
 
 class GmMesh(C.Structure):
@@ -133,7 +134,7 @@ def lib():
     L.gm_mg_num_levels.argtypes = [vp]
     L.gm_mg_setup_dev.argtypes = [vp, vp, vp, vp]
     L.gm_prof_enable.argtypes = [vp, ip]
-    L.gm_prof_get.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(C.c_double)]
+    L.gm_prof_get.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -307,7 +308,19 @@ class Context:
         _check(lib().gm_prof_enable(self.h, 1 if on else 0), "gm_prof_enable")
 
     def prof_get(self, kid: int):
+        """(launches, total ms, total algorithmic bytes) of one kernel family."""
         cnt = C.c_int(0)
         ms = C.c_double(0.0)
-        _check(lib().gm_prof_get(self.h, kid, C.byref(cnt), C.byref(ms)), "gm_prof_get")
-        return cnt.value, ms.value
+        by = C.c_double(0.0)
+        _check(lib().gm_prof_get(self.h, kid, C.byref(cnt), C.byref(ms), C.byref(by)), "gm_prof_get")
+        return cnt.value, ms.value, by.value
+
+    def prof_all(self) -> dict:
+        out = {}
+        for i, nm in enumerate(KERNEL_NAMES):
+            cnt, ms, by = self.prof_get(i)
+            if cnt:
+                out[nm] = dict(launches=cnt, total_ms=ms, avg_ms=ms / cnt,
+                               bytes_per_launch=by / cnt if by else None,
+                               GBps=(by / (ms * 1e-3) / 1e9) if (by and ms) else None)
+        return out

@@ -38,8 +38,9 @@ static void prof_fold(gm_ctx *c, int id) {
   c->ev_used[id] = 0;
 }
 
-void gm_prof_begin(gm_ctx *c, int id) {
+void gm_prof_begin(gm_ctx *c, int id, double bytes) {
   if (!c->prof) return;
+  c->prof_bytes[id] += bytes;
   if (c->ev_used[id] == kProfPool) prof_fold(c, id);
   const int k = c->ev_used[id];
   if ((int)c->ev_b[id].size() <= k) {
@@ -486,15 +487,17 @@ extern "C" int gm_prof_enable(gm_ctx *c, int on) {
   for (int i = 0; i < GM_K_COUNT; i++) {
     c->prof_count[i] = 0;
     c->prof_ms[i] = 0.0;
+    c->prof_bytes[i] = 0.0;
     c->ev_used[i] = 0;
   }
   return GM_OK;
 }
 
-extern "C" int gm_prof_get(gm_ctx *c, int id, int *count, double *ms) {
+extern "C" int gm_prof_get(gm_ctx *c, int id, int *count, double *ms, double *bytes) {
   if (!c || id < 0 || id >= GM_K_COUNT || !count || !ms) return GM_ERR_ARG;
   prof_fold(c, id);
   *count = (int)c->prof_count[id];
   *ms = c->prof_ms[id];
+  if (bytes) *bytes = c->prof_bytes[id];
   return GM_OK;
 }

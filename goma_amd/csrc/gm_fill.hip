@@ -574,9 +574,14 @@ static void launch_fill(gm_ctx *c, FillArgs &a) {
   for (int col = 0; col < c->n_ecol; col++) {
     const int cnt = c->ecol_ptr[col + 1] - c->ecol_ptr[col];
     if (cnt == 0) continue;
-    gm_prof_begin(c, GM_K_FILL);
+    // this colour's share of B_asm = nnz*s + 16 N + 8 dim n_nodes + 4 nve n_el
+    const double share = (double)cnt / c->num_elems;
+    const double basm = (double)(c->nval - 1) * sizeof(JT) + 16.0 * c->N +
+                        8.0 * DIM * c->num_nodes + 4.0 * El<DIM>::NVE * c->num_elems;
+    const int id = a.resid_only ? GM_K_FILL_RESID : GM_K_FILL;
+    gm_prof_begin(c, id, share * (a.resid_only ? basm - (double)(c->nval - 1) * sizeof(JT) : basm));
     k_fill<DIM, EN, VISC, JT, NT><<<cnt, NT, 0, c->stream>>>(a, c->d_ecol_elems + c->ecol_ptr[col]);
-    gm_prof_end(c, GM_K_FILL);
+    gm_prof_end(c, id);
   }
 }
 

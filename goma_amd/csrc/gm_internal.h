@@ -52,6 +52,7 @@ struct IluPlan {
   float *d_y32 = nullptr;
   int *d_node_first = nullptr, *d_node_last = nullptr;  // [N] node row range of each row
   int levels = 0;             // total forward levels over all patches
+  std::vector<long long> col_rows, col_nnzL, col_nnzU;  // per patch colour
   // per-patch level schedules, [0] forward / factorisation, [1] backward
   int *d_lv_ptr[2] = {nullptr, nullptr}, *d_lv_nstart[2] = {nullptr, nullptr};
   int *d_lv_node[2] = {nullptr, nullptr}, *d_lv_rstart[2] = {nullptr, nullptr};
@@ -83,6 +84,7 @@ struct MgLevel {
   void *d_val = nullptr;      // coarse Jacobian (MSR)
   void *d_ainv = nullptr;     // element-block inverses
   int vec_prec = -1, ainv_prec = -1, val_prec = -1;
+  int nnz = 0;                // entries of P
 };
 
 struct gm_ctx {
@@ -131,6 +133,7 @@ struct gm_ctx {
   bool prof = false;
   long long prof_count[GM_K_COUNT] = {0};
   double prof_ms[GM_K_COUNT] = {0};
+  double prof_bytes[GM_K_COUNT] = {0};
   std::vector<hipEvent_t> ev_b[GM_K_COUNT], ev_e[GM_K_COUNT];
   int ev_used[GM_K_COUNT] = {0};
 };
@@ -175,6 +178,6 @@ int gm_mg_apply_f(gm_ctx *c, const float *a0, const float *v, float *z, const gm
 void gm_mg_free(gm_ctx *c);
 
 // timing helpers
-void gm_prof_begin(gm_ctx *c, int id);
+void gm_prof_begin(gm_ctx *c, int id, double bytes = 0.0);
 void gm_prof_end(gm_ctx *c, int id);
 int gm_ilu_factor_export(gm_ctx *c, const double *val, int bins, int *rowptr, int *col, double *lu);

@@ -205,7 +205,8 @@ __global__ __launch_bounds__(256) void k_spmv_msr(int N, const VT *__restrict__ 
 
 template <typename VT, typename XT, typename YT>
 static void spmv(gm_ctx *c, const VT *a, const XT *x, YT *y) {
-  gm_prof_begin(c, GM_K_SPMV);
+  gm_prof_begin(c, GM_K_SPMV, (double)(c->nval - 1) * (sizeof(VT) + 4) + 4.0 * (c->N + 1) +
+                                  (double)c->N * (sizeof(XT) + sizeof(YT)));
   if (c->dim == 2) {
     constexpr int G = 16;
     const int blocks = (int)(((long long)c->N * G + 255) / 256);
@@ -630,6 +631,19 @@ int gm_ilu_plan(gm_ctx *c, int bins) {
     }
   }
   P.levels = (int)lv_ptr[0].back();
+  // per colour: rows and strictly-lower / upper (incl. diagonal) entries, for
+  // the algorithmic-byte counts of the sweeps
+  P.col_rows.assign(ncol, 0);
+  P.col_nnzL.assign(ncol, 0);
+  P.col_nnzU.assign(ncol, 0);
+  for (int col = 0; col < ncol; col++) {
+    const int ra = P.patch_row[P.colour_ptr[col]], rb = P.patch_row[P.colour_ptr[col + 1]];
+    P.col_rows[col] = rb - ra;
+    for (int r = ra; r < rb; r++) {
+      P.col_nnzL[col] += diag[r] - rowptr[r];
+      P.col_nnzU[col] += rowptr[r + 1] - diag[r];
+    }
+  }
   int rc;
   if ((rc = up(&P.d_perm, P.perm)) || (rc = up(&P.d_rowptr, rowptr)) || (rc = up(&P.d_col, col)) ||
       (rc = up(&P.d_diag, diag)) || (rc = up(&P.d_src, src)) || (rc = up(&P.d_patch_row, P.patch_row)) ||
@@ -690,7 +704,10 @@ static void ilu_apply(gm_ctx *c, const VT *lu, VT *ywork, const RT *r, ZT *z) {
   for (int col = 0; col < P.ncol; col++) {
     const int p0 = P.colour_ptr[col], cnt = P.colour_ptr[col + 1] - p0;
     if (!cnt) continue;
-    gm_prof_begin(c, GM_K_ILU_LOWER);
+    // L values + column ids, row pointers, diagonals, node ranges, level lists,
+    // permutation, input (gathered) and output vectors
+    gm_prof_begin(c, GM_K_ILU_LOWER, (double)P.col_nnzL[col] * (sizeof(VT) + 4) +
+                                         (double)P.col_rows[col] * (6 * 4 + sizeof(RT) + sizeof(VT)));
     if (c->dim == 2)
       k_ilu_lower<VT, RT, VT, 16><<<cnt, ILU_NT, 0, c->stream>>>(SL, lu, p0, P.d_perm, r, ywork);
     else
@@ -700,7 +717,8 @@ static void ilu_apply(gm_ctx *c, const VT *lu, VT *ywork, const RT *r, ZT *z) {
   for (int col = P.ncol - 1; col >= 0; col--) {
     const int p0 = P.colour_ptr[col], cnt = P.colour_ptr[col + 1] - p0;
     if (!cnt) continue;
-    gm_prof_begin(c, GM_K_ILU_UPPER);
+    gm_prof_begin(c, GM_K_ILU_UPPER, (double)P.col_nnzU[col] * (sizeof(VT) + 4) +
+                                         (double)P.col_rows[col] * (6 * 4 + 2 * sizeof(VT) + sizeof(ZT)));
     if (c->dim == 2)
       k_ilu_upper<VT, VT, ZT, 16><<<cnt, ILU_NT, 0, c->stream>>>(SU, lu, p0, P.d_perm, ywork, z);
     else
@@ -894,14 +912,14 @@ static int gmres_t(gm_ctx *c, const VT *a, const VT *lu, VT *ywork, const double
       BT *vj = V + (size_t)j * N, *vn = V + (size_t)(j + 1) * N;
       precond(vj, z);
       spmv<VT, BT, BT>(c, a, z, w);
-      gm_prof_begin(c, GM_K_MDOT);
+      gm_prof_begin(c, GM_K_MDOT, (double)(j + 2) * N * sizeof(BT));
       k_mdot<BT><<<KB, RED_THREADS, 0, c->stream>>>(N, j + 1, V, w, K.d_part);
       gm_prof_end(c, GM_K_MDOT);
       GM_HIP(hipGetLastError());
       if ((rc = sum_parts(c, K.d_part, j + 2, KB, h.data()))) return rc;
       const double ww0 = h[j + 1];
       GM_HIP(hipMemcpyAsync(K.d_h, h.data(), (j + 1) * sizeof(double), hipMemcpyHostToDevice, c->stream));
-      gm_prof_begin(c, GM_K_ORTH);
+      gm_prof_begin(c, GM_K_ORTH, (double)(j + 3) * N * sizeof(BT));
       k_orth_update<BT><<<KB, RED_THREADS, 0, c->stream>>>(N, j + 1, V, K.d_h, w, K.d_part);
       gm_prof_end(c, GM_K_ORTH);
       double ww;

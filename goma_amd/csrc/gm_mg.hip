@@ -289,6 +289,7 @@ extern "C" int gm_mg_add_level(gm_ctx *fine, gm_ctx *coarse, const int *P_rowptr
       (rc = upv(&L.d_Rcol, rcol.data(), nnz)) || (rc = upv(&L.d_Rval, rval.data(), nnz)) ||
       (rc = upv(&L.d_inject, inject, coarse->num_nodes)))
     return rc;
+  L.nnz = nnz;
   GM_HIP(hipMalloc(&L.d_state, Nc * sizeof(double)));
   GM_HIP(hipMalloc(&L.d_rdummy, Nc * sizeof(double)));
   fine->mg.push_back(L);
@@ -324,19 +325,24 @@ static int alloc_level_vectors(MgLevel &L, const gm_ctx *lc, int prec, bool coar
 }
 
 template <int DIM, int EN, typename VT, typename AT>
-static void launch_invert(gm_ctx *lc, hipStream_t s, const VT *val, AT *ainv) {
+static void launch_invert(gm_ctx *prof, gm_ctx *lc, hipStream_t s, const VT *val, AT *ainv) {
   constexpr int NT = DIM == 2 ? 128 : 256;
+  constexpr int n = Loc<DIM, EN>::N;
+  // gathered entries + map + inverses written
+  const double bytes = (double)lc->num_elems * n * n * (sizeof(VT) + 4 + sizeof(AT));
+  gm_prof_begin(prof, GM_K_VANKA_INV, bytes);
   k_vanka_invert<DIM, EN, VT, AT, NT><<<lc->num_elems, NT, 0, s>>>(level_dev(lc), val, ainv);
+  gm_prof_end(prof, GM_K_VANKA_INV);
 }
 
 template <typename VT>
-static void invert(gm_ctx *lc, hipStream_t s, const VT *val, VT *ainv) {
+static void invert(gm_ctx *prof, gm_ctx *lc, hipStream_t s, const VT *val, VT *ainv) {
   if (lc->dim == 2) {
-    if (lc->energy) launch_invert<2, 1, VT, VT>(lc, s, val, ainv);
-    else launch_invert<2, 0, VT, VT>(lc, s, val, ainv);
+    if (lc->energy) launch_invert<2, 1, VT, VT>(prof, lc, s, val, ainv);
+    else launch_invert<2, 0, VT, VT>(prof, lc, s, val, ainv);
   } else {
-    if (lc->energy) launch_invert<3, 1, VT, VT>(lc, s, val, ainv);
-    else launch_invert<3, 0, VT, VT>(lc, s, val, ainv);
+    if (lc->energy) launch_invert<3, 1, VT, VT>(prof, lc, s, val, ainv);
+    else launch_invert<3, 0, VT, VT>(prof, lc, s, val, ainv);
   }
 }
 
@@ -373,31 +379,45 @@ int gm_mg_setup(gm_ctx *c, const double *x, const double *val, const float *val3
     above = lc;
   }
   if (prec == GM_PREC_MIXED) {
-    invert<float>(c, c->stream, val32, (float *)c->mg0.d_ainv);
-    for (MgLevel &L : c->mg) invert<float>(L.ctx, c->stream, (const float *)L.d_val, (float *)L.d_ainv);
+    invert<float>(c, c, c->stream, val32, (float *)c->mg0.d_ainv);
+    for (MgLevel &L : c->mg)
+      invert<float>(c, L.ctx, c->stream, (const float *)L.d_val, (float *)L.d_ainv);
   } else {
-    invert<double>(c, c->stream, val, (double *)c->mg0.d_ainv);
-    for (MgLevel &L : c->mg) invert<double>(L.ctx, c->stream, (const double *)L.d_val, (double *)L.d_ainv);
+    invert<double>(c, c, c->stream, val, (double *)c->mg0.d_ainv);
+    for (MgLevel &L : c->mg)
+      invert<double>(c, L.ctx, c->stream, (const double *)L.d_val, (double *)L.d_ainv);
   }
   GM_HIP(hipGetLastError());
   return GM_OK;
 }
 
+static gm_ctx *g_prof_ctx = nullptr;  // context whose counters the V-cycle feeds
+
 template <typename VT, typename BT>
 static void resid(gm_ctx *lc, hipStream_t s, const VT *a, const BT *x, const BT *b, BT *r) {
   constexpr int G = 16;
   const int blocks = (int)(((long long)lc->N * G + 255) / 256);
+  const double bytes = (double)(lc->nval - 1) * (sizeof(VT) + 4) + 4.0 * (lc->N + 1) +
+                       3.0 * lc->N * sizeof(BT);
+  gm_prof_begin(g_prof_ctx, GM_K_MG_RESID, bytes);
   k_mg_resid<VT, BT, G><<<blocks, 256, 0, s>>>(lc->N, a, lc->d_bindx, x, b, r);
+  gm_prof_end(g_prof_ctx, GM_K_MG_RESID);
 }
 
 template <int DIM, int EN, typename AT, typename BT>
 static void vanka_colour(gm_ctx *lc, hipStream_t s, int col, const AT *ainv, const BT *r, BT *x,
                          double omega) {
   constexpr int NT = DIM == 2 ? 64 : 128;
+  constexpr int n = Loc<DIM, EN>::N;
   const int cnt = lc->ecol_ptr[col + 1] - lc->ecol_ptr[col];
-  if (cnt)
-    k_vanka_update<DIM, EN, AT, BT, NT><<<cnt, NT, 0, s>>>(level_dev(lc), lc->d_ecol_elems + lc->ecol_ptr[col],
-                                                           ainv, r, x, omega);
+  if (!cnt) return;
+  // inverses, element lists/connectivity, residual gathered, x read and written
+  const double bytes = (double)cnt * (n * n * sizeof(AT) + 4.0 * (Loc<DIM, EN>::NVE + 2) +
+                                      n * 3.0 * sizeof(BT));
+  gm_prof_begin(g_prof_ctx, GM_K_VANKA, bytes);
+  k_vanka_update<DIM, EN, AT, BT, NT><<<cnt, NT, 0, s>>>(level_dev(lc), lc->d_ecol_elems + lc->ecol_ptr[col],
+                                                         ainv, r, x, omega);
+  gm_prof_end(g_prof_ctx, GM_K_VANKA);
 }
 
 template <typename VT, typename BT>
@@ -417,9 +437,21 @@ static void smooth(gm_ctx *lc, hipStream_t s, const VT *a, const VT *ainv, const
 }
 
 // z = M^-1 v: one V-cycle.  VT: matrix / inverse type, BT: vector type.
+template <typename BT>
+static void transfer(gm_ctx *c, hipStream_t s, int nrows, const int *rp, const int *ci,
+                     const double *cv, int nnz, int ncols, const BT *v, const double *sc,
+                     const signed char *mask, BT *y, int acc) {
+  const double bytes = (double)nnz * 12.0 + 4.0 * (nrows + 1) + (double)ncols * sizeof(BT) +
+                       (double)nrows * (1 + (acc ? 2 : 1) * sizeof(BT));
+  gm_prof_begin(c, GM_K_TRANSFER, bytes);
+  k_csr_apply<BT><<<1024, 256, 0, s>>>(nrows, rp, ci, cv, v, sc, mask, y, acc);
+  gm_prof_end(c, GM_K_TRANSFER);
+}
+
 template <typename VT, typename BT>
 static void vcycle(gm_ctx *c, const VT *a0, const BT *v, BT *z, const gm_solver_opts *o) {
   hipStream_t s = c->stream;
+  g_prof_ctx = c;
   const int nl = (int)c->mg.size();
   const int nu = o->mg_nu > 0 ? o->mg_nu : 1;
   const double om = o->mg_omega > 0 ? o->mg_omega : 1.0;
@@ -441,8 +473,8 @@ static void vcycle(gm_ctx *c, const VT *a0, const BT *v, BT *z, const gm_solver_
     resid<VT, BT>(lc, s, amat(l), x, b, (BT *)L.d_r);
     MgLevel &C = c->mg[l];
     // restrict the (unscaled) residual: the level-0 matrix is row-sum scaled
-    k_csr_apply<BT><<<1024, 256, 0, s>>>(C.ctx->N, C.d_Rrow, C.d_Rcol, C.d_Rval, (const BT *)L.d_r,
-                                         l == 0 ? c->mg_scale : nullptr, C.ctx->d_dbc, (BT *)C.d_b, 0);
+    transfer<BT>(c, s, C.ctx->N, C.d_Rrow, C.d_Rcol, C.d_Rval, C.nnz, lc->N, (const BT *)L.d_r,
+                 l == 0 ? c->mg_scale : nullptr, C.ctx->d_dbc, (BT *)C.d_b, 0);
   }
   for (int l = nl - 1; l >= 0; l--) {
     gm_ctx *lc = lv_ctx(l);
@@ -450,24 +482,20 @@ static void vcycle(gm_ctx *c, const VT *a0, const BT *v, BT *z, const gm_solver_
     MgLevel &C = c->mg[l];
     BT *x = l == 0 ? x0 : (BT *)L.d_x;
     const BT *b = l == 0 ? b0 : (const BT *)L.d_b;
-    k_csr_apply<BT><<<1024, 256, 0, s>>>(lc->N, C.d_Prow, C.d_Pcol, C.d_Pval, (const BT *)C.d_x,
-                                         nullptr, lc->d_dbc, x, 1);
+    transfer<BT>(c, s, lc->N, C.d_Prow, C.d_Pcol, C.d_Pval, C.nnz, C.ctx->N, (const BT *)C.d_x,
+                 nullptr, lc->d_dbc, x, 1);
     smooth<VT, BT>(lc, s, amat(l), (const VT *)L.d_ainv, b, x, (BT *)L.d_r, nu, om);
   }
 }
 
 int gm_mg_apply_d(gm_ctx *c, const double *a0, const double *v, double *z, const gm_solver_opts *o) {
-  gm_prof_begin(c, GM_K_MG);
   vcycle<double, double>(c, a0, v, z, o);
-  gm_prof_end(c, GM_K_MG);
   GM_HIP(hipGetLastError());
   return GM_OK;
 }
 
 int gm_mg_apply_f(gm_ctx *c, const float *a0, const float *v, float *z, const gm_solver_opts *o) {
-  gm_prof_begin(c, GM_K_MG);
   vcycle<float, float>(c, a0, v, z, o);
-  gm_prof_end(c, GM_K_MG);
   GM_HIP(hipGetLastError());
   return GM_OK;
 }

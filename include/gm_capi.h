@@ -184,7 +184,9 @@ int gm_mg_setup_dev(gm_ctx *ctx, const double *x_dev, const double *val_dev, con
 
 /* Kernel timing: when enabled, every launch of the kernels below is
  * bracketed by HIP events on the context's stream (the stream the kernels
- * run on); gm_prof_get returns the launch count and the summed duration. */
+ * run on); gm_prof_get returns the launch count, the summed duration and the
+ * summed algorithmic HBM bytes (compulsory traffic, each array counted once
+ * per launch) of those launches. */
 #define GM_K_FILL 0       /* k_fill: assembly + scatter of one element colour */
 #define GM_K_SPMV 1       /* k_spmv_msr: MSR matvec */
 #define GM_K_ILU_LOWER 2  /* k_ilu_lower: forward sweep of one patch colour */
@@ -192,10 +194,14 @@ int gm_mg_setup_dev(gm_ctx *ctx, const double *x_dev, const double *val_dev, con
 #define GM_K_ILU_FACT 4   /* k_ilu_factor: ILU(0) of one patch colour */
 #define GM_K_MDOT 5       /* k_mdot: Gram-Schmidt projections */
 #define GM_K_ORTH 6       /* k_orth_update: Gram-Schmidt update */
-#define GM_K_MG 7         /* one multigrid V-cycle (all its kernels) */
-#define GM_K_COUNT 8
+#define GM_K_MG_RESID 7   /* k_mg_resid: multigrid residual r = b - A x, one level */
+#define GM_K_VANKA 8      /* k_vanka_update: element-block updates of one colour */
+#define GM_K_TRANSFER 9   /* k_csr_apply: restriction or prolongation */
+#define GM_K_VANKA_INV 10 /* k_vanka_invert: element-block inverses of one level */
+#define GM_K_FILL_RESID 11 /* k_fill, residual only (line-search trial), one colour */
+#define GM_K_COUNT 12
 int gm_prof_enable(gm_ctx *ctx, int on);
-int gm_prof_get(gm_ctx *ctx, int kernel_id, int *count, double *total_ms);
+int gm_prof_get(gm_ctx *ctx, int kernel_id, int *count, double *total_ms, double *bytes);
 
 #ifdef __cplusplus
 }

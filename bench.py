@@ -140,6 +140,8 @@ def main(argv=None):
     ap.add_argument("--line-search", type=int, default=1, help="1: Goma's backtracking line search")
     ap.add_argument("--mg-coarse", type=int, default=2, help="coarsest elements per side")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--prof-every", type=int, default=8,
+                    help="HIP events around one launch in this many, per kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args(argv)
@@ -175,7 +177,7 @@ def main(argv=None):
         rc, rep = ctx.solve_dev(x.data_ptr(), **opts)
         print(f"[bench] warmup {w}: newton {rep.newton_its} gmres {rep.lin_its_total} "
               f"{rep.t_total:.2f}s", file=sys.stderr, flush=True)
-    ctx.prof_enable(True)
+    ctx.prof_enable(True, every=args.prof_every)
     reps = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()

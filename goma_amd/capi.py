@@ -304,8 +304,10 @@ class Context:
         _check(lib().gm_mg_setup_dev(self.h, x_ptr, val_ptr, scale_ptr), "gm_mg_setup_dev")
 
     # ---- kernel timing
-    def prof_enable(self, on: bool = True):
-        _check(lib().gm_prof_enable(self.h, 1 if on else 0), "gm_prof_enable")
+    def prof_enable(self, on=True, every: int = 1):
+        """Time every ``every``-th launch of each kernel (0/False: off)."""
+        _check(lib().gm_prof_enable(self.h, (every if every > 1 else 1) if on else 0),
+               "gm_prof_enable")
 
     def prof_get(self, kid: int):
         """(launches, total ms, total algorithmic bytes) of one kernel family."""

@@ -38,9 +38,16 @@ static void prof_fold(gm_ctx *c, int id) {
   c->ev_used[id] = 0;
 }
 
+// Every launch adds its bytes to the family's total; HIP events bracket one
+// launch in prof_every (and the family's first), and the reported duration
+// scales the sampled time by launches / sampled launches.
 void gm_prof_begin(gm_ctx *c, int id, double bytes) {
   if (!c->prof) return;
   c->prof_bytes[id] += bytes;
+  c->prof_seen[id]++;
+  c->prof_active[id] = (c->prof_seen[id] - 1) % c->prof_every == 0;
+  if (!c->prof_active[id]) return;
+  c->prof_sampled_bytes[id] += bytes;
   if (c->ev_used[id] == kProfPool) prof_fold(c, id);
   const int k = c->ev_used[id];
   if ((int)c->ev_b[id].size() <= k) {
@@ -54,7 +61,7 @@ void gm_prof_begin(gm_ctx *c, int id, double bytes) {
 }
 
 void gm_prof_end(gm_ctx *c, int id) {
-  if (!c->prof) return;
+  if (!c->prof || !c->prof_active[id]) return;
   hipEventRecord(c->ev_e[id][c->ev_used[id]], c->stream);
   c->ev_used[id]++;
 }
@@ -85,7 +92,7 @@ static void free_ctx(gm_ctx *c) {
   void *ptrs[] = {c->d_coor, c->d_conn,   c->d_fu,    c->d_centroid, c->d_dbc,   c->d_dbc_val,
                   c->d_bindx, c->d_ecol_elems, c->d_e2nz, c->d_flag, c->d_tables, c->d_val,
                   c->d_resid, c->d_dx,    c->d_scale, c->d_val32,    c->d_red,   c->d_xwork,
-                  c->d_xsave, c->d_rtrial, c->d_w};
+                  c->d_xsave, c->d_rtrial, c->d_w, c->d_mg_rows};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->h_red) hipHostFree(c->h_red);
@@ -484,10 +491,14 @@ extern "C" int gm_prof_enable(gm_ctx *c, int on) {
   if (!c) return GM_ERR_ARG;
   if (c->stream) hipStreamSynchronize(c->stream);
   c->prof = on != 0;
+  c->prof_every = on > 1 ? on : 1;
   for (int i = 0; i < GM_K_COUNT; i++) {
     c->prof_count[i] = 0;
     c->prof_ms[i] = 0.0;
     c->prof_bytes[i] = 0.0;
+    c->prof_sampled_bytes[i] = 0.0;
+    c->prof_seen[i] = 0;
+    c->prof_active[i] = false;
     c->ev_used[i] = 0;
   }
   return GM_OK;
@@ -496,8 +507,13 @@ extern "C" int gm_prof_enable(gm_ctx *c, int on) {
 extern "C" int gm_prof_get(gm_ctx *c, int id, int *count, double *ms, double *bytes) {
   if (!c || id < 0 || id >= GM_K_COUNT || !count || !ms) return GM_ERR_ARG;
   prof_fold(c, id);
-  *count = (int)c->prof_count[id];
-  *ms = c->prof_ms[id];
+  // time of all launches = sampled time x (all bytes / sampled bytes) when the
+  // launches differ in size, x (launches / sampled) otherwise
+  const long long seen = c->prof_seen[id], sampled = c->prof_count[id];
+  double scale = sampled ? (double)seen / sampled : 0.0;
+  if (c->prof_sampled_bytes[id] > 0) scale = c->prof_bytes[id] / c->prof_sampled_bytes[id];
+  *count = (int)seen;
+  *ms = c->prof_ms[id] * scale;
   if (bytes) *bytes = c->prof_bytes[id];
   return GM_OK;
 }

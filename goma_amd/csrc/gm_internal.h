@@ -129,11 +129,18 @@ struct gm_ctx {
   std::vector<MgLevel> mg;  // coarse levels, finest first
   MgLevel mg0;              // level-0 vectors and element inverses
   const double *mg_scale = nullptr;  // row-sum scale of the level-0 matrix
+  int *d_mg_rows = nullptr;          // unknowns of each element colour (smoother residuals)
+  std::vector<int> mg_rows_ptr;
+  std::vector<long long> mg_rows_nnz;
 
   bool prof = false;
   long long prof_count[GM_K_COUNT] = {0};
   double prof_ms[GM_K_COUNT] = {0};
   double prof_bytes[GM_K_COUNT] = {0};
+  double prof_sampled_bytes[GM_K_COUNT] = {0};
+  long long prof_seen[GM_K_COUNT] = {0};
+  bool prof_active[GM_K_COUNT] = {false};
+  int prof_every = 1;
   std::vector<hipEvent_t> ev_b[GM_K_COUNT], ev_e[GM_K_COUNT];
   int ev_used[GM_K_COUNT] = {0};
 };
@@ -174,7 +181,7 @@ void gm_free_linalg(gm_ctx *c);
 int gm_mg_setup(gm_ctx *c, const double *x, const double *val, const float *val32, int prec,
                 const double *scale);
 int gm_mg_apply_d(gm_ctx *c, const double *a0, const double *v, double *z, const gm_solver_opts *o);
-int gm_mg_apply_f(gm_ctx *c, const float *a0, const float *v, float *z, const gm_solver_opts *o);
+int gm_mg_apply_f(gm_ctx *c, const float *a0, const double *v, double *z, const gm_solver_opts *o);
 void gm_mg_free(gm_ctx *c);
 
 // timing helpers

@@ -876,8 +876,8 @@ static int gmres_t(gm_ctx *c, const VT *a, const VT *lu, VT *ywork, const double
 
   auto precond = [&](const BT *in, BT *out) {
     if (o->precond == GM_PC_MG) {
-      if constexpr (sizeof(BT) == 8) gm_mg_apply_d(c, (const double *)a, (const double *)in, (double *)out, o);
-      else gm_mg_apply_f(c, (const float *)a, (const float *)in, (float *)out, o);
+      if constexpr (sizeof(VT) == 8) gm_mg_apply_d(c, (const double *)a, (const double *)in, (double *)out, o);
+      else gm_mg_apply_f(c, (const float *)a, (const double *)in, (double *)out, o);
     } else if (pc) {
       ilu_apply<VT, BT, BT>(c, lu, ywork, in, out);
     } else {
@@ -999,14 +999,16 @@ int gm_gmres_run(gm_ctx *c, const double *val, const float *val32, const double 
 int gm_gmres_run_nofactor(gm_ctx *c, const double *val, const float *val32, const double *b,
                           double *x, const gm_solver_opts *o, int *its, double *relres) {
   const int prec = o->precision;
-  int rc = ensure_krylov(c, o->kspace, prec);
+  int rc = ensure_krylov(c, o->kspace, GM_PREC_FP64);
   if (rc) return rc;
   if (prec == GM_PREC_MIXED) {
     if (!val32) {
       gm_set_error("mixed precision needs FP32 Jacobian values");
       return GM_ERR_ARG;
     }
-    return gmres_t<float, float>(c, val32, c->ilu.d_lu32, c->ilu.d_y32, b, x, o, its, relres);
+    // FP32 Jacobian values and preconditioner; Krylov basis, dot products
+    // and the update in FP64
+    return gmres_t<double, float>(c, val32, c->ilu.d_lu32, c->ilu.d_y32, b, x, o, its, relres);
   }
   return gmres_t<double, double>(c, val, c->ilu.d_lu, c->ilu.d_y, b, x, o, its, relres);
 }

@@ -163,6 +163,26 @@ __global__ __launch_bounds__(256) void k_mg_resid(int N, const VT *__restrict__ 
   if (lane == 0) r[row] = (BT)((AT)b[row] - s - (AT)a[row] * (AT)x[row]);
 }
 
+// r = b - A x on a list of rows (the unknowns of one element colour)
+template <typename VT, typename BT, int G>
+__global__ __launch_bounds__(256) void k_mg_resid_rows(int nrows, const int *__restrict__ rows,
+                                                       const VT *__restrict__ a,
+                                                       const int *__restrict__ bindx,
+                                                       const BT *__restrict__ x,
+                                                       const BT *__restrict__ b, BT *__restrict__ r) {
+  const int k = (blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const int lane = threadIdx.x & (G - 1);
+  if (k >= nrows) return;
+  const int row = rows[k];
+  using AT = typename std::conditional<sizeof(BT) == 8, double, float>::type;
+  const int k0 = bindx[row], k1 = bindx[row + 1];
+  AT s = 0;
+  for (int q = k0 + lane; q < k1; q += G) s += (AT)a[q] * (AT)x[bindx[q]];
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
+  if (lane == 0) r[row] = (BT)((AT)b[row] - s - (AT)a[row] * (AT)x[row]);
+}
+
 // x_e += omega * A_e^-1 r_e for the elements of one colour
 template <int DIM, int EN, typename AT, typename BT, int NT>
 __global__ __launch_bounds__(NT) void k_vanka_update(LevelDev L, const int *__restrict__ elems,
@@ -298,17 +318,16 @@ extern "C" int gm_mg_add_level(gm_ctx *fine, gm_ctx *coarse, const int *P_rowptr
 
 extern "C" int gm_mg_num_levels(const gm_ctx *c) { return c ? (int)c->mg.size() + 1 : GM_ERR_ARG; }
 
+// Level vectors are FP64 in both precisions; matrices and element-block
+// inverses are FP32 in the mixed precision.
 static int alloc_level_vectors(MgLevel &L, const gm_ctx *lc, int prec, bool coarse) {
   const size_t N = lc->N;
   const size_t es = prec == GM_PREC_MIXED ? sizeof(float) : sizeof(double);
-  if (L.vec_prec != prec) {
-    if (L.d_x) (void)hipFree(L.d_x);
-    if (L.d_b) (void)hipFree(L.d_b);
-    if (L.d_r) (void)hipFree(L.d_r);
-    GM_HIP(hipMalloc(&L.d_x, N * es));
-    GM_HIP(hipMalloc(&L.d_b, N * es));
-    GM_HIP(hipMalloc(&L.d_r, N * es));
-    L.vec_prec = prec;
+  if (!L.d_x) {
+    GM_HIP(hipMalloc(&L.d_x, N * sizeof(double)));
+    GM_HIP(hipMalloc(&L.d_b, N * sizeof(double)));
+    GM_HIP(hipMalloc(&L.d_r, N * sizeof(double)));
+    L.vec_prec = 0;
   }
   const size_t nl = nloc_of(lc);
   if (L.ainv_prec != prec) {
@@ -346,6 +365,35 @@ static void invert(gm_ctx *prof, gm_ctx *lc, hipStream_t s, const VT *val, VT *a
   }
 }
 
+// Rows touched by each element colour: the unknowns of its elements.
+static int build_colour_rows(gm_ctx *lc) {
+  if (lc->d_mg_rows) return GM_OK;
+  const int nve = lc->nve, cl = lc->dim == 2 ? 8 : 20;
+  std::vector<int> rows;
+  std::vector<char> seen(lc->N, 0);
+  lc->mg_rows_ptr.assign(1, 0);
+  lc->mg_rows_nnz.clear();
+  for (int col = 0; col < lc->n_ecol; col++) {
+    std::vector<int> list;
+    for (int k = lc->ecol_ptr[col]; k < lc->ecol_ptr[col + 1]; k++) {
+      const int e = lc->ecol_elems[k];
+      for (int i = 0; i < nve; i++) {
+        const int n = lc->conn[(size_t)e * nve + i];
+        const int cnt = (i == cl) ? lc->first_unknown[n + 1] - lc->first_unknown[n] : lc->nv;
+        for (int a = 0; a < cnt; a++) list.push_back(lc->first_unknown[n] + a);
+      }
+    }
+    std::sort(list.begin(), list.end());
+    list.erase(std::unique(list.begin(), list.end()), list.end());
+    long long nnz = 0;
+    for (int r : list) nnz += 1 + lc->bindx[r + 1] - lc->bindx[r];
+    rows.insert(rows.end(), list.begin(), list.end());
+    lc->mg_rows_ptr.push_back((int)rows.size());
+    lc->mg_rows_nnz.push_back(nnz);
+  }
+  return upv(&lc->d_mg_rows, rows.data(), rows.size());
+}
+
 // Per Newton step: coarse Jacobians at the injected state, local inverses on
 // every level.  val / val32 is the level-0 (scaled) matrix, x the state.
 int gm_mg_setup(gm_ctx *c, const double *x, const double *val, const float *val32, int prec,
@@ -357,6 +405,9 @@ int gm_mg_setup(gm_ctx *c, const double *x, const double *val, const float *val3
   }
   int rc;
   if ((rc = alloc_level_vectors(c->mg0, c, prec, false))) return rc;
+  if ((rc = build_colour_rows(c))) return rc;
+  for (MgLevel &L : c->mg)
+    if ((rc = build_colour_rows(L.ctx))) return rc;
   const double *xs = x;
   gm_ctx *above = c;
   for (MgLevel &L : c->mg) {
@@ -404,6 +455,21 @@ static void resid(gm_ctx *lc, hipStream_t s, const VT *a, const BT *x, const BT 
   gm_prof_end(g_prof_ctx, GM_K_MG_RESID);
 }
 
+template <typename VT, typename BT>
+static void resid_rows(gm_ctx *lc, hipStream_t s, int col, const VT *a, const BT *x, const BT *b,
+                       BT *r) {
+  constexpr int G = 16;
+  const int nr = lc->mg_rows_ptr[col + 1] - lc->mg_rows_ptr[col];
+  if (!nr) return;
+  const int blocks = (int)(((long long)nr * G + 255) / 256);
+  const double nnz_rows = (double)lc->mg_rows_nnz[col];
+  const double bytes = nnz_rows * (sizeof(VT) + 4) + nr * (4.0 * 3 + 3.0 * sizeof(BT));
+  gm_prof_begin(g_prof_ctx, GM_K_MG_RESID, bytes);
+  k_mg_resid_rows<VT, BT, G><<<blocks, 256, 0, s>>>(nr, lc->d_mg_rows + lc->mg_rows_ptr[col], a,
+                                                    lc->d_bindx, x, b, r);
+  gm_prof_end(g_prof_ctx, GM_K_MG_RESID);
+}
+
 template <int DIM, int EN, typename AT, typename BT>
 static void vanka_colour(gm_ctx *lc, hipStream_t s, int col, const AT *ainv, const BT *r, BT *x,
                          double omega) {
@@ -425,7 +491,7 @@ static void smooth(gm_ctx *lc, hipStream_t s, const VT *a, const VT *ainv, const
                    int sweeps, double omega) {
   for (int it = 0; it < sweeps; it++)
     for (int col = 0; col < lc->n_ecol; col++) {
-      resid<VT, BT>(lc, s, a, x, b, r);
+      resid_rows<VT, BT>(lc, s, col, a, x, b, r);
       if (lc->dim == 2) {
         if (lc->energy) vanka_colour<2, 1, VT, BT>(lc, s, col, ainv, r, x, omega);
         else vanka_colour<2, 0, VT, BT>(lc, s, col, ainv, r, x, omega);
@@ -494,8 +560,8 @@ int gm_mg_apply_d(gm_ctx *c, const double *a0, const double *v, double *z, const
   return GM_OK;
 }
 
-int gm_mg_apply_f(gm_ctx *c, const float *a0, const float *v, float *z, const gm_solver_opts *o) {
-  vcycle<float, float>(c, a0, v, z, o);
+int gm_mg_apply_f(gm_ctx *c, const float *a0, const double *v, double *z, const gm_solver_opts *o) {
+  vcycle<float, double>(c, a0, v, z, o);
   GM_HIP(hipGetLastError());
   return GM_OK;
 }

@@ -200,6 +200,8 @@ int gm_mg_setup_dev(gm_ctx *ctx, const double *x_dev, const double *val_dev, con
 #define GM_K_VANKA_INV 10 /* k_vanka_invert: element-block inverses of one level */
 #define GM_K_FILL_RESID 11 /* k_fill, residual only (line-search trial), one colour */
 #define GM_K_COUNT 12
+/* on = 0: off; 1: time every launch; k > 1: time one launch in k of each
+ * kernel (low overhead), durations scaled to all launches by their bytes. */
 int gm_prof_enable(gm_ctx *ctx, int on);
 int gm_prof_get(gm_ctx *ctx, int kernel_id, int *count, double *total_ms, double *bytes);
 

@@ -38,32 +38,31 @@ static void prof_fold(gm_ctx *c, int id) {
   c->ev_used[id] = 0;
 }
 
-// Every launch adds its bytes to the family's total; HIP events bracket one
-// launch in prof_every (and the family's first), and the reported duration
-// scales the sampled time by launches / sampled launches.
-void gm_prof_begin(gm_ctx *c, int id, double bytes) {
-  if (!c->prof) return;
+// Every launch adds its bytes to the family's total; one launch in prof_every
+// (and the family's first) is timed by events that its own dispatch stamps,
+// and the reported duration scales the sampled time by bytes / sampled bytes.
+bool gm_prof_take(gm_ctx *c, int id, double bytes, hipEvent_t *b, hipEvent_t *e) {
+  if (!c->prof) return false;
   c->prof_bytes[id] += bytes;
   c->prof_seen[id]++;
-  c->prof_active[id] = (c->prof_seen[id] - 1) % c->prof_every == 0;
-  if (!c->prof_active[id]) return;
-  c->prof_sampled_bytes[id] += bytes;
+  if ((c->prof_seen[id] - 1) % c->prof_every != 0) return false;
   if (c->ev_used[id] == kProfPool) prof_fold(c, id);
   const int k = c->ev_used[id];
   if ((int)c->ev_b[id].size() <= k) {
-    hipEvent_t b, e;
-    hipEventCreate(&b);
-    hipEventCreate(&e);
-    c->ev_b[id].push_back(b);
-    c->ev_e[id].push_back(e);
+    hipEvent_t eb, ee;
+    if (hipEventCreate(&eb) != hipSuccess) return false;
+    if (hipEventCreate(&ee) != hipSuccess) {
+      (void)hipEventDestroy(eb);
+      return false;
+    }
+    c->ev_b[id].push_back(eb);
+    c->ev_e[id].push_back(ee);
   }
-  hipEventRecord(c->ev_b[id][k], c->stream);
-}
-
-void gm_prof_end(gm_ctx *c, int id) {
-  if (!c->prof || !c->prof_active[id]) return;
-  hipEventRecord(c->ev_e[id][c->ev_used[id]], c->stream);
+  c->prof_sampled_bytes[id] += bytes;
   c->ev_used[id]++;
+  *b = c->ev_b[id][k];
+  *e = c->ev_e[id][k];
+  return true;
 }
 
 extern "C" void gm_default_solver_opts(gm_solver_opts *o) {
@@ -466,8 +465,10 @@ extern "C" int gm_solve_nonlinear_dev(gm_ctx *c, double *x, const gm_solver_opts
                   (cLoor < o->eps_corr);
     }
     if (o->verbose)
-      fprintf(stderr, "[gm] newton %2d |R|2 %.3e |R|oo %.3e unscaled %.3e lin_its %d damp %.4f\n",
+      fprintf(stderr,
+              "[gm] newton %2d |R|2 %.3e |R|oo %.3e unscaled %.3e lin_its %d relres %.2e damp %.4f\n",
               inewton, L2, Loo, un, inewton < GM_MAX_REPORT ? rep->lin_its[inewton] : -1,
+              inewton < GM_MAX_REPORT ? rep->lin_relres[inewton] : 0.0,
               inewton < GM_MAX_REPORT ? rep->damp[inewton] : 0.0);
     inewton++;
   }
@@ -498,7 +499,6 @@ extern "C" int gm_prof_enable(gm_ctx *c, int on) {
     c->prof_bytes[i] = 0.0;
     c->prof_sampled_bytes[i] = 0.0;
     c->prof_seen[i] = 0;
-    c->prof_active[i] = false;
     c->ev_used[i] = 0;
   }
   return GM_OK;

@@ -29,7 +29,7 @@
 #include <cmath>
 #include <vector>
 
-#include "gm_internal.h"
+#include "gm_launch.h"
 
 template <int DIM>
 struct El {
@@ -579,9 +579,9 @@ static void launch_fill(gm_ctx *c, FillArgs &a) {
     const double basm = (double)(c->nval - 1) * sizeof(JT) + 16.0 * c->N +
                         8.0 * DIM * c->num_nodes + 4.0 * El<DIM>::NVE * c->num_elems;
     const int id = a.resid_only ? GM_K_FILL_RESID : GM_K_FILL;
-    gm_prof_begin(c, id, share * (a.resid_only ? basm - (double)(c->nval - 1) * sizeof(JT) : basm));
-    k_fill<DIM, EN, VISC, JT, NT><<<cnt, NT, 0, c->stream>>>(a, c->d_ecol_elems + c->ecol_ptr[col]);
-    gm_prof_end(c, id);
+    gm_launch(c, id, share * (a.resid_only ? basm - (double)(c->nval - 1) * sizeof(JT) : basm),
+              k_fill<DIM, EN, VISC, JT, NT>, dim3(cnt), dim3(NT), 0, c->stream, a,
+              c->d_ecol_elems + c->ecol_ptr[col]);
   }
 }
 

@@ -139,7 +139,6 @@ struct gm_ctx {
   double prof_bytes[GM_K_COUNT] = {0};
   double prof_sampled_bytes[GM_K_COUNT] = {0};
   long long prof_seen[GM_K_COUNT] = {0};
-  bool prof_active[GM_K_COUNT] = {false};
   int prof_every = 1;
   std::vector<hipEvent_t> ev_b[GM_K_COUNT], ev_e[GM_K_COUNT];
   int ev_used[GM_K_COUNT] = {0};
@@ -184,7 +183,8 @@ int gm_mg_apply_d(gm_ctx *c, const double *a0, const double *v, double *z, const
 int gm_mg_apply_f(gm_ctx *c, const float *a0, const double *v, double *z, const gm_solver_opts *o);
 void gm_mg_free(gm_ctx *c);
 
-// timing helpers
-void gm_prof_begin(gm_ctx *c, int id, double bytes = 0.0);
-void gm_prof_end(gm_ctx *c, int id);
+// timing helper (used through gm_launch in gm_launch.h): adds `bytes` to the
+// family's total; returns true, with the event pair that the launch must
+// stamp, when this launch is one of the sampled ones.
+bool gm_prof_take(gm_ctx *c, int id, double bytes, hipEvent_t *b, hipEvent_t *e);
 int gm_ilu_factor_export(gm_ctx *c, const double *val, int bins, int *rowptr, int *col, double *lu);

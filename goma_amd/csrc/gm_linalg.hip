@@ -9,7 +9,7 @@
 #include <cstring>
 #include <vector>
 
-#include "gm_internal.h"
+#include "gm_launch.h"
 
 static constexpr int RED_BLOCKS = 1024;
 static constexpr int RED_THREADS = 256;
@@ -195,7 +195,10 @@ __global__ __launch_bounds__(256) void k_spmv_msr(int N, const VT *__restrict__ 
   const int lane = threadIdx.x & (G - 1);
   if (r >= N) return;
   const int k0 = bindx[r], k1 = bindx[r + 1];
-  using AT = typename std::conditional<sizeof(VT) == 8, double, float>::type;
+  // FP64 accumulation whenever the vector is FP64: an FP32 matrix times an
+  // FP64 Krylov vector must not round the vector, or the Arnoldi relation
+  // only holds to ~1e-7 |A||x| and GMRES stalls on ill-conditioned systems.
+  using AT = typename std::conditional<sizeof(VT) == 8 || sizeof(XT) == 8, double, float>::type;
   AT s = 0;
   for (int k = k0 + lane; k < k1; k += G) s += (AT)a[k] * (AT)x[bindx[k]];
 #pragma unroll
@@ -205,18 +208,19 @@ __global__ __launch_bounds__(256) void k_spmv_msr(int N, const VT *__restrict__ 
 
 template <typename VT, typename XT, typename YT>
 static void spmv(gm_ctx *c, const VT *a, const XT *x, YT *y) {
-  gm_prof_begin(c, GM_K_SPMV, (double)(c->nval - 1) * (sizeof(VT) + 4) + 4.0 * (c->N + 1) +
-                                  (double)c->N * (sizeof(XT) + sizeof(YT)));
+  const double bytes = (double)(c->nval - 1) * (sizeof(VT) + 4) + 4.0 * (c->N + 1) +
+                       (double)c->N * (sizeof(XT) + sizeof(YT));
   if (c->dim == 2) {
     constexpr int G = 16;
     const int blocks = (int)(((long long)c->N * G + 255) / 256);
-    k_spmv_msr<VT, XT, YT, G><<<blocks, 256, 0, c->stream>>>(c->N, a, c->d_bindx, x, y);
+    gm_launch(c, GM_K_SPMV, bytes, k_spmv_msr<VT, XT, YT, G>, dim3(blocks), dim3(256), 0, c->stream,
+              c->N, a, c->d_bindx, x, y);
   } else {
     constexpr int G = 64;
     const int blocks = (int)(((long long)c->N * G + 255) / 256);
-    k_spmv_msr<VT, XT, YT, G><<<blocks, 256, 0, c->stream>>>(c->N, a, c->d_bindx, x, y);
+    gm_launch(c, GM_K_SPMV, bytes, k_spmv_msr<VT, XT, YT, G>, dim3(blocks), dim3(256), 0, c->stream,
+              c->N, a, c->d_bindx, x, y);
   }
-  gm_prof_end(c, GM_K_SPMV);
 }
 
 int gm_spmv_launch(gm_ctx *c, const double *val, const double *x, double *y) {
@@ -678,12 +682,12 @@ int gm_ilu_factor(gm_ctx *c, const double *val, const float *val32, int prec) {
   for (int col = 0; col < P.ncol; col++) {
     const int p0 = P.colour_ptr[col], cnt = P.colour_ptr[col + 1] - p0;
     if (!cnt) continue;
-    gm_prof_begin(c, GM_K_ILU_FACT);
     if (prec == GM_PREC_MIXED)
-      k_ilu_factor<float><<<cnt, ILU_FT, 0, c->stream>>>(S, P.d_lu32, p0, c->d_flag);
+      gm_launch(c, GM_K_ILU_FACT, 0.0, k_ilu_factor<float>, dim3(cnt), dim3(ILU_FT), 0, c->stream,
+                S, P.d_lu32, p0, c->d_flag);
     else
-      k_ilu_factor<double><<<cnt, ILU_FT, 0, c->stream>>>(S, P.d_lu, p0, c->d_flag);
-    gm_prof_end(c, GM_K_ILU_FACT);
+      gm_launch(c, GM_K_ILU_FACT, 0.0, k_ilu_factor<double>, dim3(cnt), dim3(ILU_FT), 0, c->stream,
+                S, P.d_lu, p0, c->d_flag);
   }
   GM_HIP(hipGetLastError());
   int flag = 0;
@@ -706,24 +710,26 @@ static void ilu_apply(gm_ctx *c, const VT *lu, VT *ywork, const RT *r, ZT *z) {
     if (!cnt) continue;
     // L values + column ids, row pointers, diagonals, node ranges, level lists,
     // permutation, input (gathered) and output vectors
-    gm_prof_begin(c, GM_K_ILU_LOWER, (double)P.col_nnzL[col] * (sizeof(VT) + 4) +
-                                         (double)P.col_rows[col] * (6 * 4 + sizeof(RT) + sizeof(VT)));
+    const double bytes = (double)P.col_nnzL[col] * (sizeof(VT) + 4) +
+                         (double)P.col_rows[col] * (6 * 4 + sizeof(RT) + sizeof(VT));
     if (c->dim == 2)
-      k_ilu_lower<VT, RT, VT, 16><<<cnt, ILU_NT, 0, c->stream>>>(SL, lu, p0, P.d_perm, r, ywork);
+      gm_launch(c, GM_K_ILU_LOWER, bytes, k_ilu_lower<VT, RT, VT, 16>, dim3(cnt), dim3(ILU_NT), 0,
+                c->stream, SL, lu, p0, P.d_perm, r, ywork);
     else
-      k_ilu_lower<VT, RT, VT, 64><<<cnt, ILU_NT, 0, c->stream>>>(SL, lu, p0, P.d_perm, r, ywork);
-    gm_prof_end(c, GM_K_ILU_LOWER);
+      gm_launch(c, GM_K_ILU_LOWER, bytes, k_ilu_lower<VT, RT, VT, 64>, dim3(cnt), dim3(ILU_NT), 0,
+                c->stream, SL, lu, p0, P.d_perm, r, ywork);
   }
   for (int col = P.ncol - 1; col >= 0; col--) {
     const int p0 = P.colour_ptr[col], cnt = P.colour_ptr[col + 1] - p0;
     if (!cnt) continue;
-    gm_prof_begin(c, GM_K_ILU_UPPER, (double)P.col_nnzU[col] * (sizeof(VT) + 4) +
-                                         (double)P.col_rows[col] * (6 * 4 + 2 * sizeof(VT) + sizeof(ZT)));
+    const double bytes = (double)P.col_nnzU[col] * (sizeof(VT) + 4) +
+                         (double)P.col_rows[col] * (6 * 4 + 2 * sizeof(VT) + sizeof(ZT));
     if (c->dim == 2)
-      k_ilu_upper<VT, VT, ZT, 16><<<cnt, ILU_NT, 0, c->stream>>>(SU, lu, p0, P.d_perm, ywork, z);
+      gm_launch(c, GM_K_ILU_UPPER, bytes, k_ilu_upper<VT, VT, ZT, 16>, dim3(cnt), dim3(ILU_NT), 0,
+                c->stream, SU, lu, p0, P.d_perm, ywork, z);
     else
-      k_ilu_upper<VT, VT, ZT, 64><<<cnt, ILU_NT, 0, c->stream>>>(SU, lu, p0, P.d_perm, ywork, z);
-    gm_prof_end(c, GM_K_ILU_UPPER);
+      gm_launch(c, GM_K_ILU_UPPER, bytes, k_ilu_upper<VT, VT, ZT, 64>, dim3(cnt), dim3(ILU_NT), 0,
+                c->stream, SU, lu, p0, P.d_perm, ywork, z);
   }
 }
 
@@ -912,16 +918,15 @@ static int gmres_t(gm_ctx *c, const VT *a, const VT *lu, VT *ywork, const double
       BT *vj = V + (size_t)j * N, *vn = V + (size_t)(j + 1) * N;
       precond(vj, z);
       spmv<VT, BT, BT>(c, a, z, w);
-      gm_prof_begin(c, GM_K_MDOT, (double)(j + 2) * N * sizeof(BT));
-      k_mdot<BT><<<KB, RED_THREADS, 0, c->stream>>>(N, j + 1, V, w, K.d_part);
-      gm_prof_end(c, GM_K_MDOT);
+      gm_launch(c, GM_K_MDOT, (double)(j + 2) * N * sizeof(BT), k_mdot<BT>, dim3(KB),
+                dim3(RED_THREADS), 0, c->stream, N, j + 1, (const BT *)V, (const BT *)w, K.d_part);
       GM_HIP(hipGetLastError());
       if ((rc = sum_parts(c, K.d_part, j + 2, KB, h.data()))) return rc;
       const double ww0 = h[j + 1];
       GM_HIP(hipMemcpyAsync(K.d_h, h.data(), (j + 1) * sizeof(double), hipMemcpyHostToDevice, c->stream));
-      gm_prof_begin(c, GM_K_ORTH, (double)(j + 3) * N * sizeof(BT));
-      k_orth_update<BT><<<KB, RED_THREADS, 0, c->stream>>>(N, j + 1, V, K.d_h, w, K.d_part);
-      gm_prof_end(c, GM_K_ORTH);
+      gm_launch(c, GM_K_ORTH, (double)(j + 3) * N * sizeof(BT), k_orth_update<BT>, dim3(KB),
+                dim3(RED_THREADS), 0, c->stream, N, j + 1, (const BT *)V, (const double *)K.d_h, w,
+                K.d_part);
       double ww;
       if ((rc = sum_parts(c, K.d_part, 1, KB, &ww))) return rc;
       if (ww < eta2 * ww0) {

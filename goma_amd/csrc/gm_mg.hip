@@ -18,7 +18,7 @@
 #include <cmath>
 #include <vector>
 
-#include "gm_internal.h"
+#include "gm_launch.h"
 
 template <int DIM, int EN>
 struct Loc {
@@ -349,9 +349,8 @@ static void launch_invert(gm_ctx *prof, gm_ctx *lc, hipStream_t s, const VT *val
   constexpr int n = Loc<DIM, EN>::N;
   // gathered entries + map + inverses written
   const double bytes = (double)lc->num_elems * n * n * (sizeof(VT) + 4 + sizeof(AT));
-  gm_prof_begin(prof, GM_K_VANKA_INV, bytes);
-  k_vanka_invert<DIM, EN, VT, AT, NT><<<lc->num_elems, NT, 0, s>>>(level_dev(lc), val, ainv);
-  gm_prof_end(prof, GM_K_VANKA_INV);
+  gm_launch(prof, GM_K_VANKA_INV, bytes, k_vanka_invert<DIM, EN, VT, AT, NT>, dim3(lc->num_elems),
+            dim3(NT), 0, s, level_dev(lc), val, ainv);
 }
 
 template <typename VT>
@@ -450,9 +449,8 @@ static void resid(gm_ctx *lc, hipStream_t s, const VT *a, const BT *x, const BT 
   const int blocks = (int)(((long long)lc->N * G + 255) / 256);
   const double bytes = (double)(lc->nval - 1) * (sizeof(VT) + 4) + 4.0 * (lc->N + 1) +
                        3.0 * lc->N * sizeof(BT);
-  gm_prof_begin(g_prof_ctx, GM_K_MG_RESID, bytes);
-  k_mg_resid<VT, BT, G><<<blocks, 256, 0, s>>>(lc->N, a, lc->d_bindx, x, b, r);
-  gm_prof_end(g_prof_ctx, GM_K_MG_RESID);
+  gm_launch(g_prof_ctx, GM_K_MG_RESID, bytes, k_mg_resid<VT, BT, G>, dim3(blocks), dim3(256), 0, s,
+            lc->N, a, lc->d_bindx, x, b, r);
 }
 
 template <typename VT, typename BT>
@@ -464,10 +462,9 @@ static void resid_rows(gm_ctx *lc, hipStream_t s, int col, const VT *a, const BT
   const int blocks = (int)(((long long)nr * G + 255) / 256);
   const double nnz_rows = (double)lc->mg_rows_nnz[col];
   const double bytes = nnz_rows * (sizeof(VT) + 4) + nr * (4.0 * 3 + 3.0 * sizeof(BT));
-  gm_prof_begin(g_prof_ctx, GM_K_MG_RESID, bytes);
-  k_mg_resid_rows<VT, BT, G><<<blocks, 256, 0, s>>>(nr, lc->d_mg_rows + lc->mg_rows_ptr[col], a,
-                                                    lc->d_bindx, x, b, r);
-  gm_prof_end(g_prof_ctx, GM_K_MG_RESID);
+  gm_launch(g_prof_ctx, GM_K_MG_RESID, bytes, k_mg_resid_rows<VT, BT, G>, dim3(blocks), dim3(256),
+            0, s, nr, (const int *)(lc->d_mg_rows + lc->mg_rows_ptr[col]), a, (const int *)lc->d_bindx,
+            x, b, r);
 }
 
 template <int DIM, int EN, typename AT, typename BT>
@@ -480,10 +477,9 @@ static void vanka_colour(gm_ctx *lc, hipStream_t s, int col, const AT *ainv, con
   // inverses, element lists/connectivity, residual gathered, x read and written
   const double bytes = (double)cnt * (n * n * sizeof(AT) + 4.0 * (Loc<DIM, EN>::NVE + 2) +
                                       n * 3.0 * sizeof(BT));
-  gm_prof_begin(g_prof_ctx, GM_K_VANKA, bytes);
-  k_vanka_update<DIM, EN, AT, BT, NT><<<cnt, NT, 0, s>>>(level_dev(lc), lc->d_ecol_elems + lc->ecol_ptr[col],
-                                                         ainv, r, x, omega);
-  gm_prof_end(g_prof_ctx, GM_K_VANKA);
+  gm_launch(g_prof_ctx, GM_K_VANKA, bytes, k_vanka_update<DIM, EN, AT, BT, NT>, dim3(cnt), dim3(NT),
+            0, s, level_dev(lc), (const int *)(lc->d_ecol_elems + lc->ecol_ptr[col]), ainv, r, x,
+            omega);
 }
 
 template <typename VT, typename BT>
@@ -509,9 +505,8 @@ static void transfer(gm_ctx *c, hipStream_t s, int nrows, const int *rp, const i
                      const signed char *mask, BT *y, int acc) {
   const double bytes = (double)nnz * 12.0 + 4.0 * (nrows + 1) + (double)ncols * sizeof(BT) +
                        (double)nrows * (1 + (acc ? 2 : 1) * sizeof(BT));
-  gm_prof_begin(c, GM_K_TRANSFER, bytes);
-  k_csr_apply<BT><<<1024, 256, 0, s>>>(nrows, rp, ci, cv, v, sc, mask, y, acc);
-  gm_prof_end(c, GM_K_TRANSFER);
+  gm_launch(c, GM_K_TRANSFER, bytes, k_csr_apply<BT>, dim3(1024), dim3(256), 0, s, nrows, rp, ci, cv,
+            v, sc, mask, y, acc);
 }
 
 template <typename VT, typename BT>

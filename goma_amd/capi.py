@@ -30,7 +30,7 @@ GM_PC_ILU0 = 1
 GM_PC_MG = 2
 KERNEL_NAMES = ["k_fill", "k_spmv_msr", "k_ilu_lower", "k_ilu_upper", "k_ilu_factor", "k_mdot",
                 "k_orth_update", "k_mg_resid", "k_vanka_update", "k_csr_apply", "k_vanka_invert",
-                "k_fill(residual only)"]
+                "k_fill(residual only)", "k_mg_resid_rows"]
 
 
 class GmMesh(C.Structure):
@@ -70,7 +70,8 @@ class GmNewtonReport(C.Structure):
                 ("lin_relres", C.c_double * MAX_REPORT), ("damp", C.c_double * MAX_REPORT),
                 ("t_assemble", C.c_double),
                 ("t_scale", C.c_double), ("t_precond", C.c_double), ("t_solve", C.c_double),
-                ("t_update", C.c_double), ("t_total", C.c_double)]
+                ("t_update", C.c_double), ("t_total", C.c_double), ("fell_back", C.c_int),
+                ("discarded_its", C.c_int)]
 
     def summary(self) -> dict:
         n = min(self.newton_its, MAX_REPORT)
@@ -80,7 +81,29 @@ class GmNewtonReport(C.Structure):
                     unscaled_L2=list(self.unscaled_L2[:n]), damp=list(self.damp[:n]),
                     t_assemble=self.t_assemble,
                     t_scale=self.t_scale, t_precond=self.t_precond, t_solve=self.t_solve,
-                    t_update=self.t_update, t_total=self.t_total)
+                    t_update=self.t_update, t_total=self.t_total, fell_back=bool(self.fell_back),
+                    discarded_its=self.discarded_its)
+
+
+MLP_MAX_IN = 8
+MLP_MAX_OUT = 8
+
+
+class GmMlpShape(C.Structure):
+    _fields_ = [("n_in", C.c_int), ("n_out", C.c_int), ("hidden", C.c_int), ("width", C.c_int),
+                ("seed", C.c_ulonglong), ("in_shift", C.c_double * MLP_MAX_IN),
+                ("in_scale", C.c_double * MLP_MAX_IN)]
+
+
+class GmMlpTrainOpts(C.Structure):
+    _fields_ = [("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_double),
+                ("lr_final", C.c_double), ("beta1", C.c_double), ("beta2", C.c_double),
+                ("eps", C.c_double), ("seed", C.c_ulonglong), ("verbose", C.c_int)]
+
+
+class GmMlpQuery(C.Structure):
+    _fields_ = [("param", C.c_double * MLP_MAX_IN), ("out_scale", C.c_double * MLP_MAX_OUT),
+                ("out_shift", C.c_double * MLP_MAX_OUT)]
 
 
 # every exported symbol of include/gm_capi.h
@@ -91,7 +114,10 @@ EXPORTS = [
     "gm_matrix_fill_full_dev", "gm_matrix_fill_full", "gm_row_sum_scale_dev", "gm_msr_matvec_dev",
     "gm_gmres_dev", "gm_ilu0_factor_dev", "gm_solve_nonlinear_dev", "gm_solve_nonlinear",
     "gm_default_solver_opts", "gm_prof_enable", "gm_prof_get", "gm_mg_add_level",
-    "gm_mg_num_levels", "gm_mg_setup_dev",
+    "gm_mg_num_levels", "gm_mg_setup_dev", "gm_solve_nonlinear_guess_dev",
+    "gm_mlp_create", "gm_mlp_destroy", "gm_mlp_num_params", "gm_mlp_get_params",
+    "gm_mlp_set_params", "gm_mlp_default_train_opts", "gm_mlp_set_data", "gm_mlp_train",
+    "gm_mlp_loss_grad", "gm_mlp_predict_dev", "gm_mlp_predict", "gm_mlp_guess_dev",
 ]
 
 _lib = None
@@ -135,6 +161,23 @@ def lib():
     L.gm_mg_setup_dev.argtypes = [vp, vp, vp, vp]
     L.gm_prof_enable.argtypes = [vp, ip]
     L.gm_prof_get.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.gm_solve_nonlinear_guess_dev.argtypes = [vp, vp, vp, C.POINTER(GmSolverOpts), ip,
+                                               C.c_double, C.POINTER(GmNewtonReport)]
+    L.gm_mlp_create.argtypes = [C.POINTER(GmMlpShape), C.POINTER(vp)]
+    L.gm_mlp_destroy.argtypes = [vp]
+    L.gm_mlp_destroy.restype = None
+    L.gm_mlp_num_params.argtypes = [vp]
+    L.gm_mlp_num_params.restype = C.c_longlong
+    L.gm_mlp_get_params.argtypes = [vp, vp]
+    L.gm_mlp_set_params.argtypes = [vp, vp]
+    L.gm_mlp_default_train_opts.argtypes = [C.POINTER(GmMlpTrainOpts)]
+    L.gm_mlp_default_train_opts.restype = None
+    L.gm_mlp_set_data.argtypes = [vp, C.c_longlong, vp, vp, vp]
+    L.gm_mlp_train.argtypes = [vp, C.POINTER(GmMlpTrainOpts), vp]
+    L.gm_mlp_loss_grad.argtypes = [vp, ip, vp, vp, vp, C.POINTER(C.c_double), vp]
+    L.gm_mlp_predict_dev.argtypes = [vp, C.c_longlong, vp, vp]
+    L.gm_mlp_predict.argtypes = [vp, C.c_longlong, vp, vp]
+    L.gm_mlp_guess_dev.argtypes = [vp, vp, C.POINTER(GmMlpQuery), C.POINTER(GmMlpQuery), vp, vp]
     _lib = L
     return L
 
@@ -274,6 +317,17 @@ class Context:
         _check(rc, "gm_solve_nonlinear_dev")
         return rc, rep
 
+    def solve_guess_dev(self, x_ptr: int, x_prev_ptr: int, watch_its: int = 3, drop: float = 10.0,
+                        **opts):
+        """Newton from the guess in x, falling back to x_prev when the scaled
+        residual has not dropped by ``drop`` within ``watch_its`` passes."""
+        o = solver_opts(**opts)
+        rep = GmNewtonReport()
+        rc = lib().gm_solve_nonlinear_guess_dev(self.h, x_ptr, x_prev_ptr, C.byref(o), watch_its,
+                                                drop, C.byref(rep))
+        _check(rc, "gm_solve_nonlinear_guess_dev")
+        return rc, rep
+
     def solve_host(self, x: np.ndarray, **opts):
         o = solver_opts(**opts)
         rep = GmNewtonReport()
@@ -326,3 +380,96 @@ class Context:
                                bytes_per_launch=by / cnt if by else None,
                                GBps=(by / (ms * 1e-3) / 1e9) if (by and ms) else None)
         return out
+
+
+def mlp_query(param, out_scale, out_shift) -> GmMlpQuery:
+    q = GmMlpQuery()
+    for i, v in enumerate(param):
+        q.param[i] = float(v)
+    for i, v in enumerate(out_scale):
+        q.out_scale[i] = float(v)
+    for i, v in enumerate(out_shift):
+        q.out_shift[i] = float(v)
+    return q
+
+
+class Mlp:
+    """A gm_mlp: the learned-initial-guess network on the GPU."""
+
+    def __init__(self, n_in, n_out, hidden=4, width=256, seed=20261202, in_shift=None,
+                 in_scale=None):
+        s = GmMlpShape()
+        s.n_in, s.n_out, s.hidden, s.width, s.seed = n_in, n_out, hidden, width, seed
+        for i in range(n_in):
+            s.in_shift[i] = 0.0 if in_shift is None else float(in_shift[i])
+            s.in_scale[i] = 1.0 if in_scale is None else float(in_scale[i])
+        self.shape = s
+        self.n_in, self.n_out = n_in, n_out
+        h = C.c_void_p()
+        _check(lib().gm_mlp_create(C.byref(s), C.byref(h)), "gm_mlp_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gm_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def num_params(self) -> int:
+        return int(lib().gm_mlp_num_params(self.h))
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.num_params, np.float32)
+        _check(lib().gm_mlp_get_params(self.h, p.ctypes.data), "gm_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.num_params
+        _check(lib().gm_mlp_set_params(self.h, p.ctypes.data), "gm_mlp_set_params")
+
+    def set_data(self, x, t, m):
+        self._data = [np.ascontiguousarray(a, dtype=np.float32) for a in (x, t, m)]
+        x, t, m = self._data
+        _check(lib().gm_mlp_set_data(self.h, x.shape[0], x.ctypes.data, t.ctypes.data,
+                                     m.ctypes.data), "gm_mlp_set_data")
+        self._data = None
+
+    def train(self, **kw) -> np.ndarray:
+        o = GmMlpTrainOpts()
+        lib().gm_mlp_default_train_opts(C.byref(o))
+        for k, v in kw.items():
+            if not hasattr(o, k):
+                raise KeyError(k)
+            setattr(o, k, v)
+        hist = np.zeros(max(o.epochs, 1))
+        _check(lib().gm_mlp_train(self.h, C.byref(o), hist.ctypes.data), "gm_mlp_train")
+        return hist[: o.epochs]
+
+    def loss_grad(self, x, t, m):
+        x, t, m = (np.ascontiguousarray(a, dtype=np.float32) for a in (x, t, m))
+        g = np.zeros(self.num_params, np.float32)
+        loss = C.c_double(0.0)
+        _check(lib().gm_mlp_loss_grad(self.h, x.shape[0], x.ctypes.data, t.ctypes.data,
+                                      m.ctypes.data, C.byref(loss), g.ctypes.data),
+               "gm_mlp_loss_grad")
+        return loss.value, g
+
+    def predict(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], self.n_out), np.float32)
+        _check(lib().gm_mlp_predict(self.h, x.shape[0], x.ctypes.data, y.ctypes.data),
+               "gm_mlp_predict")
+        return y
+
+    def guess_dev(self, ctx: Context, query: GmMlpQuery, x_ptr: int, prev_query=None,
+                  x_prev_ptr=None):
+        _check(lib().gm_mlp_guess_dev(self.h, ctx.h, C.byref(query),
+                                      C.byref(prev_query) if prev_query is not None else None,
+                                      x_prev_ptr, x_ptr), "gm_mlp_guess_dev")

@@ -1,5 +1,6 @@
 // C ABI of libgm (include/gm_capi.h): context life cycle, the
 // matrix_fill_full and solve_nonlinear_problem replacements, kernel timing.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -38,14 +39,24 @@ static void prof_fold(gm_ctx *c, int id) {
   c->ev_used[id] = 0;
 }
 
-// Every launch adds its bytes to the family's total; one launch in prof_every
-// (and the family's first) is timed by events that its own dispatch stamps,
-// and the reported duration scales the sampled time by bytes / sampled bytes.
+// Every launch adds its bytes to the family's total.  With prof_every = k > 1
+// a pseudo-random one launch in k (and the family's first) is timed, by
+// events that its own dispatch stamps; a fixed stride could alias with the
+// repeating launch pattern of a V-cycle and bias the average.
+static bool sampled_launch(long long seen, int id, int every) {
+  if (every <= 1 || seen == 1) return true;
+  uint64_t z = (uint64_t)seen * 0x9e3779b97f4a7c15ull + (uint64_t)id * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  return z % (uint64_t)every == 0;
+}
+
 bool gm_prof_take(gm_ctx *c, int id, double bytes, hipEvent_t *b, hipEvent_t *e) {
   if (!c->prof) return false;
   c->prof_bytes[id] += bytes;
   c->prof_seen[id]++;
-  if ((c->prof_seen[id] - 1) % c->prof_every != 0) return false;
+  if (!sampled_launch(c->prof_seen[id], id, c->prof_every)) return false;
   if (c->ev_used[id] == kProfPool) prof_fold(c, id);
   const int k = c->ev_used[id];
   if ((int)c->ev_b[id].size() <= k) {
@@ -245,6 +256,8 @@ extern "C" int gm_set_physics(gm_ctx *c, const gm_physics *ph) {
     return GM_ERR_ARG;
   }
   c->phys = *ph;
+  // the multigrid levels re-assemble the same physics on coarser meshes
+  for (MgLevel &L : c->mg) L.ctx->phys = *ph;
   return GM_OK;
 }
 
@@ -384,9 +397,11 @@ static int backtrack(gm_ctx *c, double *x, const gm_solver_opts *o, double L2, d
 
 // solve_nonlinear_problem, steady, NLS_FULL_STEP or NLS_BACKTRACK
 // (src/mm_sol_nonlinear.c:1108-2913)
-extern "C" int gm_solve_nonlinear_dev(gm_ctx *c, double *x, const gm_solver_opts *o,
-                                      gm_newton_report *rep) {
-  if (!c || !x || !o || !rep) return GM_ERR_ARG;
+// watch_its > 0: stop early (*stalled = true) when, at pass watch_its, no
+// pass since the first has brought ||D^-1 R||_2 below its first value / drop.
+static int solve_impl(gm_ctx *c, double *x, const gm_solver_opts *o, gm_newton_report *rep,
+                      int watch_its, double drop, bool *stalled) {
+  *stalled = false;
   std::memset(rep, 0, sizeof(*rep));
   const int prec = o->precision;
   int rc = ensure_newton_buffers(c, prec);
@@ -419,6 +434,16 @@ extern "C" int gm_solve_nonlinear_dev(gm_ctx *c, double *x, const gm_solver_opts
       rep->unscaled_L2[inewton] = un;
     }
     if (inewton > 0 && L2 < o->eps_resid && Loo < o->eps_resid) converged = o->eps_corr > 1.0;
+    if (watch_its > 0 && inewton == watch_its && !converged && watch_its < GM_MAX_REPORT) {
+      double best = L2;
+      for (int i = 1; i < inewton; i++) best = std::min(best, rep->resid_L2[i]);
+      if (!(best * drop <= rep->resid_L2[0])) {
+        *stalled = true;
+        rep->newton_its = inewton;
+        rep->t_total = now_s() - t0;
+        return inewton;
+      }
+    }
     if (!std::isfinite(L2) || !std::isfinite(Loo)) {
       gm_set_error("non-finite residual norm");
       rep->newton_its = inewton;
@@ -478,6 +503,37 @@ extern "C" int gm_solve_nonlinear_dev(gm_ctx *c, double *x, const gm_solver_opts
   return inewton;
 }
 
+extern "C" int gm_solve_nonlinear_dev(gm_ctx *c, double *x, const gm_solver_opts *o,
+                                      gm_newton_report *rep) {
+  if (!c || !x || !o || !rep) return GM_ERR_ARG;
+  bool stalled = false;
+  return solve_impl(c, x, o, rep, 0, 0.0, &stalled);
+}
+
+extern "C" int gm_solve_nonlinear_guess_dev(gm_ctx *c, double *x, const double *x_prev,
+                                            const gm_solver_opts *o, int watch_its, double drop,
+                                            gm_newton_report *rep) {
+  if (!c || !x || !x_prev || !o || !rep || watch_its < 1 || watch_its >= GM_MAX_REPORT ||
+      !(drop > 1.0))
+    return GM_ERR_ARG;
+  bool stalled = false;
+  int rc = solve_impl(c, x, o, rep, watch_its, drop, &stalled);
+  if (rc < 0 || !stalled) return rc;
+  const gm_newton_report first = *rep;
+  GM_HIP(hipMemcpyAsync(x, x_prev, (size_t)c->N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  rc = solve_impl(c, x, o, rep, 0, 0.0, &stalled);
+  rep->fell_back = 1;
+  rep->discarded_its = first.newton_its;
+  rep->lin_its_total += first.lin_its_total;
+  rep->t_assemble += first.t_assemble;
+  rep->t_scale += first.t_scale;
+  rep->t_precond += first.t_precond;
+  rep->t_solve += first.t_solve;
+  rep->t_update += first.t_update;
+  rep->t_total += first.t_total;
+  return rc;
+}
+
 extern "C" int gm_solve_nonlinear(gm_ctx *c, double *x, const gm_solver_opts *o,
                                   gm_newton_report *rep) {
   if (!c || !x) return GM_ERR_ARG;
@@ -507,11 +563,10 @@ extern "C" int gm_prof_enable(gm_ctx *c, int on) {
 extern "C" int gm_prof_get(gm_ctx *c, int id, int *count, double *ms, double *bytes) {
   if (!c || id < 0 || id >= GM_K_COUNT || !count || !ms) return GM_ERR_ARG;
   prof_fold(c, id);
-  // time of all launches = sampled time x (all bytes / sampled bytes) when the
-  // launches differ in size, x (launches / sampled) otherwise
+  // time of all launches = sampled time x launches / sampled launches (the
+  // sample is pseudo-random, so this is unbiased over launches of any size)
   const long long seen = c->prof_seen[id], sampled = c->prof_count[id];
-  double scale = sampled ? (double)seen / sampled : 0.0;
-  if (c->prof_sampled_bytes[id] > 0) scale = c->prof_bytes[id] / c->prof_sampled_bytes[id];
+  const double scale = sampled ? (double)seen / sampled : 0.0;
   *count = (int)seen;
   *ms = c->prof_ms[id] * scale;
   if (bytes) *bytes = c->prof_bytes[id];

@@ -500,6 +500,20 @@ static void lagrange1(int node, double s, double &v, double &dv) {
   }
 }
 
+// Q2 basis (QUAD9 / HEX27, Goma local node order) at one reference point.
+void gm_q2_basis(int dim, const double *xi, double *phi) {
+  const int nve = dim == 2 ? 9 : 27;
+  for (int k = 0; k < nve; k++) {
+    const signed char *pos = dim == 2 ? kQuad9[k] : kHex27[k];
+    double v = 1.0, a, da;
+    for (int d = 0; d < dim; d++) {
+      lagrange1(pos[d], xi[d], a, da);
+      v *= a;
+    }
+    phi[k] = v;
+  }
+}
+
 static std::vector<double> make_tables(int dim) {
   const int nve = dim == 2 ? 9 : 27, nq = nve, np = dim + 1;
   std::vector<double> t((size_t)nq * nve + (size_t)nq * nve * dim + (size_t)nq * np + nq);

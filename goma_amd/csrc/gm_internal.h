@@ -153,6 +153,7 @@ int gm_default_bins(const gm_ctx *c);
 // device set-up and kernels (gm_fill.hip)
 int gm_fill_setup_device(gm_ctx *c);
 int gm_fill_launch(gm_ctx *c, const double *x, double *resid, double *val, float *val32);
+void gm_q2_basis(int dim, const double *xi, double *phi);  // host: Q2 basis at one reference point
 int gm_fill_launch_ex(gm_ctx *c, const double *x, double *resid, double *val, float *val32,
                       int resid_only);
 

@@ -462,7 +462,7 @@ static void resid_rows(gm_ctx *lc, hipStream_t s, int col, const VT *a, const BT
   const int blocks = (int)(((long long)nr * G + 255) / 256);
   const double nnz_rows = (double)lc->mg_rows_nnz[col];
   const double bytes = nnz_rows * (sizeof(VT) + 4) + nr * (4.0 * 3 + 3.0 * sizeof(BT));
-  gm_launch(g_prof_ctx, GM_K_MG_RESID, bytes, k_mg_resid_rows<VT, BT, G>, dim3(blocks), dim3(256),
+  gm_launch(g_prof_ctx, GM_K_MG_RESID_ROWS, bytes, k_mg_resid_rows<VT, BT, G>, dim3(blocks), dim3(256),
             0, s, nr, (const int *)(lc->d_mg_rows + lc->mg_rows_ptr[col]), a, (const int *)lc->d_bindx,
             x, b, r);
 }

@@ -19,7 +19,7 @@ rho*g*(-beta*(T-Tref)), src/mm_std_models.c:256-284):
 """
 from __future__ import annotations
 
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
@@ -167,6 +167,22 @@ def config(idx: int, n: int | None = None, Ra: float | None = None) -> Problem:
     if idx in (3, 4):
         return natural_convection(n=n or 128, Ra=Ra or 1e3, carreau=True, jitter=0.15)
     raise ValueError(idx)
+
+
+def physics_at(prob: Problem, Ra: float) -> Physics:
+    """The problem's material constants with the Rayleigh number changed (the
+    continuation parameter; Goma changes the gravity component of the momentum
+    source through update_parameterC, src/ac_update_parameter.c:48)."""
+    m = prob.meta
+    if "Ra" not in m or not prob.phys.energy:
+        raise ValueError(f"{prob.name} has no Rayleigh number")
+    if "Re" in m:
+        g = -Ra / (m["Re"] ** 2 * m["Pr"])
+    elif m.get("carreau"):
+        g = -Ra
+    else:
+        g = -Ra * m["Pr"]
+    return replace(prob.phys, g=(0.0, g, 0.0))
 
 
 def sweep_ra(count: int = 64, lo: float = 1e3, hi: float = 1e6) -> np.ndarray:

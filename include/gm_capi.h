@@ -111,6 +111,8 @@ typedef struct gm_newton_report {
   double lin_relres[GM_MAX_REPORT];
   double damp[GM_MAX_REPORT];        /* step length taken at each pass */
   double t_assemble, t_scale, t_precond, t_solve, t_update, t_total; /* seconds */
+  int fell_back;       /* gm_solve_nonlinear_guess_dev: 1 if the guess was abandoned */
+  int discarded_its;   /* Newton passes spent on the abandoned guess */
 } gm_newton_report;
 
 typedef struct gm_ctx gm_ctx;
@@ -135,7 +137,8 @@ int gm_copy_elem_colours(const gm_ctx *ctx, int *colour);    /* [num_elems] */
 int gm_copy_patch_permutation(gm_ctx *ctx, int patch_bins, int *perm, int *colour_ptr);
 
 /* Replaces the material constants (continuation parameter update,
- * update_parameterC, src/ac_update_parameter.c:48). */
+ * update_parameterC, src/ac_update_parameter.c:48), on this context and on
+ * the multigrid levels attached to it. */
 int gm_set_physics(gm_ctx *ctx, const gm_physics *phys);
 
 /* matrix_fill_full (src/mm_fill.c:158-312): accumulates the residual and the
@@ -166,6 +169,19 @@ int gm_solve_nonlinear_dev(gm_ctx *ctx, double *x_dev, const gm_solver_opts *opt
                            gm_newton_report *rep);
 int gm_solve_nonlinear(gm_ctx *ctx, double *x, const gm_solver_opts *opt, gm_newton_report *rep);
 
+/* Newton from a predicted guess, with the previous-step guess as fallback (no
+ * counterpart in the reference: Goma's zeroth-order continuation,
+ * src/ac_conti.c:760-787, starts each step from the previous solution, which
+ * is what x_prev_dev is here).  x_dev holds the predicted guess on entry and
+ * the solution on exit.  If the scaled residual ||D^-1 R||_2 has not dropped
+ * below its value at the guess divided by `drop` within `watch_its` Newton
+ * passes, the guess is abandoned and Newton restarts from x_prev_dev;
+ * rep->fell_back and rep->discarded_its record that, and the timings cover
+ * both attempts. */
+int gm_solve_nonlinear_guess_dev(gm_ctx *ctx, double *x_dev, const double *x_prev_dev,
+                                 const gm_solver_opts *opt, int watch_its, double drop,
+                                 gm_newton_report *rep);
+
 void gm_default_solver_opts(gm_solver_opts *opt);
 
 /* Multigrid hierarchy for GM_PC_MG (the reference has no counterpart: Goma's
@@ -182,11 +198,11 @@ int gm_mg_num_levels(const gm_ctx *ctx);
  * the level-0 matrix val_dev (row-sum scaled by scale_dev, or NULL). */
 int gm_mg_setup_dev(gm_ctx *ctx, const double *x_dev, const double *val_dev, const double *scale_dev);
 
-/* Kernel timing: when enabled, every launch of the kernels below is
- * bracketed by HIP events on the context's stream (the stream the kernels
- * run on); gm_prof_get returns the launch count, the summed duration and the
- * summed algorithmic HBM bytes (compulsory traffic, each array counted once
- * per launch) of those launches. */
+/* Kernel timing: when enabled, launches of the kernels below carry HIP
+ * events that their own dispatch stamps (hipExtLaunchKernel), so a duration
+ * is the kernel's execution alone; gm_prof_get returns the launch count, the
+ * summed duration and the summed algorithmic HBM bytes (compulsory traffic,
+ * each array counted once per launch) of those launches. */
 #define GM_K_FILL 0       /* k_fill: assembly + scatter of one element colour */
 #define GM_K_SPMV 1       /* k_spmv_msr: MSR matvec */
 #define GM_K_ILU_LOWER 2  /* k_ilu_lower: forward sweep of one patch colour */
@@ -194,16 +210,92 @@ int gm_mg_setup_dev(gm_ctx *ctx, const double *x_dev, const double *val_dev, con
 #define GM_K_ILU_FACT 4   /* k_ilu_factor: ILU(0) of one patch colour */
 #define GM_K_MDOT 5       /* k_mdot: Gram-Schmidt projections */
 #define GM_K_ORTH 6       /* k_orth_update: Gram-Schmidt update */
-#define GM_K_MG_RESID 7   /* k_mg_resid: multigrid residual r = b - A x, one level */
+#define GM_K_MG_RESID 7   /* k_mg_resid: multigrid residual r = b - A x, all rows of one level */
 #define GM_K_VANKA 8      /* k_vanka_update: element-block updates of one colour */
 #define GM_K_TRANSFER 9   /* k_csr_apply: restriction or prolongation */
 #define GM_K_VANKA_INV 10 /* k_vanka_invert: element-block inverses of one level */
 #define GM_K_FILL_RESID 11 /* k_fill, residual only (line-search trial), one colour */
-#define GM_K_COUNT 12
-/* on = 0: off; 1: time every launch; k > 1: time one launch in k of each
- * kernel (low overhead), durations scaled to all launches by their bytes. */
+#define GM_K_MG_RESID_ROWS 12 /* k_mg_resid_rows: smoother residual on one element colour's rows */
+#define GM_K_COUNT 13
+/* on = 0: off; 1: time every launch; k > 1: time a pseudo-random one launch
+ * in k of each kernel (lower overhead), durations scaled to all launches by
+ * the launch count. */
 int gm_prof_enable(gm_ctx *ctx, int on);
 int gm_prof_get(gm_ctx *ctx, int kernel_id, int *count, double *total_ms, double *bytes);
+
+/* ------------------------------------------------------------------------
+ * Learned initial guess: a coordinate-plus-parameter MLP trained on the
+ * solutions of a continuation sweep (no counterpart in the reference; it
+ * replaces the previous-step predictor of src/ac_conti.c:760-787 when it does
+ * better, and gm_solve_nonlinear_guess_dev falls back to that predictor).
+ *
+ *   u = (raw - in_shift) * in_scale,  a_{l+1} = tanh(W_l a_l + b_l),
+ *   y = W_H a_H + b_H,  loss = sum mask (y - t)^2 / (npts * n_out).
+ * GEMMs run on the matrix cores in BF16 with FP32 accumulation (inputs,
+ * weights, activations and back-propagated deltas rounded to BF16; biases,
+ * outputs, gradients, Adam state and master weights FP32).
+ * Parameters are exchanged unpadded, layer by layer: W_l [N_l][K_l]
+ * row-major, then b_l [N_l].
+ * ------------------------------------------------------------------------ */
+typedef struct gm_mlp gm_mlp;
+
+#define GM_MLP_MAX_IN 8
+#define GM_MLP_MAX_OUT 8
+typedef struct gm_mlp_shape {
+  int n_in;             /* inputs: coordinates then parameters, <= GM_MLP_MAX_IN */
+  int n_out;            /* outputs, <= GM_MLP_MAX_OUT */
+  int hidden;           /* hidden layers (>= 1) */
+  int width;            /* hidden width, a multiple of 64 */
+  unsigned long long seed; /* Glorot-uniform initialisation */
+  double in_shift[GM_MLP_MAX_IN], in_scale[GM_MLP_MAX_IN];
+} gm_mlp_shape;
+
+typedef struct gm_mlp_train_opts {
+  int epochs;
+  int batch;            /* points per step, multiple of 64 */
+  double lr, lr_final;  /* Adam step size, cosine-decayed from lr to lr_final */
+  double beta1, beta2, eps;
+  unsigned long long seed; /* point shuffling */
+  int verbose;
+} gm_mlp_train_opts;
+
+int gm_mlp_create(const gm_mlp_shape *shape, gm_mlp **out);
+void gm_mlp_destroy(gm_mlp *m);
+long long gm_mlp_num_params(const gm_mlp *m);
+int gm_mlp_get_params(gm_mlp *m, float *params);
+int gm_mlp_set_params(gm_mlp *m, const float *params); /* also resets the Adam state */
+void gm_mlp_default_train_opts(gm_mlp_train_opts *o);
+/* Training set (host arrays, copied to HBM): in [npts][n_in],
+ * targ and mask [npts][n_out]. */
+int gm_mlp_set_data(gm_mlp *m, long long npts, const float *in, const float *targ,
+                    const float *mask);
+/* Adam on the training set; loss_per_epoch[epochs] (may be NULL) receives
+ * the mean batch loss of each epoch. */
+int gm_mlp_train(gm_mlp *m, const gm_mlp_train_opts *o, double *loss_per_epoch);
+/* Loss and gradient (unpadded parameter layout) of one batch, no update:
+ * host arrays, npts <= 65536. */
+int gm_mlp_loss_grad(gm_mlp *m, int npts, const float *in, const float *targ, const float *mask,
+                     double *loss, float *grad);
+/* y [npts][n_out] = network(in [npts][n_in]); device or host arrays. */
+int gm_mlp_predict_dev(gm_mlp *m, long long npts, const float *in_dev, float *out_dev);
+int gm_mlp_predict(gm_mlp *m, long long npts, const float *in, float *out);
+
+/* One query of the network for a context: the parameter inputs that follow
+ * the coordinates, and the affine map from network outputs to fields. */
+typedef struct gm_mlp_query {
+  double param[GM_MLP_MAX_IN];     /* inputs n_in - dim .. n_in - 1 */
+  double out_scale[GM_MLP_MAX_OUT];
+  double out_shift[GM_MLP_MAX_OUT];
+} gm_mlp_query;
+/* Initial guess in the context's unknown layout.  Outputs are
+ * (U1, U2[, U3], P[, T]): nodal values at the mesh nodes; each element's P1
+ * coefficients {P0, P_s, P_t[, P_u]} from the pressure output at the element
+ * centroid and at reference points +-1/2 along each axis (central
+ * differences, exact for pressure fields linear along those lines).
+ * Dirichlet unknowns take their prescribed values.  With q_prev and
+ * x_prev_dev (delta form): x = x_prev + G(q) - G(q_prev). */
+int gm_mlp_guess_dev(gm_mlp *m, gm_ctx *ctx, const gm_mlp_query *q, const gm_mlp_query *q_prev,
+                     const double *x_prev_dev, double *x_dev);
 
 #ifdef __cplusplus
 }

@@ -29,7 +29,7 @@
  * GMRES(m)/ILU(0) restate the AztecOO behaviour Goma selects
  * (src/sl_util.c:392-960); AztecOO itself is not in the reference tree.
  *
- * Parity pins are listed in DESIGN.md and exercised by tests/test_oracle_*.py.
+ * The pins of this restatement are the tests in tests/test_oracle_*.py.
  */
 #ifndef GM_ORACLE_H
 #define GM_ORACLE_H

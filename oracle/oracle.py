@@ -52,10 +52,16 @@ class OmReport(C.Structure):
                 ("t_solve", C.c_double), ("t_total", C.c_double), ("damp", C.c_double * 64)]
 
 
+class OmlpShape(C.Structure):
+    _fields_ = [("n_in", C.c_int), ("n_out", C.c_int), ("hidden", C.c_int), ("width", C.c_int)]
+
+
 def build(force: bool = False) -> str:
     """Compile the oracle with its Makefile (gcc, plain C)."""
-    src = os.path.join(HERE, "gm_oracle.c")
-    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
+    srcs = [os.path.join(HERE, f) for f in ("gm_oracle.c", "gm_oracle.h", "gm_mlp_oracle.c",
+                                            "gm_mlp_oracle.h", "Makefile")]
+    newest = max(os.path.getmtime(s) for s in srcs)
+    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < newest:
         subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"], check=True)
     return LIB_PATH
 
@@ -97,6 +103,15 @@ def lib():
         L.om_gmres_perm.argtypes = [C.c_int, _dp, _ip, _dp, _dp, C.c_void_p, C.c_void_p, C.c_int,
                                     C.c_int, C.c_double, C.POINTER(C.c_double)]
         L.om_newton.argtypes = [C.c_void_p, _dp, C.POINTER(OmSolverOpts), C.POINTER(OmReport)]
+        _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+        L.omlp_num_params.restype = C.c_longlong
+        L.omlp_num_params.argtypes = [C.POINTER(OmlpShape)]
+        L.omlp_bf16.restype = C.c_float
+        L.omlp_bf16.argtypes = [C.c_float]
+        L.omlp_forward.argtypes = [C.POINTER(OmlpShape), _fp, _dp, _dp, C.c_int, C.c_longlong, _fp,
+                                   _dp]
+        L.omlp_loss_grad.argtypes = [C.POINTER(OmlpShape), _fp, _dp, _dp, C.c_int, C.c_longlong,
+                                     _fp, _fp, _fp, C.POINTER(C.c_double), _dp]
         _lib = L
     return _lib
 
@@ -232,3 +247,45 @@ def gauss(dim):
         lib().om_gauss(dim, q, v)
         out[q] = v
     return out
+
+
+class MlpOracle:
+    """The learned-guess network on the CPU (gm_mlp_oracle.h)."""
+
+    def __init__(self, n_in, n_out, hidden, width, in_shift=None, in_scale=None):
+        self.shape = OmlpShape(n_in, n_out, hidden, width)
+        self.n_in, self.n_out = n_in, n_out
+        self.in_shift = np.ascontiguousarray(
+            np.zeros(n_in) if in_shift is None else in_shift, dtype=np.float64)
+        self.in_scale = np.ascontiguousarray(
+            np.ones(n_in) if in_scale is None else in_scale, dtype=np.float64)
+
+    @property
+    def num_params(self):
+        return int(lib().omlp_num_params(C.byref(self.shape)))
+
+    def forward(self, params, x, quant=True):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], self.n_out))
+        rc = lib().omlp_forward(C.byref(self.shape), np.ascontiguousarray(params, np.float32),
+                                self.in_shift, self.in_scale, int(quant), x.shape[0], x, y)
+        assert rc == 0, rc
+        return y
+
+    def loss_grad(self, params, x, t, m, quant=True):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        g = np.zeros(self.num_params)
+        loss = C.c_double(0.0)
+        rc = lib().omlp_loss_grad(C.byref(self.shape), np.ascontiguousarray(params, np.float32),
+                                  self.in_shift, self.in_scale, int(quant), x.shape[0], x,
+                                  np.ascontiguousarray(t, np.float32),
+                                  np.ascontiguousarray(m, np.float32), C.byref(loss), g)
+        assert rc == 0, rc
+        return loss.value, g
+
+
+def bf16(x):
+    """Round float32 values to bfloat16 (nearest even), as float32."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)

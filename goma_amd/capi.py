@@ -30,7 +30,7 @@ GM_PC_ILU0 = 1
 GM_PC_MG = 2
 KERNEL_NAMES = ["k_fill", "k_spmv_msr", "k_ilu_lower", "k_ilu_upper", "k_ilu_factor", "k_mdot",
                 "k_orth_update", "k_mg_resid", "k_vanka_update", "k_csr_apply", "k_vanka_invert",
-                "k_fill(residual only)", "k_mg_resid_rows"]
+                "k_fill(residual only)", "k_mg_resid_rows", "k_dense_gemv"]
 
 
 class GmMesh(C.Structure):

@@ -90,7 +90,7 @@ extern "C" void gm_default_solver_opts(gm_solver_opts *o) {
   o->verbose = 0;
   o->mg_nu = 1;
   o->mg_omega = 0.7;
-  o->mg_coarse_sweeps = 20;
+  o->mg_coarse_sweeps = 0;
   o->line_search = 0;
   o->min_damp = 0.05;
 }
@@ -460,7 +460,9 @@ static int solve_impl(gm_ctx *c, double *x, const gm_solver_opts *o, gm_newton_r
         if ((rc = gm_ilu_plan(c, bins))) return rc;
         if ((rc = gm_ilu_factor(c, c->d_val, c->d_val32, prec))) return rc;
       } else if (o->precond == GM_PC_MG) {
-        if ((rc = gm_mg_setup(c, x, c->d_val, c->d_val32, prec, c->d_scale))) return rc;
+        if ((rc = gm_mg_setup(c, x, c->d_val, c->d_val32, prec, c->d_scale,
+                              o->mg_coarse_sweeps <= 0)))
+          return rc;
         GM_HIP(hipStreamSynchronize(c->stream));
       }
       double td = now_s();

@@ -85,7 +85,14 @@ struct MgLevel {
   void *d_ainv = nullptr;     // element-block inverses
   int vec_prec = -1, ainv_prec = -1, val_prec = -1;
   int nnz = 0;                // entries of P
+  // coarsest level only: dense FP64 inverse for the exact coarse solve
+  double *d_dense = nullptr;  // [n][n]
+  double *d_pcol = nullptr, *d_prow = nullptr;  // pivot column / row of one elimination step
+  int *d_perm = nullptr;      // row interchange of each step
+  int dense_n = 0;
 };
+
+#define GM_MG_DIRECT_MAX 4096 /* largest coarsest level solved exactly */
 
 struct gm_ctx {
   int dim = 0, nve = 0, np = 0, nv = 0, num_nodes = 0, num_elems = 0, energy = 0;
@@ -179,7 +186,7 @@ void gm_free_linalg(gm_ctx *c);
 
 // multigrid (gm_mg.hip)
 int gm_mg_setup(gm_ctx *c, const double *x, const double *val, const float *val32, int prec,
-                const double *scale);
+                const double *scale, int coarse_direct);
 int gm_mg_apply_d(gm_ctx *c, const double *a0, const double *v, double *z, const gm_solver_opts *o);
 int gm_mg_apply_f(gm_ctx *c, const float *a0, const double *v, double *z, const gm_solver_opts *o);
 void gm_mg_free(gm_ctx *c);

@@ -183,6 +183,107 @@ __global__ __launch_bounds__(256) void k_mg_resid_rows(int nrows, const int *__r
   if (lane == 0) r[row] = (BT)((AT)b[row] - s - (AT)a[row] * (AT)x[row]);
 }
 
+// ---- exact coarse solve: dense inverse by in-place Gauss-Jordan with
+// partial pivoting (two launches per pivot), applied as a dense GEMV.
+template <typename VT>
+__global__ void k_msr_to_dense(int n, const VT *__restrict__ a, const int *__restrict__ bindx,
+                               double *__restrict__ D) {
+  const int row = blockIdx.x;
+  for (int k = bindx[row] + threadIdx.x; k < bindx[row + 1]; k += blockDim.x)
+    D[(size_t)row * n + bindx[k]] = (double)a[k];
+  if (threadIdx.x == 0) D[(size_t)row * n + row] = (double)a[row];
+}
+
+// step k, one block: pivot row p = argmax_{i>=k} |D[i][k]| (first on ties),
+// swap rows k and p, keep copies of the pivot column and row
+__global__ __launch_bounds__(256) void k_gj_pivot(int n, int k, double *__restrict__ D,
+                                                  int *__restrict__ perm, double *__restrict__ pcol,
+                                                  double *__restrict__ prow, int *__restrict__ flag) {
+  __shared__ double sv[256];
+  __shared__ int si[256];
+  const int tid = threadIdx.x;
+  double best = -1.0;
+  int bi = n;
+  for (int i = k + tid; i < n; i += 256) {
+    const double v = fabs(D[(size_t)i * n + k]);
+    if (v > best) {
+      best = v;
+      bi = i;
+    }
+  }
+  sv[tid] = best;
+  si[tid] = bi;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      const double v = sv[tid + s];
+      const int i = si[tid + s];
+      if (v > sv[tid] || (v == sv[tid] && i < si[tid])) {
+        sv[tid] = v;
+        si[tid] = i;
+      }
+    }
+    __syncthreads();
+  }
+  const int p = si[0];
+  if (tid == 0) {
+    perm[k] = p;
+    if (!(sv[0] > 0.0)) flag[3] = 1;  // singular coarse matrix
+  }
+  if (p != k)
+    for (int j = tid; j < n; j += 256) {
+      const double t = D[(size_t)k * n + j];
+      D[(size_t)k * n + j] = D[(size_t)p * n + j];
+      D[(size_t)p * n + j] = t;
+    }
+  __syncthreads();
+  for (int i = tid; i < n; i += 256) {
+    pcol[i] = D[(size_t)i * n + k];
+    prow[i] = D[(size_t)k * n + i];
+  }
+}
+
+__global__ void k_gj_elim(int n, int k, double *__restrict__ D, const double *__restrict__ pcol,
+                          const double *__restrict__ prow) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)n * n) return;
+  const int i = (int)(idx / n), j = (int)(idx - (long long)i * n);
+  const double a = prow[k];
+  if (a == 0.0) return;  // singular: flagged by k_gj_pivot
+  if (i == k) D[idx] = j == k ? 1.0 / a : prow[j] / a;
+  else if (j == k) D[idx] = -pcol[i] / a;
+  else D[idx] -= pcol[i] * prow[j] / a;
+}
+
+// inverse of A = inverse of (P A) with the columns swapped back, last step first
+__global__ void k_gj_unpermute(int n, const int *__restrict__ perm, double *__restrict__ D) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double *r = D + (size_t)i * n;
+  for (int k = n - 1; k >= 0; k--) {
+    const int p = perm[k];
+    if (p != k) {
+      const double t = r[k];
+      r[k] = r[p];
+      r[p] = t;
+    }
+  }
+}
+
+// x = D b, one wave per row
+template <typename BT>
+__global__ __launch_bounds__(256) void k_dense_gemv(int n, const double *__restrict__ D,
+                                                    const BT *__restrict__ b, BT *__restrict__ x) {
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const double *r = D + (size_t)row * n;
+  double s = 0.0;
+  for (int j = lane; j < n; j += 64) s += r[j] * (double)b[j];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) x[row] = (BT)s;
+}
+
 // x_e += omega * A_e^-1 r_e for the elements of one colour
 template <int DIM, int EN, typename AT, typename BT, int NT>
 __global__ __launch_bounds__(NT) void k_vanka_update(LevelDev L, const int *__restrict__ elems,
@@ -393,10 +494,58 @@ static int build_colour_rows(gm_ctx *lc) {
   return upv(&lc->d_mg_rows, rows.data(), rows.size());
 }
 
+// Dense inverse of the coarsest level's Jacobian (exact coarse solve).
+static int coarse_inverse(gm_ctx *c, MgLevel &L, int prec) {
+  gm_ctx *lc = L.ctx;
+  const int n = lc->N;
+  if (n > GM_MG_DIRECT_MAX) {
+    gm_set_error("coarsest multigrid level has %d unknowns (exact solve up to %d): coarsen "
+                 "further or set mg_coarse_sweeps > 0", n, GM_MG_DIRECT_MAX);
+    return GM_ERR_ARG;
+  }
+  if (L.dense_n != n) {
+    for (void *p : {(void *)L.d_dense, (void *)L.d_pcol, (void *)L.d_prow, (void *)L.d_perm})
+      if (p) (void)hipFree(p);
+    L.d_dense = L.d_pcol = L.d_prow = nullptr;
+    L.d_perm = nullptr;
+    L.dense_n = 0;
+    GM_HIP(hipMalloc(&L.d_dense, (size_t)n * n * sizeof(double)));
+    GM_HIP(hipMalloc(&L.d_pcol, (size_t)n * sizeof(double)));
+    GM_HIP(hipMalloc(&L.d_prow, (size_t)n * sizeof(double)));
+    GM_HIP(hipMalloc(&L.d_perm, (size_t)n * sizeof(int)));
+    L.dense_n = n;
+  }
+  hipStream_t s = c->stream;
+  const long long nn = (long long)n * n;
+  k_set<double><<<1024, 256, 0, s>>>(L.d_dense, nn, 0.0);
+  if (prec == GM_PREC_MIXED)
+    k_msr_to_dense<float><<<n, 64, 0, s>>>(n, (const float *)L.d_val, lc->d_bindx, L.d_dense);
+  else
+    k_msr_to_dense<double><<<n, 64, 0, s>>>(n, (const double *)L.d_val, lc->d_bindx, L.d_dense);
+  GM_HIP(hipMemsetAsync(c->d_flag + 3, 0, sizeof(int), s));
+  const int eb = (int)((nn + 255) / 256);
+  for (int k = 0; k < n; k++) {
+    k_gj_pivot<<<1, 256, 0, s>>>(n, k, L.d_dense, L.d_perm, L.d_pcol, L.d_prow, c->d_flag);
+    k_gj_elim<<<eb, 256, 0, s>>>(n, k, L.d_dense, L.d_pcol, L.d_prow);
+  }
+  k_gj_unpermute<<<(n + 255) / 256, 256, 0, s>>>(n, L.d_perm, L.d_dense);
+  GM_HIP(hipGetLastError());
+  int flag = 0;
+  GM_HIP(hipMemcpyAsync(&flag, c->d_flag + 3, sizeof(int), hipMemcpyDeviceToHost, s));
+  GM_HIP(hipStreamSynchronize(s));
+  if (flag) {
+    gm_set_error("singular Jacobian on the coarsest multigrid level");
+    return GM_ERR_PIVOT;
+  }
+  return GM_OK;
+}
+
 // Per Newton step: coarse Jacobians at the injected state, local inverses on
-// every level.  val / val32 is the level-0 (scaled) matrix, x the state.
+// every level, and the dense inverse of the coarsest level when it is solved
+// exactly (coarse_direct).  val / val32 is the level-0 (scaled) matrix, x the
+// state.
 int gm_mg_setup(gm_ctx *c, const double *x, const double *val, const float *val32, int prec,
-                const double *scale) {
+                const double *scale, int coarse_direct) {
   c->mg_scale = scale;
   if (c->mg.empty()) {
     gm_set_error("multigrid preconditioner requested but no coarse level attached");
@@ -438,6 +587,7 @@ int gm_mg_setup(gm_ctx *c, const double *x, const double *val, const float *val3
       invert<double>(c, L.ctx, c->stream, (const double *)L.d_val, (double *)L.d_ainv);
   }
   GM_HIP(hipGetLastError());
+  if (coarse_direct && (rc = coarse_inverse(c, c->mg.back(), prec))) return rc;
   return GM_OK;
 }
 
@@ -516,7 +666,7 @@ static void vcycle(gm_ctx *c, const VT *a0, const BT *v, BT *z, const gm_solver_
   const int nl = (int)c->mg.size();
   const int nu = o->mg_nu > 0 ? o->mg_nu : 1;
   const double om = o->mg_omega > 0 ? o->mg_omega : 1.0;
-  const int cs = o->mg_coarse_sweeps > 0 ? o->mg_coarse_sweeps : 20;
+  const int cs = o->mg_coarse_sweeps;  // 0: exact solve with the dense inverse
   auto lv_ctx = [&](int l) { return l == 0 ? c : c->mg[l - 1].ctx; };
   auto lv = [&](int l) -> MgLevel & { return l == 0 ? c->mg0 : c->mg[l - 1]; };
   auto amat = [&](int l) -> const VT * { return l == 0 ? a0 : (const VT *)c->mg[l - 1].d_val; };
@@ -528,8 +678,16 @@ static void vcycle(gm_ctx *c, const VT *a0, const BT *v, BT *z, const gm_solver_
     MgLevel &L = lv(l);
     BT *x = l == 0 ? x0 : (BT *)L.d_x;
     const BT *b = l == 0 ? b0 : (const BT *)L.d_b;
+    if (l == nl && cs <= 0 && L.dense_n == lc->N) {
+      const int n = lc->N;
+      gm_launch(c, GM_K_MG_DENSE, (double)n * n * 8.0 + 2.0 * n * sizeof(BT), k_dense_gemv<BT>,
+                dim3((unsigned)(((long long)n * 64 + 255) / 256)), dim3(256), 0, s, n,
+                (const double *)L.d_dense, b, x);
+      break;
+    }
     k_set<BT><<<1024, 256, 0, s>>>(x, lc->N, (BT)0);
-    smooth<VT, BT>(lc, s, amat(l), (const VT *)L.d_ainv, b, x, (BT *)L.d_r, l == nl ? cs : nu, om);
+    smooth<VT, BT>(lc, s, amat(l), (const VT *)L.d_ainv, b, x, (BT *)L.d_r,
+                   l == nl ? (cs > 0 ? cs : 20) : nu, om);
     if (l == nl) break;
     resid<VT, BT>(lc, s, amat(l), x, b, (BT *)L.d_r);
     MgLevel &C = c->mg[l];
@@ -564,7 +722,8 @@ int gm_mg_apply_f(gm_ctx *c, const float *a0, const double *v, double *z, const 
 void gm_mg_free(gm_ctx *c) {
   auto fr = [](MgLevel &L) {
     void *p[] = {L.d_Prow, L.d_Pcol, L.d_Pval, L.d_Rrow, L.d_Rcol, L.d_Rval, L.d_inject,
-                 L.d_state, L.d_rdummy, L.d_x, L.d_b, L.d_r, L.d_val, L.d_ainv};
+                 L.d_state, L.d_rdummy, L.d_x, L.d_b, L.d_r, L.d_val, L.d_ainv,
+                 L.d_dense, L.d_pcol, L.d_prow, L.d_perm};
     for (void *q : p)
       if (q) (void)hipFree(q);
   };
@@ -576,7 +735,7 @@ void gm_mg_free(gm_ctx *c) {
 
 extern "C" int gm_mg_setup_dev(gm_ctx *c, const double *x, const double *val, const double *scale) {
   if (!c || !x || !val) return GM_ERR_ARG;
-  int rc = gm_mg_setup(c, x, val, nullptr, GM_PREC_FP64, scale);
+  int rc = gm_mg_setup(c, x, val, nullptr, GM_PREC_FP64, scale, 1);
   if (rc) return rc;
   GM_HIP(hipStreamSynchronize(c->stream));
   return GM_OK;

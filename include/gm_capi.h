@@ -94,7 +94,8 @@ typedef struct gm_solver_opts {
   int verbose;         /* 1: one line per Newton step on stderr */
   int mg_nu;           /* multigrid: smoothing sweeps before and after the coarse correction */
   double mg_omega;     /* multigrid: damping of the element-block updates */
-  int mg_coarse_sweeps;/* multigrid: sweeps on the coarsest level */
+  int mg_coarse_sweeps;/* multigrid: 0 = exact solve on the coarsest level (dense
+                          inverse, at most 4096 unknowns); k > 0 = k smoothing sweeps */
   int line_search;     /* 0: full step (NLS_FULL_STEP); 1: backtracking (NLS_BACKTRACK) */
   double min_damp;     /* Line_Search_Minimum_Damping */
 } gm_solver_opts;
@@ -216,7 +217,8 @@ int gm_mg_setup_dev(gm_ctx *ctx, const double *x_dev, const double *val_dev, con
 #define GM_K_VANKA_INV 10 /* k_vanka_invert: element-block inverses of one level */
 #define GM_K_FILL_RESID 11 /* k_fill, residual only (line-search trial), one colour */
 #define GM_K_MG_RESID_ROWS 12 /* k_mg_resid_rows: smoother residual on one element colour's rows */
-#define GM_K_COUNT 13
+#define GM_K_MG_DENSE 13  /* k_dense_gemv: exact solve on the coarsest level (dense inverse) */
+#define GM_K_COUNT 14
 /* on = 0: off; 1: time every launch; k > 1: time a pseudo-random one launch
  * in k of each kernel (lower overhead), durations scaled to all launches by
  * the launch count. */

@@ -161,8 +161,8 @@ def test_learned_guess_beats_previous_step_guess():
     print("[case] learned vs previous-step guess", flush=True)
     prob = problems.natural_convection(n=12, Ra=1e3, carreau=True, jitter=0.15)
     ctx = capi.Context(prob)
-    opts = dict(precond=capi.GM_PC_ILU0, patch_bins=2, line_search=1, max_newton=30, kspace=50,
-                max_its=300)
+    ctx.attach_multigrid(3)
+    opts = dict(precond=capi.GM_PC_MG, line_search=1, max_newton=30, kspace=50, max_its=300)
     ras = problems.sweep_ra(12, 1e3, 1e5)
     steps, sols = continuation.sweep(ctx, prob, ras, opts)
     assert all(s.converged for s in steps)

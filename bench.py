@@ -7,11 +7,14 @@ mesh, matrix graph and all work arrays already resident in HBM.  The default
 workload is 2D Boussinesq natural convection on a 256x256 QUAD9 mesh
 (Q2/P1 velocity-pressure plus Q2 temperature, Pr = 0.71, Ra = 1e5), FP64.
 
-Prints one JSON line.  Besides the headline value it reports, for the kernel
-that takes the most time, its algorithmic bytes per launch over its average
-launch duration (HIP events around every launch on the library's stream) as
-a fraction of the MI355X HBM peak, and the oracle's FP64 CPU path timed on a
-bounded sample of the same workload.
+Prints one JSON line.  Besides the headline value (wall time of the timed
+solves, no instrumentation) it reports, for the kernel that takes the most
+time, its algorithmic bytes per launch over its average launch duration as a
+fraction of the MI355X HBM peak.  Durations come from a repeat of the timed
+solve in which every launch carries HIP events stamped by its own dispatch on
+the library's stream.  It also reports the mixed-precision solve of the same
+workload with its nodal error against the FP64 solution, and the oracle's
+FP64 CPU path timed on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -123,6 +126,155 @@ def cpu_baseline(prob, gpu_counts: dict, budget_s: float):
                 ilu_factor_s=t_fact)
 
 
+def field_groups(ctx, prob):
+    """Unknown indices of the velocity, temperature and pressure fields."""
+    from goma_amd.mesh import CENTROID_LOCAL
+
+    m = prob.mesh
+    d, en = m.dim, int(prob.phys.energy)
+    fu = ctx.first_unknown()[: m.num_nodes].astype(np.int64)
+    groups = {"velocity": np.concatenate([fu + k for k in range(d)])}
+    if en:
+        groups["temperature"] = fu + d
+    cen = fu[m.connect[:, CENTROID_LOCAL[d]]] + d + en
+    groups["pressure"] = np.concatenate([cen + k for k in range(d + 1)])
+    return groups
+
+
+def mixed_variant(ctx, prob, x_fp64, opts, torch):
+    """The mixed-precision solve of the same workload at the same tolerance,
+    and its nodal error per field against the FP64 solution."""
+    ref = x_fp64.cpu().numpy()
+    xm = torch.zeros_like(x_fp64)
+    o = dict(opts)
+    o["precision"] = 1
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rc, rep = ctx.solve_dev(xm.data_ptr(), **o)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    xh = xm.cpu().numpy()
+    err = {k: float(np.max(np.abs(xh[i] - ref[i])) / max(np.max(np.abs(ref[i])), 1e-300))
+           for k, i in field_groups(ctx, prob).items()}
+    print(f"[bench] mixed precision: newton {rep.newton_its} gmres {rep.lin_its_total} {dt:.2f}s "
+          f"nodal error {err}", file=sys.stderr, flush=True)
+    return {"fp64_s": None, "mixed_s": round(dt, 4), "mixed_converged": bool(rep.converged),
+            "mixed_newton_its": rep.newton_its, "mixed_gmres_its": rep.lin_its_total,
+            "mixed_nodal_error_vs_fp64": err}
+
+
+def mlp_flops_per_point(n_in, n_out, hidden, width):
+    """Algorithmic (unpadded) forward flops of the network per point."""
+    dims = [n_in] + [width] * hidden + [n_out]
+    return 2 * sum(a * b for a, b in zip(dims[:-1], dims[1:]))
+
+
+def sweep_bench(args, torch, capi):
+    """Carreau-Yasuda Ra continuation and the learned initial guess.
+
+    Sweep: zeroth-order continuation over 64 log-spaced Ra in [1e3, 1e6] on
+    the 128^2 jittered mesh (previous-step guess, FP64 Newton to 1e-10).
+    Network: trained on the GPU on the converged sweep solutions.
+    Evaluation: at the 32 held-out geometric midpoints of sweep intervals
+    (0,1), (2,3), ..., Newton from the previous-step guess (the solution at
+    the interval's lower end) against Newton from the learned guess (delta
+    form from the same solution, with fallback), same tolerance.
+    """
+    from goma_amd import continuation, learned, problems
+
+    n = args.n or 128
+    prob = problems.natural_convection(n=n, Ra=1e3, carreau=True, jitter=0.15)
+    ctx = capi.Context(prob)
+    nlev = ctx.attach_multigrid(args.mg_coarse)
+    prec = 1 if args.precision == "mixed" else 0
+    # shear-thinning flow weakens the V(1,1) cycle as Ra grows: V(2,2), larger Krylov space
+    opts = dict(kspace=max(args.kspace, 100), max_its=max(args.max_its, 400), precision=prec,
+                max_newton=25, precond=2, mg_nu=max(args.mg_nu, 2), mg_omega=args.mg_omega,
+                line_search=args.line_search)
+    log = lambda s: print(s, file=sys.stderr, flush=True)
+    ras = problems.sweep_ra(args.sweep_count, 1e3, 1e6)
+    log(f"[sweep] {prob.name}: N={ctx.N} levels {nlev}, {len(ras)} Ra values")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    steps, sols = continuation.sweep(ctx, prob, ras, opts, log=log if args.verbose else None)
+    torch.cuda.synchronize()
+    t_sweep = time.perf_counter() - t0
+    ok = 0
+    while ok < len(steps) and steps[ok].converged:
+        ok += 1
+    log(f"[sweep] {ok}/{len(ras)} steps converged, {t_sweep:.1f}s, "
+        f"{sum(s.newton_its for s in steps)} Newton passes")
+    ras_ok, sols_ok = ras[:ok], sols[:ok]
+    lg = learned.LearnedGuess(prob, ctx, ras_ok, sols_ok, hidden=4, width=256, seed=20261202)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = lg.train(epochs=args.epochs, batch=8192, lr=2e-3, lr_final=2e-5, seed=1,
+                    verbose=1 if args.verbose else 0)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    fpp = mlp_flops_per_point(3, 4, 4, 256)
+    train_flops = 3.0 * fpp * lg.num_points * args.epochs  # forward + backward (2x forward)
+    log(f"[sweep] trained on {lg.num_points} points x {args.epochs} epochs in {t_train:.1f}s "
+        f"({train_flops / t_train / 1e12:.1f} TFLOP/s), loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    nb = lg.set_basis(sols_ok)
+    log(f"[sweep] snapshot subspace of dimension {nb} for the delta-form guess")
+    rows = []
+    for k in range(0, ok - 1, 2):
+        Ra = float(np.sqrt(ras_ok[k] * ras_ok[k + 1]))
+        ctx.set_physics(problems.physics_at(prob, Ra))
+        xprev = torch.tensor(sols_ok[k], device="cuda")
+        x = xprev.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, rb = ctx.solve_dev(x.data_ptr(), **opts)
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - t0
+        x = torch.empty_like(xprev)
+        t0 = time.perf_counter()
+        lg.guess(x.data_ptr(), Ra, float(ras_ok[k]), xprev.data_ptr())
+        _, rl = ctx.solve_guess_dev(x.data_ptr(), xprev.data_ptr(), 3, 10.0, **opts)
+        torch.cuda.synchronize()
+        tl = time.perf_counter() - t0
+        rows.append(dict(Ra=Ra, prev_its=rb.newton_its, prev_s=tb, prev_ok=bool(rb.converged),
+                         prev_r0=rb.resid_L2[0], learned_its=rl.newton_its + rl.discarded_its,
+                         learned_s=tl, learned_ok=bool(rl.converged), fell_back=bool(rl.fell_back),
+                         learned_r0=rl.resid_L2[0]))
+        if args.verbose:
+            log(f"[eval] Ra={Ra:.4g} previous-step {rb.newton_its} its {tb:.3f}s r0 {rb.resid_L2[0]:.2e}"
+                f" | learned {rl.newton_its}+{rl.discarded_its} its {tl:.3f}s "
+                f"r0 {rl.resid_L2[0]:.2e}{' FELL BACK' if rl.fell_back else ''}")
+    prev_s = sum(r["prev_s"] for r in rows)
+    learn_s = sum(r["learned_s"] for r in rows)
+    out = {
+        "metric": "s per held-out converged Newton solve (‖R‖<1e-10): learned vs previous-step guess",
+        "value": round(learn_s / max(len(rows), 1), 4), "unit": "s", "n_gpus": 1,
+        "higher_is_better": False, "dtype": "f64" if prec == 0 else "f32/f64",
+        "data": "synthetic",
+        "config": {"workload": f"{prob.name} Carreau-Yasuda natural convection, jitter 0.15h, "
+                               f"Ra sweep [1e3, 1e6] x {len(ras)}", "unknowns": ctx.N,
+                   "held_out": len(rows), "network": "3-256-256-256-256-4 tanh, BF16 MFMA"},
+        "sweep": {"converged_steps": ok, "seconds": round(t_sweep, 2),
+                  "newton_its": sum(s.newton_its for s in steps),
+                  "retries": sum(s.retries for s in steps)},
+        "subspace_dim": nb,
+        "training": {"points": lg.num_points, "epochs": args.epochs, "seconds": round(t_train, 2),
+                     "algorithmic_TFLOPs": round(train_flops / t_train / 1e12, 2),
+                     "bf16_peak_TFLOPs": 2500.0, "loss_first": hist[0], "loss_last": hist[-1]},
+        "previous_step": {"newton_its": sum(r["prev_its"] for r in rows),
+                          "seconds": round(prev_s, 3),
+                          "all_converged": all(r["prev_ok"] for r in rows)},
+        "learned": {"newton_its": sum(r["learned_its"] for r in rows), "seconds": round(learn_s, 3),
+                    "fallbacks": sum(r["fell_back"] for r in rows),
+                    "all_converged": all(r["learned_ok"] for r in rows)},
+        "speedup": round(prev_s / learn_s, 3) if learn_s > 0 else None,
+        "per_ra": rows if args.verbose else None,
+    }
+    print(json.dumps(out), flush=True)
+    lg.close()
+    ctx.close()
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,9 +292,13 @@ def main(argv=None):
     ap.add_argument("--line-search", type=int, default=1, help="1: Goma's backtracking line search")
     ap.add_argument("--mg-coarse", type=int, default=2, help="coarsest elements per side")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--prof-every", type=int, default=8,
-                    help="HIP events around one launch in this many, per kernel")
+    ap.add_argument("--prof-every", type=int, default=1,
+                    help="time one launch in this many (pseudo-random), per kernel")
+    ap.add_argument("--variants", type=int, default=1,
+                    help="1: also time the mixed-precision solve and report its nodal error")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sweep-count", type=int, default=64, help="sweep workload: Ra values")
+    ap.add_argument("--epochs", type=int, default=60, help="sweep workload: training epochs")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -156,6 +312,8 @@ def main(argv=None):
 
     from goma_amd import capi
 
+    if args.workload == "sweep":
+        return sweep_bench(args, torch, capi)
     prec = 1 if args.precision == "mixed" else 0
     prob, params = make_problem(args.workload, args.n)
     t0 = time.perf_counter()
@@ -177,7 +335,6 @@ def main(argv=None):
         rc, rep = ctx.solve_dev(x.data_ptr(), **opts)
         print(f"[bench] warmup {w}: newton {rep.newton_its} gmres {rep.lin_its_total} "
               f"{rep.t_total:.2f}s", file=sys.stderr, flush=True)
-    ctx.prof_enable(True, every=args.prof_every)
     reps = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -189,10 +346,25 @@ def main(argv=None):
               f"converged {bool(rep.converged)} {rep.t_total:.2f}s", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    kern = ctx.prof_all()
-    ctx.prof_enable(False)
     if not all(r["converged"] for r in reps):
         print("[bench] WARNING: a step did not converge", file=sys.stderr, flush=True)
+    # Kernel timing on a repeat of the timed solve (same inputs, same launches):
+    # every launch carries events stamped by its own dispatch.  Kept out of the
+    # timed steps because per-launch events slow this launch-bound solve.
+    ctx.prof_enable(True, every=args.prof_every)
+    x.zero_()
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    rc, prep = ctx.solve_dev(x.data_ptr(), **opts)
+    torch.cuda.synchronize()
+    t_prof = time.perf_counter() - tp
+    kern = ctx.prof_all()
+    ctx.prof_enable(False)
+    print(f"[bench] profiled repeat: newton {prep.newton_its} gmres {prep.lin_its_total} "
+          f"{t_prof:.2f}s", file=sys.stderr, flush=True)
+    variants = None
+    if args.variants and prec == 0:
+        variants = mixed_variant(ctx, prob, x, opts, torch)
 
     ms_step = elapsed / args.steps * 1e3
     n_ecol = capi.lib().gm_num_elem_colours(ctx.h)
@@ -237,7 +409,12 @@ def main(argv=None):
         "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kern.items()},
         "roofline": roof,
+        "kernel_timing": ("HIP events stamped by each launch's own dispatch, on the library "
+                          f"stream, over a repeat of the timed solve ({t_prof:.2f} s with events)"),
     }
+    if variants:
+        variants["fp64_s"] = out["value"]
+        out["variants"] = variants
     if not args.no_cpu_baseline:
         nres = kern.get("k_fill(residual only)", {}).get("launches", 0) // max(n_ecol, 1)
         counts = dict(assemblies=assemblies // max(args.steps, 1),

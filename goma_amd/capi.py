@@ -118,6 +118,7 @@ EXPORTS = [
     "gm_mlp_create", "gm_mlp_destroy", "gm_mlp_num_params", "gm_mlp_get_params",
     "gm_mlp_set_params", "gm_mlp_default_train_opts", "gm_mlp_set_data", "gm_mlp_train",
     "gm_mlp_loss_grad", "gm_mlp_predict_dev", "gm_mlp_predict", "gm_mlp_guess_dev",
+    "gm_mlp_set_basis", "gm_mlp_clear_basis",
 ]
 
 _lib = None
@@ -178,6 +179,9 @@ def lib():
     L.gm_mlp_predict_dev.argtypes = [vp, C.c_longlong, vp, vp]
     L.gm_mlp_predict.argtypes = [vp, C.c_longlong, vp, vp]
     L.gm_mlp_guess_dev.argtypes = [vp, vp, C.POINTER(GmMlpQuery), C.POINTER(GmMlpQuery), vp, vp]
+    L.gm_mlp_set_basis.argtypes = [vp, vp, ip, vp, vp]
+    L.gm_mlp_clear_basis.argtypes = [vp]
+    L.gm_mlp_clear_basis.restype = None
     _lib = L
     return L
 
@@ -473,3 +477,12 @@ class Mlp:
         _check(lib().gm_mlp_guess_dev(self.h, ctx.h, C.byref(query),
                                       C.byref(prev_query) if prev_query is not None else None,
                                       x_prev_ptr, x_ptr), "gm_mlp_guess_dev")
+
+    def set_basis(self, ctx: Context, snaps_ptr: int, nsnap: int, weight_ptr: int) -> int:
+        """Snapshot subspace for the delta-form guess on ctx (device arrays:
+        snapshots [nsnap][N], weights [N]); returns its dimension."""
+        return _check(lib().gm_mlp_set_basis(self.h, ctx.h, nsnap, snaps_ptr, weight_ptr),
+                      "gm_mlp_set_basis")
+
+    def clear_basis(self):
+        lib().gm_mlp_clear_basis(self.h)

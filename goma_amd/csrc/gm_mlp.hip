@@ -32,6 +32,8 @@ static constexpr int MT = 64;           // GEMM block tile (points and features)
 static constexpr int SPLIT = 512;       // points per weight-gradient partial
 static constexpr int LOSS_BLOCKS = 256;
 static constexpr int IN_PAD = 64, OUT_PAD = 64;
+static constexpr int QBLK = 512;        // blocks of the subspace dot products
+static constexpr int MAX_BASIS = 128;
 
 // round to nearest even (NaN kept quiet), as the oracle's omlp_bf16
 __device__ inline bf16_t f2bf(float x) {
@@ -44,17 +46,6 @@ __device__ inline bf16_t f2bf(float x) {
   return (bf16_t)(u >> 16);
 }
 __device__ inline float bf2f(bf16_t h) { return __uint_as_float((unsigned)h << 16); }
-
-static bf16_t f2bf_host(float x) {
-  uint32_t u;
-  std::memcpy(&u, &x, 4);
-  if ((u & 0x7f800000u) == 0x7f800000u) {
-    if (u & 0x007fffffu) u |= 0x00400000u;
-  } else {
-    u += 0x7fffu + ((u >> 16) & 1u);
-  }
-  return (bf16_t)(u >> 16);
-}
 
 __device__ inline bf8 load8(const bf16_t *p) { return __builtin_bit_cast(bf8, *(const u4v *)p); }
 
@@ -346,6 +337,53 @@ __global__ void k_mlp_dirichlet(int N, const signed char *__restrict__ dbc,
   if (i < N && dbc[i]) x[i] = val[i];
 }
 
+// ---- snapshot subspace (weighted Gram-Schmidt, projection)
+__global__ void k_mlp_wdiff(int N, const double *__restrict__ a, const double *__restrict__ b,
+                            const double *__restrict__ w, double *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) out[i] = w[i] * (a[i] - b[i]);
+}
+// part[k][blk] = sum over the block's share of Q[k] . v, k < nb (fixed order)
+__global__ __launch_bounds__(256) void k_mlp_qdots(int N, int nb, const double *__restrict__ Q,
+                                                   const double *__restrict__ v,
+                                                   double *__restrict__ part) {
+  __shared__ double sh[4];
+  for (int k = 0; k < nb; k++) {
+    const double *q = Q + (size_t)k * N;
+    double s = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) s += q[i] * v[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[(size_t)k * gridDim.x + blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+  }
+}
+// v -= sum_k c_k Q[k]
+__global__ void k_mlp_qsub(int N, int nb, const double *__restrict__ Q, const double *__restrict__ c,
+                           double *__restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double s = 0.0;
+  for (int k = 0; k < nb; k++) s += c[k] * Q[(size_t)k * N + i];
+  v[i] -= s;
+}
+__global__ void k_mlp_scale(int N, double *v, double s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) v[i] *= s;
+}
+// x = x_prev + W^-1 sum_k c_k Q[k]  (zero change where the weight is zero)
+__global__ void k_mlp_qapply(int N, int nb, const double *__restrict__ Q, const double *__restrict__ c,
+                             const double *__restrict__ w, const double *__restrict__ xprev,
+                             double *__restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double s = 0.0;
+  for (int k = 0; k < nb; k++) s += c[k] * Q[(size_t)k * N + i];
+  x[i] = xprev[i] + (w[i] > 0.0 ? s / w[i] : 0.0);
+}
+
 // ------------------------------------------------------------------ host side
 struct gm_mlp {
   gm_mlp_shape s{};
@@ -373,6 +411,14 @@ struct gm_mlp {
   long long gx_n = 0;
   const gm_ctx *gctx = nullptr;       // context whose sample points are cached
   std::vector<float> gpts;            // [npts][dim] coordinates of nodes + element points
+  // snapshot subspace of the delta form
+  const gm_ctx *bctx = nullptr;
+  int nbasis = 0;
+  double *Q = nullptr;                // [nbasis][N], orthonormal in the weighted norm
+  double *wgt = nullptr;              // [N]
+  double *qpart = nullptr;            // [GM_MLP_MAX_BASIS][QBLK] partial dot products
+  double *qc = nullptr;               // [GM_MLP_MAX_BASIS] coefficients
+  double *qv = nullptr;               // [N] work vector
 };
 
 static uint64_t splitmix(uint64_t &st) {
@@ -531,7 +577,8 @@ static void free_mlp(gm_mlp *m) {
   for (bf16_t *p : m->WbT) ps.push_back(p);
   for (bf16_t *p : m->act) ps.push_back(p);
   void *more[] = {m->y,   m->dz[0], m->dz[1], m->part, m->partb, m->bt,  m->bm,  m->lpart,
-                  m->lossbuf, m->din, m->dt,  m->dm,   m->dperm, m->gin, m->gout, m->gx};
+                  m->lossbuf, m->din, m->dt,  m->dm,   m->dperm, m->gin, m->gout, m->gx,
+                  m->Q,   m->wgt,  m->qpart, m->qc,  m->qv};
   for (void *p : more) ps.push_back(p);
   for (void *p : ps) dfree(p);
   if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -854,6 +901,86 @@ static void sample_points(const gm_ctx *c, std::vector<float> &pts) {
   }
 }
 
+// c[k] = Q[k] . v for k < nb, reduced in a fixed order on the host
+static int qdots(gm_mlp *m, int N, int nb, const double *v, std::vector<double> &c) {
+  c.assign(nb, 0.0);
+  if (!nb) return GM_OK;
+  k_mlp_qdots<<<QBLK, 256, 0, m->stream>>>(N, nb, m->Q, v, m->qpart);
+  GM_HIP(hipGetLastError());
+  std::vector<double> part((size_t)nb * QBLK);
+  GM_HIP(hipMemcpyAsync(part.data(), m->qpart, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        m->stream));
+  GM_HIP(hipStreamSynchronize(m->stream));
+  for (int k = 0; k < nb; k++)
+    for (int b = 0; b < QBLK; b++) c[k] += part[(size_t)k * QBLK + b];
+  return GM_OK;
+}
+
+extern "C" void gm_mlp_clear_basis(gm_mlp *m) {
+  if (!m) return;
+  m->bctx = nullptr;
+  m->nbasis = 0;
+}
+
+extern "C" int gm_mlp_set_basis(gm_mlp *m, gm_ctx *c, int nsnap, const double *snaps,
+                                const double *weight) {
+  if (!m || !c || nsnap < 2 || !snaps || !weight) return GM_ERR_ARG;
+  if (nsnap - 1 > MAX_BASIS) {
+    gm_set_error("at most %d snapshot differences in the subspace", MAX_BASIS);
+    return GM_ERR_ARG;
+  }
+  const int N = c->N;
+  gm_mlp_clear_basis(m);
+  for (void *p : {(void *)m->Q, (void *)m->wgt, (void *)m->qpart, (void *)m->qc, (void *)m->qv}) dfree(p);
+  m->Q = m->wgt = m->qpart = m->qc = m->qv = nullptr;
+  int rc;
+  if ((rc = dalloc(&m->Q, (size_t)(nsnap - 1) * N)) || (rc = dalloc(&m->wgt, (size_t)N)) ||
+      (rc = dalloc(&m->qpart, (size_t)MAX_BASIS * QBLK)) || (rc = dalloc(&m->qc, (size_t)MAX_BASIS)) ||
+      (rc = dalloc(&m->qv, (size_t)N)))
+    return rc;
+  GM_HIP(hipStreamSynchronize(c->stream));
+  GM_HIP(hipMemcpyAsync(m->wgt, weight, (size_t)N * sizeof(double), hipMemcpyDeviceToDevice,
+                        m->stream));
+  const int g = (N + 255) / 256;
+  int nb = 0;
+  std::vector<double> cf;
+  for (int k = 1; k < nsnap; k++) {
+    double *v = m->Q + (size_t)nb * N;  // candidate written in place
+    k_mlp_wdiff<<<g, 256, 0, m->stream>>>(N, snaps + (size_t)k * N, snaps, m->wgt, v);
+    double n0 = 0.0;
+    // two classical Gram-Schmidt passes against the accepted vectors
+    for (int pass = 0; pass < 2; pass++) {
+      if ((rc = qdots(m, N, nb, v, cf))) return rc;
+      if (nb) {
+        GM_HIP(hipMemcpyAsync(m->qc, cf.data(), nb * sizeof(double), hipMemcpyHostToDevice, m->stream));
+        k_mlp_qsub<<<g, 256, 0, m->stream>>>(N, nb, m->Q, m->qc, v);
+      }
+      if (pass == 0)  // |v|^2 before = |v|^2 after the first pass + sum c^2
+        for (double x : cf) n0 += x * x;
+    }
+    // norm of the orthogonalised vector: one dot with itself
+    {
+      k_mlp_qdots<<<QBLK, 256, 0, m->stream>>>(N, 1, v, v, m->qpart);
+      std::vector<double> part(QBLK);
+      GM_HIP(hipMemcpyAsync(part.data(), m->qpart, QBLK * sizeof(double), hipMemcpyDeviceToHost,
+                            m->stream));
+      GM_HIP(hipStreamSynchronize(m->stream));
+      double s = 0.0;
+      for (double x : part) s += x;
+      const double nrm = std::sqrt(s);
+      if (nrm > 1e-10 * std::sqrt(n0 + s) && nrm > 0.0) {
+        k_mlp_scale<<<g, 256, 0, m->stream>>>(N, v, 1.0 / nrm);
+        nb++;
+      }
+    }
+  }
+  GM_HIP(hipGetLastError());
+  GM_HIP(hipStreamSynchronize(m->stream));
+  m->nbasis = nb;
+  m->bctx = nb ? c : nullptr;
+  return nb;
+}
+
 static int guess_one(gm_mlp *m, gm_ctx *c, const gm_mlp_query *q, double *x) {
   const int dim = c->dim, spe = 1 + 2 * dim, n_in = m->s.n_in;
   const long long np = (long long)c->num_nodes + (long long)c->num_elems * spe;
@@ -918,7 +1045,18 @@ extern "C" int gm_mlp_guess_dev(gm_mlp *m, gm_ctx *c, const gm_mlp_query *q, con
       m->gx_n = N;
     }
     if ((rc = guess_one(m, c, q_prev, m->gx))) return rc;
-    k_mlp_delta<<<(N + 255) / 256, 256, 0, m->stream>>>(N, x_prev_dev, m->gx, x_dev);
+    if (m->bctx == c && m->nbasis > 0) {
+      // change = G(q) - G(q_prev), projected onto the snapshot subspace
+      k_mlp_wdiff<<<(N + 255) / 256, 256, 0, m->stream>>>(N, x_dev, m->gx, m->wgt, m->qv);
+      std::vector<double> cf;
+      if ((rc = qdots(m, N, m->nbasis, m->qv, cf))) return rc;
+      GM_HIP(hipMemcpyAsync(m->qc, cf.data(), cf.size() * sizeof(double), hipMemcpyHostToDevice,
+                            m->stream));
+      k_mlp_qapply<<<(N + 255) / 256, 256, 0, m->stream>>>(N, m->nbasis, m->Q, m->qc, m->wgt,
+                                                           x_prev_dev, x_dev);
+    } else {
+      k_mlp_delta<<<(N + 255) / 256, 256, 0, m->stream>>>(N, x_prev_dev, m->gx, x_dev);
+    }
   }
   k_mlp_dirichlet<<<(N + 255) / 256, 256, 0, m->stream>>>(N, c->d_dbc, c->d_dbc_val, x_dev);
   GM_HIP(hipGetLastError());

@@ -156,5 +156,33 @@ class LearnedGuess:
         else:
             self.net.guess_dev(self.ctx, self.query(Ra), x_ptr, self.query(Ra_prev), x_prev_ptr)
 
+    def set_basis(self, sols):
+        """Project delta-form guesses onto the span of the sweep's solution
+        differences, weighted per field by 1 / RMS of those differences
+        (Dirichlet unknowns excluded).  Returns the subspace dimension."""
+        import torch
+
+        from .mesh import CENTROID_LOCAL
+
+        S = np.asarray(sols, dtype=np.float64)
+        D = S[1:] - S[:1]
+        m = self.prob.mesh
+        d, en = m.dim, int(self.prob.phys.energy)
+        fu = self.ctx.first_unknown()[: m.num_nodes].astype(np.int64)
+        cen = fu[m.connect[:, CENTROID_LOCAL[d]]] + d + en
+        groups = [fu + k for k in range(d)] + ([fu + d] if en else []) + \
+                 [np.concatenate([cen + k for k in range(d + 1)])]
+        w = np.zeros(self.ctx.N)
+        for g in groups:
+            rms = np.sqrt(np.mean(D[:, g] ** 2))
+            w[g] = 1.0 / rms if rms > 0 else 0.0
+        mask, _ = self.ctx.dirichlet()
+        w[mask > 0] = 0.0
+        self._snaps = torch.tensor(S, device="cuda")
+        self._w = torch.tensor(w, device="cuda")
+        nb = self.net.set_basis(self.ctx, self._snaps.data_ptr(), len(S), self._w.data_ptr())
+        self._snaps = None
+        return nb
+
     def close(self):
         self.net.close()

@@ -298,6 +298,16 @@ typedef struct gm_mlp_query {
  * x_prev_dev (delta form): x = x_prev + G(q) - G(q_prev). */
 int gm_mlp_guess_dev(gm_mlp *m, gm_ctx *ctx, const gm_mlp_query *q, const gm_mlp_query *q_prev,
                      const double *x_prev_dev, double *x_dev);
+/* Snapshot subspace for the delta form: an orthonormal basis (in the norm
+ * weighted by weight_dev[N], zero on excluded unknowns) of the differences
+ * snaps[k] - snaps[0] of the sweep solutions snaps_dev [nsnap][N].  While it is
+ * set, gm_mlp_guess_dev on this context projects the network's predicted
+ * change G(q) - G(q_prev) onto the subspace before adding it to x_prev: the
+ * network supplies the change, the subspace removes its mesh-scale noise.
+ * Returns the basis size; gm_mlp_clear_basis drops it. */
+int gm_mlp_set_basis(gm_mlp *m, gm_ctx *ctx, int nsnap, const double *snaps_dev,
+                     const double *weight_dev);
+void gm_mlp_clear_basis(gm_mlp *m);
 
 #ifdef __cplusplus
 }

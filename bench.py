@@ -190,7 +190,7 @@ def sweep_bench(args, torch, capi):
     # shear-thinning flow weakens the V(1,1) cycle as Ra grows: V(2,2), larger Krylov space
     opts = dict(kspace=max(args.kspace, 100), max_its=max(args.max_its, 400), precision=prec,
                 max_newton=25, precond=2, mg_nu=max(args.mg_nu, 2), mg_omega=args.mg_omega,
-                line_search=args.line_search)
+                line_search=args.line_search, pc_reuse=args.pc_reuse)
     log = lambda s: print(s, file=sys.stderr, flush=True)
     ras = problems.sweep_ra(args.sweep_count, 1e3, 1e6)
     log(f"[sweep] {prob.name}: N={ctx.N} levels {nlev}, {len(ras)} Ra values")
@@ -275,6 +275,118 @@ def sweep_bench(args, torch, capi):
     return 0
 
 
+def learned_bench(args, torch, capi):
+    """Mixed precision plus the learned initial guess against the plain FP64
+    path, on one target Rayleigh number.
+
+    Plain path: FP64 Newton from the zero guess at the target Ra.  Learned
+    path: a continuation sweep over Ra values a quarter decade apart, offset
+    so that the target is the geometric midpoint of two of them (the target is
+    held out), trains the network on the GPU; at the target, Newton in mixed
+    precision starts from the learned guess (delta form from the sweep
+    solution just below, projected onto the sweep's subspace) with the
+    previous-step guess as fallback.  The sweep and the training are reported
+    separately: they are done once per parameter study, not per solve.
+    """
+    from goma_amd import continuation, learned, problems
+
+    log = lambda s: print(s, file=sys.stderr, flush=True)
+    prob, params = make_problem(args.learned_base, args.n)
+    target = float(params["Ra"])
+    ctx = capi.Context(prob)
+    nlev = ctx.attach_multigrid(args.mg_coarse)
+    base = dict(kspace=args.kspace, max_its=args.max_its, max_newton=40, precond=2,
+                mg_nu=args.mg_nu, mg_omega=args.mg_omega, line_search=args.line_search,
+                pc_reuse=args.pc_reuse)
+    fp64, mixed = dict(base, precision=0), dict(base, precision=1)
+    groups = field_groups(ctx, prob)
+    log(f"[learned] {prob.name}: N={ctx.N} levels {nlev}, target Ra={target:g}")
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    def nodal_error(x, ref):
+        xh, rh = x.cpu().numpy(), ref.cpu().numpy()
+        return {k: float(np.max(np.abs(xh[i] - rh[i])) / max(np.max(np.abs(rh[i])), 1e-300))
+                for k, i in groups.items()}
+
+    # plain FP64 path at the target, from zero (warm-up solve first)
+    ctx.set_physics(problems.physics_at(prob, target))
+    xw = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    ctx.solve_dev(xw.data_ptr(), **fp64)
+    x_ref = torch.zeros_like(xw)
+    (_, rp), t_plain = timed(lambda: ctx.solve_dev(x_ref.data_ptr(), **fp64))
+    log(f"[learned] plain FP64 from zero: {rp.newton_its} passes, {rp.lin_its_total} GMRES, "
+        f"{t_plain:.2f}s")
+    # sweep with the target held out
+    lt = np.log10(target)
+    ks = np.arange(args.sweep_count) - args.sweep_count // 2
+    ras = 10 ** (lt + (ks + 0.5) * args.sweep_step)
+    (steps, sols), t_sweep = timed(lambda: continuation.sweep(ctx, prob, ras, fp64, log=log if args.verbose else None))
+    if not all(s.converged for s in steps):
+        raise SystemExit("[learned] the sweep did not converge")
+    log(f"[learned] sweep of {len(ras)} values [{ras[0]:.4g}, {ras[-1]:.4g}] in {t_sweep:.1f}s")
+    lg = learned.LearnedGuess(prob, ctx, ras, sols, hidden=4, width=256, seed=20261202)
+    hist, t_train = timed(lambda: lg.train(epochs=args.epochs, batch=8192, lr=2e-3, lr_final=2e-5,
+                                           seed=1))
+    fpp = mlp_flops_per_point(prob.mesh.dim + 1, lg.n_out, 4, 256)
+    tflops = 3.0 * fpp * lg.num_points * args.epochs / t_train / 1e12
+    nb = lg.set_basis(sols)
+    log(f"[learned] trained on {lg.num_points} points x {args.epochs} epochs in {t_train:.1f}s "
+        f"({tflops:.1f} TFLOP/s), loss {hist[0]:.2e} -> {hist[-1]:.2e}, subspace {nb}")
+    kp = int(np.searchsorted(ras, target)) - 1
+    ra_prev = float(ras[kp])
+    xprev = torch.tensor(sols[kp], device="cuda")
+    ctx.set_physics(problems.physics_at(prob, target))
+    out_v = {}
+    for name, opts, use_net in (("mixed+learned", mixed, True), ("fp64+learned", fp64, True),
+                                ("mixed+previous", mixed, False), ("fp64+previous", fp64, False)):
+        x = torch.empty_like(xprev)
+
+        def run():
+            if use_net:
+                lg.guess(x.data_ptr(), target, ra_prev, xprev.data_ptr())
+                return ctx.solve_guess_dev(x.data_ptr(), xprev.data_ptr(), 3, 10.0, **opts)
+            x.copy_(xprev)
+            return ctx.solve_dev(x.data_ptr(), **opts)
+
+        (_, rep), dt = timed(run)
+        out_v[name] = dict(seconds=round(dt, 4), newton_its=rep.newton_its + rep.discarded_its,
+                           gmres_its=rep.lin_its_total, converged=bool(rep.converged),
+                           fell_back=bool(rep.fell_back), r0=rep.resid_L2[0],
+                           nodal_error_vs_fp64=nodal_error(x, x_ref))
+        log(f"[learned] {name}: {dt:.2f}s, {out_v[name]['newton_its']} passes, r0 "
+            f"{rep.resid_L2[0]:.2e}, fallback {bool(rep.fell_back)}")
+    t_ml = out_v["mixed+learned"]["seconds"]
+    out = {
+        "metric": "s per converged Newton solve (‖R‖<1e-10): mixed precision + learned guess vs "
+                  "plain FP64 GPU",
+        "value": t_ml, "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f32/f64",
+        "data": "synthetic",
+        "config": {"workload": f"{prob.name} at Ra={target:g} " +
+                   ", ".join(f"{k}={v}" for k, v in params.items()),
+                   "unknowns": ctx.N, "nnz": ctx.nval - 1, "elements": prob.mesh.num_elems,
+                   "sweep": [round(float(r), 1) for r in ras], "network":
+                   f"{prob.mesh.dim + 1}-256-256-256-256-{lg.n_out} tanh, BF16 MFMA"},
+        "plain_fp64": {"seconds": round(t_plain, 4), "newton_its": rp.newton_its,
+                       "gmres_its": rp.lin_its_total, "initial_guess": "zero"},
+        "speedup_vs_plain_fp64": round(t_plain / t_ml, 3),
+        "variants": out_v,
+        "sweep_seconds": round(t_sweep, 2),
+        "training": {"points": lg.num_points, "epochs": args.epochs, "seconds": round(t_train, 2),
+                     "algorithmic_TFLOPs": round(tflops, 2), "loss_last": hist[-1]},
+        "subspace_dim": nb,
+    }
+    print(json.dumps(out), flush=True)
+    lg.close()
+    ctx.close()
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -291,6 +403,8 @@ def main(argv=None):
     ap.add_argument("--mg-omega", type=float, default=0.7)
     ap.add_argument("--line-search", type=int, default=1, help="1: Goma's backtracking line search")
     ap.add_argument("--mg-coarse", type=int, default=2, help="coarsest elements per side")
+    ap.add_argument("--pc-reuse", type=int, default=0,
+                    help="preconditioner reuse across Newton passes: 0 recalc, 1 reuse, 2 adaptive")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--prof-every", type=int, default=1,
                     help="time one launch in this many (pseudo-random), per kernel")
@@ -298,7 +412,10 @@ def main(argv=None):
                     help="1: also time the mixed-precision solve and report its nodal error")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sweep-count", type=int, default=64, help="sweep workload: Ra values")
-    ap.add_argument("--epochs", type=int, default=60, help="sweep workload: training epochs")
+    ap.add_argument("--epochs", type=int, default=60, help="sweep / learned: training epochs")
+    ap.add_argument("--learned-base", default="cavity3d", help="learned workload: the problem")
+    ap.add_argument("--sweep-step", type=float, default=0.25,
+                    help="learned workload: log10 Ra spacing of the sweep")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -314,6 +431,10 @@ def main(argv=None):
 
     if args.workload == "sweep":
         return sweep_bench(args, torch, capi)
+    if args.workload == "learned":
+        if args.sweep_count == 64:
+            args.sweep_count = 8
+        return learned_bench(args, torch, capi)
     prec = 1 if args.precision == "mixed" else 0
     prob, params = make_problem(args.workload, args.n)
     t0 = time.perf_counter()
@@ -328,7 +449,7 @@ def main(argv=None):
     x = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
     opts = dict(kspace=args.kspace, max_its=args.max_its, patch_bins=bins, precision=prec,
                 max_newton=40, verbose=1 if args.verbose else 0, precond=pc, mg_nu=args.mg_nu,
-                mg_omega=args.mg_omega, line_search=args.line_search)
+                mg_omega=args.mg_omega, line_search=args.line_search, pc_reuse=args.pc_reuse)
     for w in range(args.warmup):
         x.zero_()
         torch.cuda.synchronize()
@@ -397,7 +518,7 @@ def main(argv=None):
         "data": "synthetic",
         "config": {"workload": f"{prob.name} ({args.workload}) " + ", ".join(f"{k}={v}" for k, v in params.items()),
                    "unknowns": ctx.N, "nnz": ctx.nval - 1, "elements": prob.mesh.num_elems,
-                   "precision": args.precision, "solver": f"GMRES({args.kspace}) + " + ("multigrid V(%d,%d) element-block smoothing, %d levels" % (args.mg_nu, args.mg_nu, nlev) if pc == 2 else "ILU(0) patch order"),
+                   "precision": args.precision, "solver": f"GMRES({args.kspace}) + " + ("multigrid V(%d,%d) element-block smoothing, %d levels, exact coarsest solve" % (args.mg_nu, args.mg_nu, nlev) if pc == 2 else "ILU(0) patch order") + ", preconditioner reuse " + ("recalc", "reuse", "adaptive")[args.pc_reuse],
                    "initial_guess": "zero"},
         "newton_its": newton,
         "gmres_its": gm_its,

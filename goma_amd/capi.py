@@ -57,7 +57,7 @@ class GmSolverOpts(C.Structure):
                 ("lin_tol", C.c_double), ("precond", C.c_int), ("patch_bins", C.c_int),
                 ("precision", C.c_int), ("verbose", C.c_int), ("mg_nu", C.c_int),
                 ("mg_omega", C.c_double), ("mg_coarse_sweeps", C.c_int), ("line_search", C.c_int),
-                ("min_damp", C.c_double)]
+                ("min_damp", C.c_double), ("pc_reuse", C.c_int)]
 
 
 MAX_REPORT = 64
@@ -71,7 +71,7 @@ class GmNewtonReport(C.Structure):
                 ("t_assemble", C.c_double),
                 ("t_scale", C.c_double), ("t_precond", C.c_double), ("t_solve", C.c_double),
                 ("t_update", C.c_double), ("t_total", C.c_double), ("fell_back", C.c_int),
-                ("discarded_its", C.c_int)]
+                ("discarded_its", C.c_int), ("pc_builds", C.c_int)]
 
     def summary(self) -> dict:
         n = min(self.newton_its, MAX_REPORT)
@@ -82,7 +82,7 @@ class GmNewtonReport(C.Structure):
                     t_assemble=self.t_assemble,
                     t_scale=self.t_scale, t_precond=self.t_precond, t_solve=self.t_solve,
                     t_update=self.t_update, t_total=self.t_total, fell_back=bool(self.fell_back),
-                    discarded_its=self.discarded_its)
+                    discarded_its=self.discarded_its, pc_builds=self.pc_builds)
 
 
 MLP_MAX_IN = 8

@@ -93,6 +93,7 @@ extern "C" void gm_default_solver_opts(gm_solver_opts *o) {
   o->mg_coarse_sweeps = 0;
   o->line_search = 0;
   o->min_damp = 0.05;
+  o->pc_reuse = 0;
 }
 
 static void free_ctx(gm_ctx *c) {
@@ -408,6 +409,8 @@ static int solve_impl(gm_ctx *c, double *x, const gm_solver_opts *o, gm_newton_r
   if (rc) return rc;
   const double t0 = now_s();
   int converged = 0, inewton = 0;
+  bool have_pc = false, last_ok = true;  // preconditioner reuse state
+  int its_ref = 0, last_its = 0;
   while (!converged && inewton < o->max_newton) {
     double ta = now_s();
     if ((rc = gm_zero(c, c->d_resid, c->N))) return rc;
@@ -454,12 +457,14 @@ static int solve_impl(gm_ctx *c, double *x, const gm_solver_opts *o, gm_newton_r
       int its = 0;
       double rr = 0.0;
       gm_solver_opts oo = *o;
+      const bool rebuild = !have_pc || o->pc_reuse == 0 ||
+                           (o->pc_reuse == 2 && (!last_ok || last_its > 1.5 * its_ref + 2));
       // preconditioner set-up timed separately, then GMRES
-      if (o->precond == GM_PC_ILU0) {
+      if (rebuild && o->precond == GM_PC_ILU0) {
         const int bins = o->patch_bins > 0 ? o->patch_bins : gm_default_bins(c);
         if ((rc = gm_ilu_plan(c, bins))) return rc;
         if ((rc = gm_ilu_factor(c, c->d_val, c->d_val32, prec))) return rc;
-      } else if (o->precond == GM_PC_MG) {
+      } else if (rebuild && o->precond == GM_PC_MG) {
         if ((rc = gm_mg_setup(c, x, c->d_val, c->d_val32, prec, c->d_scale,
                               o->mg_coarse_sweeps <= 0)))
           return rc;
@@ -477,6 +482,13 @@ static int solve_impl(gm_ctx *c, double *x, const gm_solver_opts *o, gm_newton_r
         rep->lin_relres[inewton] = rr;
       }
       rep->lin_its_total += its;
+      if (rebuild) {
+        have_pc = true;
+        its_ref = its;
+        rep->pc_builds++;
+      }
+      last_its = its;
+      last_ok = rr <= o->lin_tol;
       double cL2r, cLoor;
       if ((rc = gm_corr_norms(c, c->d_dx, x, &cL2r, &cLoor))) return rc;
       double taken = o->damp;
@@ -527,6 +539,7 @@ extern "C" int gm_solve_nonlinear_guess_dev(gm_ctx *c, double *x, const double *
   rep->fell_back = 1;
   rep->discarded_its = first.newton_its;
   rep->lin_its_total += first.lin_its_total;
+  rep->pc_builds += first.pc_builds;
   rep->t_assemble += first.t_assemble;
   rep->t_scale += first.t_scale;
   rep->t_precond += first.t_precond;

@@ -68,76 +68,90 @@ __device__ inline long long local_pos(const LevelDev &L, int li, int lj) {
 }
 
 // local matrix -> inverse (stored transposed: ainv[e][j][i] = (A_e^-1)[i][j])
-template <int DIM, int EN, typename VT, typename AT, int NT>
+// Element-block inverse by Gauss-Jordan with partial pivoting in LDS, one
+// workgroup per element, in compute type CT (FP64, or FP32 for the mixed
+// precision).  Per pivot step: wave 0 finds the pivot while all threads copy
+// column k; then the pivot row is copied and the rows are interchanged; then
+// every entry is updated from the two copies.  The row interchanges are undone
+// as column interchanges at the end, one row per thread.  The inverse is
+// stored transposed (column-major) for the update kernel.
+template <int DIM, int EN, typename VT, typename AT, typename CT, int NT>
 __global__ __launch_bounds__(NT) void k_vanka_invert(LevelDev L, const VT *__restrict__ val,
                                                      AT *__restrict__ ainv) {
   using Lc = Loc<DIM, EN>;
   constexpr int n = Lc::N;
-  __shared__ double A[n][n + 1];
+  __shared__ CT A[n][n + 1];
+  __shared__ CT s_col[n], s_row[n];
   __shared__ int s_piv[n];
-  __shared__ double s_f[n];
-  __shared__ double s_d;
   __shared__ int s_p;
   const int e = blockIdx.x, tid = threadIdx.x;
   const int *map = L.e2nz + (size_t)e * L.E;
   for (int idx = tid; idx < n * n; idx += NT) {
     const int i = idx / n, j = idx - i * n;
     const long long p = local_pos<DIM, EN>(L, i, j);
-    A[i][j] = p >= 0 ? (double)val[map[p]] : 0.0;
+    A[i][j] = p >= 0 ? (CT)val[map[p]] : (CT)0;
   }
   __syncthreads();
   for (int k = 0; k < n; k++) {
-    if (tid == 0) {
-      int p = k;
-      double m = fabs(A[k][k]);
-      for (int i = k + 1; i < n; i++)
-        if (fabs(A[i][k]) > m) {
-          m = fabs(A[i][k]);
-          p = i;
+    for (int i = tid; i < n; i += NT) s_col[i] = A[i][k];
+    if (tid < 64) {
+      CT best = (CT)-1;
+      int bi = n;
+      for (int i = k + tid; i < n; i += 64) {
+        const CT v = fabs(A[i][k]);
+        if (v > best) {
+          best = v;
+          bi = i;
         }
-      s_p = p;
-      s_piv[k] = p;
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const CT ov = __shfl_xor(best, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ov > best || (ov == best && oi < bi)) {
+          best = ov;
+          bi = oi;
+        }
+      }
+      if (tid == 0) {
+        if (bi >= n) bi = k;  // no comparable candidate (NaN column): keep the row
+        s_p = bi;
+        s_piv[k] = bi;
+      }
     }
     __syncthreads();
     const int p = s_p;
-    if (p != k)
-      for (int j = tid; j < n; j += NT) {
-        const double t = A[k][j];
-        A[k][j] = A[p][j];
-        A[p][j] = t;
-      }
-    __syncthreads();
-    if (tid == 0) {
-      const double piv = A[k][k];
-      // a structurally empty column (a local pressure mode with no
-      // velocity coupling inside the element) leaves that unknown untouched
-      s_d = fabs(piv) > 1e-300 ? 1.0 / piv : 0.0;
+    for (int j = tid; j < n; j += NT) {
+      s_row[j] = A[p][j];
+      if (p != k) A[p][j] = A[k][j];
     }
     __syncthreads();
-    const double d = s_d;
-    for (int j = tid; j < n; j += NT) A[k][j] = (j == k) ? d : A[k][j] * d;
-    for (int i = tid; i < n; i += NT) s_f[i] = A[i][k];
-    __syncthreads();
+    const CT piv = s_row[k];
+    // a structurally empty column (a local pressure mode with no velocity
+    // coupling inside the element) leaves that unknown untouched
+    const CT d = fabs(piv) > (CT)1e-300 ? (CT)1 / piv : (CT)0;
     for (int idx = tid; idx < n * n; idx += NT) {
       const int i = idx / n, j = idx - i * n;
-      if (i == k) continue;
-      const double f = s_f[i];
-      if (j == k) A[i][j] = -f * d;
-      else A[i][j] -= f * A[k][j];
+      if (i == k) {
+        A[k][j] = j == k ? d : s_row[j] * d;
+      } else {
+        const CT f = i == p ? s_col[k] : s_col[i];
+        A[i][j] = j == k ? -f * d : A[i][j] - f * (s_row[j] * d);
+      }
     }
     __syncthreads();
   }
-  // undo the row interchanges as column interchanges, in reverse order
-  for (int k = n - 1; k >= 0; k--) {
-    const int p = s_piv[k];
-    if (p != k)
-      for (int i = tid; i < n; i += NT) {
-        const double t = A[i][k];
+  // undo the row interchanges as column interchanges, last step first
+  for (int i = tid; i < n; i += NT)
+    for (int k = n - 1; k >= 0; k--) {
+      const int p = s_piv[k];
+      if (p != k) {
+        const CT t = A[i][k];
         A[i][k] = A[i][p];
         A[i][p] = t;
       }
-    __syncthreads();
-  }
+    }
+  __syncthreads();
   AT *out = ainv + (size_t)e * n * n;
   for (int idx = tid; idx < n * n; idx += NT) {
     const int j = idx / n, i = idx - j * n;
@@ -446,12 +460,13 @@ static int alloc_level_vectors(MgLevel &L, const gm_ctx *lc, int prec, bool coar
 
 template <int DIM, int EN, typename VT, typename AT>
 static void launch_invert(gm_ctx *prof, gm_ctx *lc, hipStream_t s, const VT *val, AT *ainv) {
-  constexpr int NT = DIM == 2 ? 128 : 256;
+  // 3D blocks (n = 85 / 112) fill one CU's LDS in FP64: eight waves per block
+  constexpr int NT = DIM == 2 ? 128 : 512;
   constexpr int n = Loc<DIM, EN>::N;
   // gathered entries + map + inverses written
   const double bytes = (double)lc->num_elems * n * n * (sizeof(VT) + 4 + sizeof(AT));
-  gm_launch(prof, GM_K_VANKA_INV, bytes, k_vanka_invert<DIM, EN, VT, AT, NT>, dim3(lc->num_elems),
-            dim3(NT), 0, s, level_dev(lc), val, ainv);
+  gm_launch(prof, GM_K_VANKA_INV, bytes, k_vanka_invert<DIM, EN, VT, AT, AT, NT>,
+            dim3(lc->num_elems), dim3(NT), 0, s, level_dev(lc), val, ainv);
 }
 
 template <typename VT>

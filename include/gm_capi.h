@@ -98,6 +98,11 @@ typedef struct gm_solver_opts {
                           inverse, at most 4096 unknowns); k > 0 = k smoothing sweeps */
   int line_search;     /* 0: full step (NLS_FULL_STEP); 1: backtracking (NLS_BACKTRACK) */
   double min_damp;     /* Line_Search_Minimum_Damping */
+  int pc_reuse;        /* Matrix_Factorization_Reuse (src/mm_sol_nonlinear.c:1687-1708):
+                          0 "recalc": rebuild the preconditioner at every Newton pass;
+                          1 "reuse": build it at the first pass and keep it;
+                          2 adaptive: keep it until GMRES misses lin_tol or needs
+                            more than 1.5x the iterations of its first use */
 } gm_solver_opts;
 
 #define GM_MAX_REPORT 64
@@ -114,6 +119,7 @@ typedef struct gm_newton_report {
   double t_assemble, t_scale, t_precond, t_solve, t_update, t_total; /* seconds */
   int fell_back;       /* gm_solve_nonlinear_guess_dev: 1 if the guess was abandoned */
   int discarded_its;   /* Newton passes spent on the abandoned guess */
+  int pc_builds;       /* preconditioner set-ups (see pc_reuse) */
 } gm_newton_report;
 
 typedef struct gm_ctx gm_ctx;

@@ -387,6 +387,24 @@ def learned_bench(args, torch, capi):
     return 0
 
 
+def copy_calibration(torch):
+    """A plain device copy of known size (1 GiB read, 1 GiB written, three
+    times): the reference for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE
+    counters to bytes."""
+    n = 1 << 27
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 3
+    print(json.dumps({"workload": "copy calibration", "bytes_read": n * 8, "bytes_written": n * 8,
+                      "launches": 3, "GBps": round(2 * n * 8 / dt / 1e9, 1)}), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -429,6 +447,8 @@ def main(argv=None):
 
     from goma_amd import capi
 
+    if args.workload == "copycal":
+        return copy_calibration(torch)
     if args.workload == "sweep":
         return sweep_bench(args, torch, capi)
     if args.workload == "learned":

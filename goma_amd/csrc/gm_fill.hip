@@ -75,7 +75,7 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
   constexpr int NV = DIM + EN;
   constexpr int NVE2 = NVE * NVE;
   using T = Tab<DIM>;
-  const int e = elems[blockIdx.x];
+  const int e = elems[xcd_block()];
   const int tid = threadIdx.x;
   const double *__restrict__ tb = A.tables;
 

@@ -199,8 +199,7 @@ __global__ __launch_bounds__(256) void k_spmv_msr(int N, const VT *__restrict__ 
   // FP64 Krylov vector must not round the vector, or the Arnoldi relation
   // only holds to ~1e-7 |A||x| and GMRES stalls on ill-conditioned systems.
   using AT = typename std::conditional<sizeof(VT) == 8 || sizeof(XT) == 8, double, float>::type;
-  AT s = 0;
-  for (int k = k0 + lane; k < k1; k += G) s += (AT)a[k] * (AT)x[bindx[k]];
+  AT s = row_dot<AT, VT, XT, G>(a, bindx, x, k0, k1, lane);
 #pragma unroll
   for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
   if (lane == 0) y[r] = (YT)(s + (AT)a[r] * (AT)x[r]);

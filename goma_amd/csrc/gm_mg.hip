@@ -170,8 +170,7 @@ __global__ __launch_bounds__(256) void k_mg_resid(int N, const VT *__restrict__ 
   if (row >= N) return;
   using AT = typename std::conditional<sizeof(BT) == 8, double, float>::type;
   const int k0 = bindx[row], k1 = bindx[row + 1];
-  AT s = 0;
-  for (int k = k0 + lane; k < k1; k += G) s += (AT)a[k] * (AT)x[bindx[k]];
+  AT s = row_dot<AT, VT, BT, G>(a, bindx, x, k0, k1, lane);
 #pragma unroll
   for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
   if (lane == 0) r[row] = (BT)((AT)b[row] - s - (AT)a[row] * (AT)x[row]);
@@ -190,8 +189,7 @@ __global__ __launch_bounds__(256) void k_mg_resid_rows(int nrows, const int *__r
   const int row = rows[k];
   using AT = typename std::conditional<sizeof(BT) == 8, double, float>::type;
   const int k0 = bindx[row], k1 = bindx[row + 1];
-  AT s = 0;
-  for (int q = k0 + lane; q < k1; q += G) s += (AT)a[q] * (AT)x[bindx[q]];
+  AT s = row_dot<AT, VT, BT, G>(a, bindx, x, k0, k1, lane);
 #pragma unroll
   for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
   if (lane == 0) r[row] = (BT)((AT)b[row] - s - (AT)a[row] * (AT)x[row]);
@@ -308,7 +306,7 @@ __global__ __launch_bounds__(NT) void k_vanka_update(LevelDev L, const int *__re
   constexpr int n = Lc::N;
   __shared__ double s_r[n];
   __shared__ int s_g[n];
-  const int e = elems[blockIdx.x], tid = threadIdx.x;
+  const int e = elems[xcd_block()], tid = threadIdx.x;
   for (int i = tid; i < n; i += NT) {
     const int g = local_dof<DIM, EN>(L, e, i);
     s_g[i] = g;
@@ -608,9 +606,9 @@ int gm_mg_setup(gm_ctx *c, const double *x, const double *val, const float *val3
 
 static gm_ctx *g_prof_ctx = nullptr;  // context whose counters the V-cycle feeds
 
-template <typename VT, typename BT>
-static void resid(gm_ctx *lc, hipStream_t s, const VT *a, const BT *x, const BT *b, BT *r) {
-  constexpr int G = 16;
+// lanes per row: 2D rows hold ~100-250 entries, 3D rows ~500-2100
+template <typename VT, typename BT, int G>
+static void resid_g(gm_ctx *lc, hipStream_t s, const VT *a, const BT *x, const BT *b, BT *r) {
   const int blocks = (int)(((long long)lc->N * G + 255) / 256);
   const double bytes = (double)(lc->nval - 1) * (sizeof(VT) + 4) + 4.0 * (lc->N + 1) +
                        3.0 * lc->N * sizeof(BT);
@@ -619,9 +617,14 @@ static void resid(gm_ctx *lc, hipStream_t s, const VT *a, const BT *x, const BT 
 }
 
 template <typename VT, typename BT>
-static void resid_rows(gm_ctx *lc, hipStream_t s, int col, const VT *a, const BT *x, const BT *b,
-                       BT *r) {
-  constexpr int G = 16;
+static void resid(gm_ctx *lc, hipStream_t s, const VT *a, const BT *x, const BT *b, BT *r) {
+  if (lc->dim == 3) resid_g<VT, BT, 64>(lc, s, a, x, b, r);
+  else resid_g<VT, BT, 16>(lc, s, a, x, b, r);
+}
+
+template <typename VT, typename BT, int G>
+static void resid_rows_g(gm_ctx *lc, hipStream_t s, int col, const VT *a, const BT *x, const BT *b,
+                         BT *r) {
   const int nr = lc->mg_rows_ptr[col + 1] - lc->mg_rows_ptr[col];
   if (!nr) return;
   const int blocks = (int)(((long long)nr * G + 255) / 256);
@@ -630,6 +633,13 @@ static void resid_rows(gm_ctx *lc, hipStream_t s, int col, const VT *a, const BT
   gm_launch(g_prof_ctx, GM_K_MG_RESID_ROWS, bytes, k_mg_resid_rows<VT, BT, G>, dim3(blocks), dim3(256),
             0, s, nr, (const int *)(lc->d_mg_rows + lc->mg_rows_ptr[col]), a, (const int *)lc->d_bindx,
             x, b, r);
+}
+
+template <typename VT, typename BT>
+static void resid_rows(gm_ctx *lc, hipStream_t s, int col, const VT *a, const BT *x, const BT *b,
+                       BT *r) {
+  if (lc->dim == 3) resid_rows_g<VT, BT, 64>(lc, s, col, a, x, b, r);
+  else resid_rows_g<VT, BT, 16>(lc, s, col, a, x, b, r);
 }
 
 template <int DIM, int EN, typename AT, typename BT>

@@ -126,6 +126,20 @@ def cpu_baseline(prob, gpu_counts: dict, budget_s: float):
                 ilu_factor_s=t_fact)
 
 
+def recorded_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of a kernel on a workload, measured with
+    rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (calibrated on a device copy of
+    known size) and recorded in profiles/traffic.txt; None if not recorded."""
+    path = os.path.join(ROOT, "profiles", "traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and not line.startswith("#") and f[0] == workload and f[1] == kernel:
+            return float(f[2])
+    return None
+
+
 def field_groups(ctx, prob):
     """Unknown indices of the velocity, temperature and pressure fields."""
     from goma_amd.mesh import CENTROID_LOCAL
@@ -515,8 +529,9 @@ def main(argv=None):
     roof = None
     if kern[dom]["GBps"]:
         a = kern[dom]["GBps"]
+        tr = recorded_traffic(prob.name, dom) if prec == 0 else None
         roof = dict(kernel=dom, bound="hbm", achieved=round(a, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(a / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(a / HBM_PEAK_GBS, 4), traffic=round(tr) if tr else None,
                     bytes_per_launch=round(kern[dom]["bytes_per_launch"]),
                     avg_launch_ms=round(kern[dom]["avg_ms"], 5))
     gm_its = sum(r["lin_its_total"] for r in reps) // len(reps)

@@ -28,7 +28,8 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u4v __attribute__((ext_vector_type(4)));
 
-static constexpr int MT = 64;           // GEMM block tile (points and features)
+static constexpr int MT = 64;           // feature tile: widths and padded layer sizes
+static constexpr int PT = 128;          // point granularity of a padded batch
 static constexpr int SPLIT = 512;       // points per weight-gradient partial
 static constexpr int LOSS_BLOCKS = 256;
 static constexpr int IN_PAD = 64, OUT_PAD = 64;
@@ -79,37 +80,44 @@ __global__ __launch_bounds__(256) void k_mlp_gather(int M, int Mpad, const int *
 enum { EPI_TANH = 0, EPI_LIN = 1, EPI_DTANH = 2 };
 
 // C [M][N] = A [M][K] * B[N][K]^T, BF16 operands, FP32 accumulation.
-// Block 64x64, four waves of 32x32 (2x2 MFMA tiles).  M, N multiples of 64,
-// K a multiple of 32.  Epilogues: tanh(C + bias) -> BF16; C + bias -> FP32;
-// C * (1 - aux^2) -> BF16 (back-propagation through the tanh that made aux).
-template <int EPI>
+// Four waves as 2 x 2; each wave computes (16 TM) x (16 TN) with TM x TN MFMA
+// tiles, so one k-step issues TM x TN MFMAs for TM + TN operand loads; the
+// block covers (32 TM) x (32 TN).  M a multiple of 32 TM, N of 32 TN, K of 32.
+// Epilogues: tanh(C + bias) -> BF16; C + bias -> FP32; C * (1 - aux^2) -> BF16
+// (back-propagation through the tanh that made aux).
+template <int EPI, int TM, int TN>
 __global__ __launch_bounds__(256) void k_mlp_gemm(int M, int N, int K, const bf16_t *__restrict__ A,
                                                   const bf16_t *__restrict__ B,
                                                   const float *__restrict__ bias,
                                                   const bf16_t *__restrict__ aux,
                                                   bf16_t *__restrict__ o16, float *__restrict__ o32) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int m0 = blockIdx.x * MT + (wave >> 1) * 32, n0 = blockIdx.y * MT + (wave & 1) * 32;
+  const int m0 = blockIdx.x * (32 * TM) + (wave >> 1) * (16 * TM);
+  const int n0 = blockIdx.y * (32 * TN) + (wave & 1) * (16 * TN);
   const int r = lane & 15, kg = (lane >> 4) * 8;
-  f4 acc[2][2];
+  f4 acc[TM][TN];
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < TM; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  const bf16_t *pa0 = A + (size_t)(m0 + r) * K + kg, *pa1 = pa0 + (size_t)16 * K;
-  const bf16_t *pb0 = B + (size_t)(n0 + r) * K + kg, *pb1 = pb0 + (size_t)16 * K;
+    for (int j = 0; j < TN; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  const bf16_t *pa = A + (size_t)(m0 + r) * K + kg;
+  const bf16_t *pb = B + (size_t)(n0 + r) * K + kg;
   for (int k = 0; k < K; k += 32) {
-    const bf8 a0 = load8(pa0 + k), a1 = load8(pa1 + k);
-    const bf8 b0 = load8(pb0 + k), b1 = load8(pb1 + k);
-    acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
+    bf8 a[TM], b[TN];
+#pragma unroll
+    for (int i = 0; i < TM; i++) a[i] = load8(pa + (size_t)16 * i * K + k);
+#pragma unroll
+    for (int j = 0; j < TN; j++) b[j] = load8(pb + (size_t)16 * j * K + k);
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+      for (int j = 0; j < TN; j++)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
   }
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < TM; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++)
+    for (int j = 0; j < TN; j++)
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const int m = m0 + 16 * i + 4 * (lane >> 4) + q, n = n0 + 16 * j + r;
@@ -124,6 +132,16 @@ __global__ __launch_bounds__(256) void k_mlp_gemm(int M, int N, int K, const bf1
           o16[o] = f2bf(v * (1.f - a * a));
         }
       }
+}
+
+// GEMM launch: 128 x 128 blocks when N allows, 128 x 64 otherwise
+template <int EPI>
+static void gemm(hipStream_t s, int M, int N, int K, const bf16_t *A, const bf16_t *B,
+                 const float *bias, const bf16_t *aux, bf16_t *o16, float *o32) {
+  if (N % 128 == 0)
+    k_mlp_gemm<EPI, 4, 4><<<dim3(M / 128, N / 128), 256, 0, s>>>(M, N, K, A, B, bias, aux, o16, o32);
+  else
+    k_mlp_gemm<EPI, 4, 2><<<dim3(M / 128, N / 64), 256, 0, s>>>(M, N, K, A, B, bias, aux, o16, o32);
 }
 
 // Weight and bias gradients of one layer over one chunk of points:
@@ -446,7 +464,7 @@ static int maxdim(const gm_mlp *m) {
 
 static int ensure_cap(gm_mlp *m, long long M) {
   if (M <= m->cap) return GM_OK;
-  const long long cap = std::max<long long>((M + MT - 1) / MT * MT, 4096);
+  const long long cap = std::max<long long>((M + PT - 1) / PT * PT, 4096);
   for (bf16_t *p : m->act) dfree(p);
   dfree(m->y);
   dfree(m->dz[0]);
@@ -492,13 +510,12 @@ static void gather(gm_mlp *m, int M, int Mpad, const int *idx, const float *in, 
 
 static void forward(gm_mlp *m, int Mpad) {
   for (int l = 0; l < m->L; l++) {
-    const dim3 grid(Mpad / MT, m->N[l] / MT);
     if (l < m->L - 1)
-      k_mlp_gemm<EPI_TANH><<<grid, 256, 0, m->stream>>>(Mpad, m->N[l], m->K[l], m->act[l], m->Wb[l],
-                                                        m->b[l], nullptr, m->act[l + 1], nullptr);
+      gemm<EPI_TANH>(m->stream, Mpad, m->N[l], m->K[l], m->act[l], m->Wb[l], m->b[l], nullptr,
+                     m->act[l + 1], nullptr);
     else
-      k_mlp_gemm<EPI_LIN><<<grid, 256, 0, m->stream>>>(Mpad, m->N[l], m->K[l], m->act[l], m->Wb[l],
-                                                       m->b[l], nullptr, nullptr, m->y);
+      gemm<EPI_LIN>(m->stream, Mpad, m->N[l], m->K[l], m->act[l], m->Wb[l], m->b[l], nullptr,
+                    nullptr, m->y);
   }
 }
 
@@ -521,8 +538,8 @@ static void backward(gm_mlp *m, int Mpad) {
                                                           m->gW[l], m->gb[l]);
     if (l == 0) break;
     // dz_prev [Mpad][K_l] = (dz [Mpad][N_l] * W_l) * (1 - act_l^2)
-    k_mlp_gemm<EPI_DTANH><<<dim3(Mpad / MT, m->K[l] / MT), 256, 0, m->stream>>>(
-        Mpad, m->K[l], m->N[l], m->dz[cur], m->WbT[l], nullptr, m->act[l], m->dz[1 - cur], nullptr);
+    gemm<EPI_DTANH>(m->stream, Mpad, m->K[l], m->N[l], m->dz[cur], m->WbT[l], nullptr, m->act[l],
+                    m->dz[1 - cur], nullptr);
     cur = 1 - cur;
   }
 }
@@ -731,7 +748,7 @@ static int ensure_lossbuf(gm_mlp *m, long long n) {
 }
 
 extern "C" int gm_mlp_train(gm_mlp *m, const gm_mlp_train_opts *o, double *loss_per_epoch) {
-  if (!m || !o || o->epochs < 0 || o->batch < MT || o->batch % MT) return GM_ERR_ARG;
+  if (!m || !o || o->epochs < 0 || o->batch < PT || o->batch % PT) return GM_ERR_ARG;
   if (!m->npts) {
     gm_set_error("gm_mlp_train: no training data");
     return GM_ERR_ARG;
@@ -755,7 +772,7 @@ extern "C" int gm_mlp_train(gm_mlp *m, const gm_mlp_train_opts *o, double *loss_
                           m->stream));
     for (long long b = 0; b < steps; b++, step++) {
       const int M = (int)std::min<long long>(o->batch, m->npts - b * o->batch);
-      const int Mpad = (M + MT - 1) / MT * MT;
+      const int Mpad = (M + PT - 1) / PT * PT;
       const double frac = total > 1 ? (double)step / (double)(total - 1) : 1.0;
       const double lr = o->lr_final + 0.5 * (o->lr - o->lr_final) * (1.0 + std::cos(M_PI * frac));
       gather(m, M, Mpad, m->dperm + b * o->batch, m->din, m->dt, m->dm);
@@ -786,7 +803,7 @@ extern "C" int gm_mlp_loss_grad(gm_mlp *m, int npts, const float *in, const floa
                                 const float *mask, double *loss_out, float *grad) {
   if (!m || npts <= 0 || npts > 65536 || !in || !targ || !mask || !loss_out || !grad)
     return GM_ERR_ARG;
-  const int Mpad = (npts + MT - 1) / MT * MT;
+  const int Mpad = (npts + PT - 1) / PT * PT;
   int rc;
   if ((rc = ensure_cap(m, Mpad)) || (rc = ensure_lossbuf(m, 1))) return rc;
   float *din, *dt, *dm;
@@ -839,7 +856,7 @@ extern "C" int gm_mlp_predict_dev(gm_mlp *m, long long npts, const float *in_dev
   if ((rc = ensure_cap(m, std::min<long long>(std::max<long long>(npts, 1), 65536)))) return rc;
   for (long long off = 0; off < npts; off += m->cap) {
     const int M = (int)std::min<long long>(m->cap, npts - off);
-    const int Mpad = (M + MT - 1) / MT * MT;
+    const int Mpad = (M + PT - 1) / PT * PT;
     gather(m, M, Mpad, nullptr, in_dev + off * m->s.n_in, nullptr, nullptr);
     forward(m, Mpad);
     k_mlp_copy_out<<<(M * m->s.n_out + 255) / 256, 256, 0, m->stream>>>(M, m->s.n_out, m->y,

@@ -260,7 +260,7 @@ typedef struct gm_mlp_shape {
 
 typedef struct gm_mlp_train_opts {
   int epochs;
-  int batch;            /* points per step, multiple of 64 */
+  int batch;            /* points per step, multiple of 128 */
   double lr, lr_final;  /* Adam step size, cosine-decayed from lr to lr_final */
   double beta1, beta2, eps;
   unsigned long long seed; /* point shuffling */

@@ -23,6 +23,45 @@ import numpy as np
 from . import problems
 
 
+def _dlagrange1(pos, s):
+    if pos < 0:
+        return -0.5 + s
+    if pos > 0:
+        return 0.5 + s
+    return -2.0 * s
+
+
+def cell_reynolds(prob, first_unknown, x):
+    """Largest element Reynolds number rho |u| h / mu over the mesh, with u and
+    the shear rate at each element centroid (Q2 fields, 2D) and mu from the
+    problem's viscosity model (Carreau-Yasuda, src/mm_viscosity.c:1065-1072).
+    Returns (max cell Re, min mu, max |u|)."""
+    from .mesh import QUAD9_POS
+
+    m, ph = prob.mesh, prob.phys
+    if m.dim != 2:
+        raise ValueError("2D meshes only")
+    dphi = np.array([[_dlagrange1(p[0], 0.0) * (1.0 if p[1] == 0 else 0.0),
+                      (1.0 if p[0] == 0 else 0.0) * _dlagrange1(p[1], 0.0)] for p in QUAD9_POS])
+    fu = first_unknown[: m.num_nodes].astype(np.int64)
+    conn = m.connect
+    X = m.coor.T[conn]  # [ne, 9, 2]
+    U = np.stack([x[fu[conn] + 0], x[fu[conn] + 1]], axis=-1)  # [ne, 9, 2]
+    J = np.einsum("enk,nd->ekd", X, dphi)  # dx_k / dxi_d
+    Jinv = np.linalg.inv(J)
+    G = np.einsum("enk,nd,edj->ekj", U, dphi, Jinv)  # du_k / dx_j
+    gam = G + np.transpose(G, (0, 2, 1))
+    gdot = np.sqrt(0.5 * np.einsum("eij,eji->e", gam, gam))
+    if ph.visc_model == 1:
+        mu = ph.muinf + (ph.mu0 - ph.muinf) * (1 + (ph.lam * gdot) ** ph.aexp) ** ((ph.nexp - 1) / ph.aexp)
+    else:
+        mu = np.full(len(gdot), ph.mu0)
+    uc = np.linalg.norm(U[:, 8], axis=1)
+    h = np.sqrt(np.abs(np.linalg.det(J)) * 4.0)
+    re = ph.rho * uc * h / mu
+    return float(re.max()), float(mu.min()), float(uc.max())
+
+
 @dataclass
 class Step:
     Ra: float

@@ -296,28 +296,37 @@ __global__ __launch_bounds__(256) void k_dense_gemv(int n, const double *__restr
   if (lane == 0) x[row] = (BT)s;
 }
 
-// x_e += omega * A_e^-1 r_e for the elements of one colour
+// x_e += omega * A_e^-1 r_e for the nelem elements of one colour
 template <int DIM, int EN, typename AT, typename BT, int NT>
-__global__ __launch_bounds__(NT) void k_vanka_update(LevelDev L, const int *__restrict__ elems,
+__global__ __launch_bounds__(NT) void k_vanka_update(LevelDev L, int nelem, const int *__restrict__ elems,
                                                      const AT *__restrict__ ainv,
                                                      const BT *__restrict__ r, BT *__restrict__ x,
                                                      double omega) {
   using Lc = Loc<DIM, EN>;
   constexpr int n = Lc::N;
-  __shared__ double s_r[n];
-  __shared__ int s_g[n];
-  const int e = elems[xcd_block()], tid = threadIdx.x;
-  for (int i = tid; i < n; i += NT) {
-    const int g = local_dof<DIM, EN>(L, e, i);
-    s_g[i] = g;
-    s_r[i] = (double)r[g];
-  }
+  // two elements per workgroup when a block fits in half of it (2D: 30 of 64 lanes)
+  constexpr int EPB = (2 * n <= NT) ? 2 : 1;
+  constexpr int HALF = NT / EPB;
+  __shared__ double s_r2[EPB][n];
+  __shared__ int s_g2[EPB][n];
+  const int tid = threadIdx.x, sub = tid / HALF, t = tid - sub * HALF;
+  const int eidx = xcd_block() * EPB + sub;
+  const bool act = eidx < nelem;
+  const int e = act ? elems[eidx] : 0;
+  if (act)
+    for (int i = t; i < n; i += HALF) {
+      const int g = local_dof<DIM, EN>(L, e, i);
+      s_g2[sub][i] = g;
+      s_r2[sub][i] = (double)r[g];
+    }
   __syncthreads();
-  const AT *M = ainv + (size_t)e * n * n;
-  for (int i = tid; i < n; i += NT) {
-    double s = 0.0;
-    for (int j = 0; j < n; j++) s += (double)M[j * n + i] * s_r[j];
-    x[s_g[i]] += (BT)(omega * s);
+  if (act) {
+    const AT *M = ainv + (size_t)e * n * n;
+    for (int i = t; i < n; i += HALF) {
+      double s = 0.0;
+      for (int j = 0; j < n; j++) s += (double)M[j * n + i] * s_r2[sub][j];
+      x[s_g2[sub][i]] += (BT)(omega * s);
+    }
   }
 }
 
@@ -652,8 +661,9 @@ static void vanka_colour(gm_ctx *lc, hipStream_t s, int col, const AT *ainv, con
   // inverses, element lists/connectivity, residual gathered, x read and written
   const double bytes = (double)cnt * (n * n * sizeof(AT) + 4.0 * (Loc<DIM, EN>::NVE + 2) +
                                       n * 3.0 * sizeof(BT));
-  gm_launch(g_prof_ctx, GM_K_VANKA, bytes, k_vanka_update<DIM, EN, AT, BT, NT>, dim3(cnt), dim3(NT),
-            0, s, level_dev(lc), (const int *)(lc->d_ecol_elems + lc->ecol_ptr[col]), ainv, r, x,
+  constexpr int EPB = (2 * n <= NT) ? 2 : 1;
+  gm_launch(g_prof_ctx, GM_K_VANKA, bytes, k_vanka_update<DIM, EN, AT, BT, NT>, dim3((cnt + EPB - 1) / EPB), dim3(NT),
+            0, s, level_dev(lc), cnt, (const int *)(lc->d_ecol_elems + lc->ecol_ptr[col]), ainv, r, x,
             omega);
 }
 

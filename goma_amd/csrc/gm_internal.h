@@ -81,9 +81,11 @@ struct MgLevel {
   double *d_state = nullptr;  // injected state for re-assembly
   double *d_rdummy = nullptr; // residual written by the coarse assembly (unused)
   void *d_x = nullptr, *d_b = nullptr, *d_r = nullptr;  // level vectors
-  void *d_val = nullptr;      // coarse Jacobian (MSR)
-  void *d_ainv = nullptr;     // element-block inverses
-  int vec_prec = -1, ainv_prec = -1, val_prec = -1;
+  void *d_val = nullptr;      // coarse Jacobian (MSR), in the active precision
+  void *d_ainv = nullptr;     // element-block inverses, in the active precision
+  void *d_val_p[2] = {nullptr, nullptr};   // storage per precision (FP64, mixed): kept, so
+  void *d_ainv_p[2] = {nullptr, nullptr};  // switching precision never reallocates
+  int vec_prec = -1;
   int nnz = 0;                // entries of P
   // coarsest level only: dense FP64 inverse for the exact coarse solve
   double *d_dense = nullptr;  // [n][n]

@@ -993,7 +993,8 @@ int gm_gmres_run(gm_ctx *c, const double *val, const float *val32, const double 
     int bins = o->patch_bins > 0 ? o->patch_bins : gm_default_bins(c);
     if ((rc = gm_ilu_plan(c, bins))) return rc;
     if ((rc = gm_ilu_factor(c, val, val32, prec))) return rc;
-  } else if (o->precond == GM_PC_MG && c->mg0.ainv_prec != prec) {
+  } else if (o->precond == GM_PC_MG &&
+             (!c->mg0.d_ainv || c->mg0.d_ainv != c->mg0.d_ainv_p[prec == GM_PREC_MIXED ? 1 : 0])) {
     gm_set_error("multigrid preconditioner not set up (gm_mg_setup_dev)");
     return GM_ERR_ARG;
   }

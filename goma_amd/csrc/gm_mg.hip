@@ -452,15 +452,12 @@ static int alloc_level_vectors(MgLevel &L, const gm_ctx *lc, int prec, bool coar
     L.vec_prec = 0;
   }
   const size_t nl = nloc_of(lc);
-  if (L.ainv_prec != prec) {
-    if (L.d_ainv) (void)hipFree(L.d_ainv);
-    GM_HIP(hipMalloc(&L.d_ainv, (size_t)lc->num_elems * nl * nl * es));
-    L.ainv_prec = prec;
-  }
-  if (coarse && L.val_prec != prec) {
-    if (L.d_val) (void)hipFree(L.d_val);
-    GM_HIP(hipMalloc(&L.d_val, (size_t)lc->nval * es));
-    L.val_prec = prec;
+  const int pi = prec == GM_PREC_MIXED ? 1 : 0;
+  if (!L.d_ainv_p[pi]) GM_HIP(hipMalloc(&L.d_ainv_p[pi], (size_t)lc->num_elems * nl * nl * es));
+  L.d_ainv = L.d_ainv_p[pi];
+  if (coarse) {
+    if (!L.d_val_p[pi]) GM_HIP(hipMalloc(&L.d_val_p[pi], (size_t)lc->nval * es));
+    L.d_val = L.d_val_p[pi];
   }
   return GM_OK;
 }
@@ -757,7 +754,8 @@ int gm_mg_apply_f(gm_ctx *c, const float *a0, const double *v, double *z, const 
 void gm_mg_free(gm_ctx *c) {
   auto fr = [](MgLevel &L) {
     void *p[] = {L.d_Prow, L.d_Pcol, L.d_Pval, L.d_Rrow, L.d_Rcol, L.d_Rval, L.d_inject,
-                 L.d_state, L.d_rdummy, L.d_x, L.d_b, L.d_r, L.d_val, L.d_ainv,
+                 L.d_state, L.d_rdummy, L.d_x, L.d_b, L.d_r, L.d_val_p[0], L.d_val_p[1],
+                 L.d_ainv_p[0], L.d_ainv_p[1],
                  L.d_dense, L.d_pcol, L.d_prow, L.d_perm};
     for (void *q : p)
       if (q) (void)hipFree(q);

@@ -206,7 +206,7 @@ def sweep_bench(args, torch, capi):
     # shear-thinning flow weakens the V(1,1) cycle as Ra grows: V(2,2), larger Krylov space
     opts = dict(kspace=max(args.kspace, 100), max_its=max(args.max_its, 400), precision=prec,
                 max_newton=25, precond=2, mg_nu=max(args.mg_nu, 2), mg_omega=args.mg_omega,
-                line_search=args.line_search, pc_reuse=args.pc_reuse)
+                line_search=args.line_search, pc_reuse=args.pc_reuse, mg_smoother=0)
     log = lambda s: print(s, file=sys.stderr, flush=True)
     ras = problems.sweep_ra(args.sweep_count, 1e3, 1e6)
     log(f"[sweep] {prob.name}: N={ctx.N} levels {nlev}, {len(ras)} Ra values")
@@ -321,7 +321,7 @@ def learned_bench(args, torch, capi):
     nlev = ctx.attach_multigrid(args.mg_coarse)
     base = dict(kspace=args.kspace, max_its=args.max_its, max_newton=40, precond=2,
                 mg_nu=args.mg_nu, mg_omega=args.mg_omega, line_search=args.line_search,
-                pc_reuse=args.pc_reuse)
+                pc_reuse=args.pc_reuse, mg_smoother=args.mg_smoother)
     fp64, mixed = dict(base, precision=0), dict(base, precision=1)
     groups = field_groups(ctx, prob)
     log(f"[learned] {prob.name}: N={ctx.N} levels {nlev}, target Ra={target:g}")
@@ -448,10 +448,13 @@ def main(argv=None):
     ap.add_argument("--max-its", type=int, default=200)
     ap.add_argument("--patch-bins", type=int, default=0)
     ap.add_argument("--precond", choices=["mg", "ilu"], default="mg")
-    ap.add_argument("--mg-nu", type=int, default=1)
+    ap.add_argument("--mg-nu", type=int, default=None,
+                    help="smoothing sweeps per level (default: 2 in 2D, 1 in 3D)")
     ap.add_argument("--mg-omega", type=float, default=0.7)
     ap.add_argument("--line-search", type=int, default=1, help="1: Goma's backtracking line search")
     ap.add_argument("--mg-coarse", type=int, default=2, help="coarsest elements per side")
+    ap.add_argument("--mg-smoother", type=int, default=1,
+                    help="element-block smoother: 0 multiplicative, 1 additive (one residual per sweep)")
     ap.add_argument("--pc-reuse", type=int, default=0,
                     help="preconditioner reuse across Newton passes: 0 recalc, 1 reuse, 2 adaptive")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -468,6 +471,8 @@ def main(argv=None):
                     help="learned workload: log10 Ra spacing of the sweep")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args(argv)
+    if args.mg_nu is None:  # measured: additive V(2,2) in 2D, V(1,1) in 3D
+        args.mg_nu = 1 if args.workload in ("cavity3d", "learned") else 2
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
@@ -501,7 +506,7 @@ def main(argv=None):
     x = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
     opts = dict(kspace=args.kspace, max_its=args.max_its, patch_bins=bins, precision=prec,
                 max_newton=40, verbose=1 if args.verbose else 0, precond=pc, mg_nu=args.mg_nu,
-                mg_omega=args.mg_omega, line_search=args.line_search, pc_reuse=args.pc_reuse)
+                mg_omega=args.mg_omega, line_search=args.line_search, pc_reuse=args.pc_reuse, mg_smoother=args.mg_smoother)
     for w in range(args.warmup):
         x.zero_()
         torch.cuda.synchronize()
@@ -571,7 +576,7 @@ def main(argv=None):
         "data": "synthetic",
         "config": {"workload": f"{prob.name} ({args.workload}) " + ", ".join(f"{k}={v}" for k, v in params.items()),
                    "unknowns": ctx.N, "nnz": ctx.nval - 1, "elements": prob.mesh.num_elems,
-                   "precision": args.precision, "solver": f"GMRES({args.kspace}) + " + ("multigrid V(%d,%d) element-block smoothing, %d levels, exact coarsest solve" % (args.mg_nu, args.mg_nu, nlev) if pc == 2 else "ILU(0) patch order") + ", preconditioner reuse " + ("recalc", "reuse", "adaptive")[args.pc_reuse],
+                   "precision": args.precision, "solver": f"GMRES({args.kspace}) + " + ("multigrid V(%d,%d) %s element-block smoothing, %d levels, exact coarsest solve" % (args.mg_nu, args.mg_nu, ("multiplicative", "additive")[args.mg_smoother], nlev) if pc == 2 else "ILU(0) patch order") + ", preconditioner reuse " + ("recalc", "reuse", "adaptive")[args.pc_reuse],
                    "initial_guess": "zero"},
         "newton_its": newton,
         "gmres_its": gm_its,

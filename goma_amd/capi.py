@@ -57,7 +57,7 @@ class GmSolverOpts(C.Structure):
                 ("lin_tol", C.c_double), ("precond", C.c_int), ("patch_bins", C.c_int),
                 ("precision", C.c_int), ("verbose", C.c_int), ("mg_nu", C.c_int),
                 ("mg_omega", C.c_double), ("mg_coarse_sweeps", C.c_int), ("line_search", C.c_int),
-                ("min_damp", C.c_double), ("pc_reuse", C.c_int)]
+                ("min_damp", C.c_double), ("pc_reuse", C.c_int), ("mg_smoother", C.c_int)]
 
 
 MAX_REPORT = 64

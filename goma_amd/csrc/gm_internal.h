@@ -139,6 +139,7 @@ struct gm_ctx {
   MgLevel mg0;              // level-0 vectors and element inverses
   const double *mg_scale = nullptr;  // row-sum scale of the level-0 matrix
   int *d_mg_rows = nullptr;          // unknowns of each element colour (smoother residuals)
+  double *d_mg_wmul = nullptr;       // 1 / elements sharing each unknown (additive smoother)
   std::vector<int> mg_rows_ptr;
   std::vector<long long> mg_rows_nnz;
 

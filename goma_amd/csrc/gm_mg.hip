@@ -301,7 +301,7 @@ template <int DIM, int EN, typename AT, typename BT, int NT>
 __global__ __launch_bounds__(NT) void k_vanka_update(LevelDev L, int nelem, const int *__restrict__ elems,
                                                      const AT *__restrict__ ainv,
                                                      const BT *__restrict__ r, BT *__restrict__ x,
-                                                     double omega) {
+                                                     double omega, const double *__restrict__ wmul) {
   using Lc = Loc<DIM, EN>;
   constexpr int n = Lc::N;
   // two elements per workgroup when a block fits in half of it (2D: 30 of 64 lanes)
@@ -325,7 +325,8 @@ __global__ __launch_bounds__(NT) void k_vanka_update(LevelDev L, int nelem, cons
     for (int i = t; i < n; i += HALF) {
       double s = 0.0;
       for (int j = 0; j < n; j++) s += (double)M[j * n + i] * s_r2[sub][j];
-      x[s_g2[sub][i]] += (BT)(omega * s);
+      const int g = s_g2[sub][i];
+      x[g] += (BT)(omega * (wmul ? wmul[g] : 1.0) * s);
     }
   }
 }
@@ -510,6 +511,17 @@ static int build_colour_rows(gm_ctx *lc) {
     lc->mg_rows_ptr.push_back((int)rows.size());
     lc->mg_rows_nnz.push_back(nnz);
   }
+  // multiplicity of every unknown among the element blocks
+  std::vector<double> w(lc->N, 0.0);
+  for (int e = 0; e < lc->num_elems; e++)
+    for (int i = 0; i < nve; i++) {
+      const int n = lc->conn[(size_t)e * nve + i];
+      const int cnt = (i == cl) ? lc->first_unknown[n + 1] - lc->first_unknown[n] : lc->nv;
+      for (int a = 0; a < cnt; a++) w[lc->first_unknown[n] + a] += 1.0;
+    }
+  for (double &v : w) v = v > 0.0 ? 1.0 / v : 0.0;
+  int rc = upv(&lc->d_mg_wmul, w.data(), w.size());
+  if (rc) return rc;
   return upv(&lc->d_mg_rows, rows.data(), rows.size());
 }
 
@@ -650,7 +662,7 @@ static void resid_rows(gm_ctx *lc, hipStream_t s, int col, const VT *a, const BT
 
 template <int DIM, int EN, typename AT, typename BT>
 static void vanka_colour(gm_ctx *lc, hipStream_t s, int col, const AT *ainv, const BT *r, BT *x,
-                         double omega) {
+                         double omega, const double *wmul) {
   constexpr int NT = DIM == 2 ? 64 : 128;
   constexpr int n = Loc<DIM, EN>::N;
   const int cnt = lc->ecol_ptr[col + 1] - lc->ecol_ptr[col];
@@ -661,23 +673,30 @@ static void vanka_colour(gm_ctx *lc, hipStream_t s, int col, const AT *ainv, con
   constexpr int EPB = (2 * n <= NT) ? 2 : 1;
   gm_launch(g_prof_ctx, GM_K_VANKA, bytes, k_vanka_update<DIM, EN, AT, BT, NT>, dim3((cnt + EPB - 1) / EPB), dim3(NT),
             0, s, level_dev(lc), cnt, (const int *)(lc->d_ecol_elems + lc->ecol_ptr[col]), ainv, r, x,
-            omega);
+            omega, wmul);
 }
 
+// Multiplicative: coloured block Gauss-Seidel, the residual recomputed on each
+// colour's rows.  Additive: one residual per sweep, every element's correction
+// from it, weighted by 1 / elements sharing the unknown (colours only keep the
+// writes apart).
 template <typename VT, typename BT>
 static void smooth(gm_ctx *lc, hipStream_t s, const VT *a, const VT *ainv, const BT *b, BT *x, BT *r,
-                   int sweeps, double omega) {
-  for (int it = 0; it < sweeps; it++)
+                   int sweeps, double omega, int additive) {
+  const double *w = additive ? lc->d_mg_wmul : nullptr;
+  for (int it = 0; it < sweeps; it++) {
+    if (additive) resid<VT, BT>(lc, s, a, x, b, r);
     for (int col = 0; col < lc->n_ecol; col++) {
-      resid_rows<VT, BT>(lc, s, col, a, x, b, r);
+      if (!additive) resid_rows<VT, BT>(lc, s, col, a, x, b, r);
       if (lc->dim == 2) {
-        if (lc->energy) vanka_colour<2, 1, VT, BT>(lc, s, col, ainv, r, x, omega);
-        else vanka_colour<2, 0, VT, BT>(lc, s, col, ainv, r, x, omega);
+        if (lc->energy) vanka_colour<2, 1, VT, BT>(lc, s, col, ainv, r, x, omega, w);
+        else vanka_colour<2, 0, VT, BT>(lc, s, col, ainv, r, x, omega, w);
       } else {
-        if (lc->energy) vanka_colour<3, 1, VT, BT>(lc, s, col, ainv, r, x, omega);
-        else vanka_colour<3, 0, VT, BT>(lc, s, col, ainv, r, x, omega);
+        if (lc->energy) vanka_colour<3, 1, VT, BT>(lc, s, col, ainv, r, x, omega, w);
+        else vanka_colour<3, 0, VT, BT>(lc, s, col, ainv, r, x, omega, w);
       }
     }
+  }
 }
 
 // z = M^-1 v: one V-cycle.  VT: matrix / inverse type, BT: vector type.
@@ -719,7 +738,7 @@ static void vcycle(gm_ctx *c, const VT *a0, const BT *v, BT *z, const gm_solver_
     }
     k_set<BT><<<1024, 256, 0, s>>>(x, lc->N, (BT)0);
     smooth<VT, BT>(lc, s, amat(l), (const VT *)L.d_ainv, b, x, (BT *)L.d_r,
-                   l == nl ? (cs > 0 ? cs : 20) : nu, om);
+                   l == nl ? (cs > 0 ? cs : 20) : nu, om, o->mg_smoother == 1);
     if (l == nl) break;
     resid<VT, BT>(lc, s, amat(l), x, b, (BT *)L.d_r);
     MgLevel &C = c->mg[l];
@@ -735,7 +754,8 @@ static void vcycle(gm_ctx *c, const VT *a0, const BT *v, BT *z, const gm_solver_
     const BT *b = l == 0 ? b0 : (const BT *)L.d_b;
     transfer<BT>(c, s, lc->N, C.d_Prow, C.d_Pcol, C.d_Pval, C.nnz, C.ctx->N, (const BT *)C.d_x,
                  nullptr, lc->d_dbc, x, 1);
-    smooth<VT, BT>(lc, s, amat(l), (const VT *)L.d_ainv, b, x, (BT *)L.d_r, nu, om);
+    smooth<VT, BT>(lc, s, amat(l), (const VT *)L.d_ainv, b, x, (BT *)L.d_r, nu, om,
+                   o->mg_smoother == 1);
   }
 }
 

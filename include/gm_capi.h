@@ -103,6 +103,10 @@ typedef struct gm_solver_opts {
                           1 "reuse": build it at the first pass and keep it;
                           2 adaptive: keep it until GMRES misses lin_tol or needs
                             more than 1.5x the iterations of its first use */
+  int mg_smoother;     /* multigrid element-block smoother: 0 multiplicative (coloured block
+                          Gauss-Seidel, a residual on each colour's rows); 1 additive (one
+                          residual per sweep, corrections weighted by 1 / elements sharing
+                          the unknown) */
 } gm_solver_opts;
 
 #define GM_MAX_REPORT 64

@@ -186,11 +186,12 @@ def test_newton_matches_oracle(prec, capi, oracle_mod):
     ctx.close()
 
 
+@pytest.mark.parametrize("smoother", [0, 1])
 @pytest.mark.parametrize("name", ["natconv2d", "cavity3d_energy"])
-def test_multigrid_gmres_true_residual(name, capi, oracle_mod):
+def test_multigrid_gmres_true_residual(name, smoother, capi, oracle_mod):
     prob = (problems.natural_convection(n=8, Ra=1e4) if name == "natconv2d"
             else problems.lid_cavity(n=8, Re=100.0, dim=3, energy=True, Ra=1e3))
-    print(f"[case] multigrid gmres {name}", flush=True)
+    print(f"[case] multigrid gmres {name} smoother={smoother}", flush=True)
     ctx = capi.Context(prob)
     nl = ctx.attach_multigrid(2 if name == "natconv2d" else 4)
     assert nl >= 2
@@ -203,7 +204,7 @@ def test_multigrid_gmres_true_residual(name, capi, oracle_mod):
     ctx.mg_setup_dev(xd.data_ptr(), vd.data_ptr(), sd.data_ptr())
     zd = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
     its, rr = ctx.gmres_dev(vd.data_ptr(), bd.data_ptr(), zd.data_ptr(), kspace=50, max_its=200,
-                            lin_tol=1e-8, precond=2)
+                            lin_tol=1e-8, precond=2, mg_smoother=smoother)
     A = oracle_mod.msr_to_scipy(orc.ija, v, orc.N)
     z = zd.cpu().numpy()
     assert np.linalg.norm(r - A @ z) <= 1.01e-8 * np.linalg.norm(r)
@@ -211,10 +212,11 @@ def test_multigrid_gmres_true_residual(name, capi, oracle_mod):
     ctx.close()
 
 
+@pytest.mark.parametrize("smoother", [0, 1])
 @pytest.mark.parametrize("prec", [0, 1])
-def test_newton_multigrid_matches_oracle(prec, capi, oracle_mod):
+def test_newton_multigrid_matches_oracle(prec, smoother, capi, oracle_mod):
     prob = problems.natural_convection(n=8, Ra=1e4)
-    print(f"[case] newton multigrid natconv n=8 prec={prec}", flush=True)
+    print(f"[case] newton multigrid natconv n=8 prec={prec} smoother={smoother}", flush=True)
     ctx = capi.Context(prob)
     ctx.attach_multigrid(2)
     orc = oracle_mod.Oracle(prob)
@@ -223,7 +225,7 @@ def test_newton_multigrid_matches_oracle(prec, capi, oracle_mod):
     xd = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
     rc, rep = ctx.solve_dev(xd.data_ptr(), kspace=50, max_its=200, precond=2, precision=prec,
-                            line_search=1)
+                            line_search=1, mg_smoother=smoother, mg_nu=2 if smoother else 1)
     assert rep.converged
     x = xd.cpu().numpy()
     assert np.abs(x - x_ref).max() <= 1e-8 * np.abs(x_ref).max()

@@ -159,6 +159,144 @@ __global__ __launch_bounds__(NT) void k_vanka_invert(LevelDev L, const VT *__res
   }
 }
 
+// The same inverse with the block held in registers: thread (tr, tc) of a
+// TR x TC grid owns rows tr + TR a and columns tc + TC b.  Per pivot step only
+// the pivot column, the pivot row and the old row k pass through LDS (RA + 2 CB
+// reads per thread instead of three per entry); register arrays are indexed
+// at compile time only, the owner row / column selected by predicates.  The
+// row interchanges are undone by writing column c of the result to position
+// pos[c] of the output.
+template <int DIM, int EN, typename VT, typename AT, typename CT, int TR, int TC>
+__global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const VT *__restrict__ val,
+                                                             AT *__restrict__ ainv) {
+  using Lc = Loc<DIM, EN>;
+  constexpr int n = Lc::N;
+  constexpr int RA = (n + TR - 1) / TR, CB = (n + TC - 1) / TC;
+  __shared__ CT s_col[n], s_row[n], s_rowk[n];
+  __shared__ int s_piv[n], s_idx[n], s_pos[n];
+  __shared__ int s_p;
+  const int e = blockIdx.x, tid = threadIdx.x;
+  const int tr = tid % TR, tc = tid / TR;
+  const int *map = L.e2nz + (size_t)e * L.E;
+  CT A[RA][CB];
+#pragma unroll
+  for (int a = 0; a < RA; a++)
+#pragma unroll
+    for (int b = 0; b < CB; b++) {
+      const int i = tr + TR * a, j = tc + TC * b;
+      const long long p = (i < n && j < n) ? local_pos<DIM, EN>(L, i, j) : -1;
+      A[a][b] = p >= 0 ? (CT)val[map[p]] : (CT)0;
+    }
+  for (int k = 0; k < n; k++) {
+    const int bk = k / TC, ck = k - bk * TC;
+    if (tc == ck) {
+#pragma unroll
+      for (int a = 0; a < RA; a++) {
+        CT v = 0;
+#pragma unroll
+        for (int b = 0; b < CB; b++)
+          if (b == bk) v = A[a][b];
+        const int i = tr + TR * a;
+        if (i < n) s_col[i] = v;
+      }
+    }
+    __syncthreads();
+    if (tid < 64) {
+      CT best = (CT)-1;
+      int bi = n;
+      for (int i = k + tid; i < n; i += 64) {
+        const CT v = fabs(s_col[i]);
+        if (v > best) {
+          best = v;
+          bi = i;
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const CT ov = __shfl_xor(best, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ov > best || (ov == best && oi < bi)) {
+          best = ov;
+          bi = oi;
+        }
+      }
+      if (tid == 0) {
+        if (bi >= n) bi = k;  // no comparable candidate (NaN column): keep the row
+        s_p = bi;
+        s_piv[k] = bi;
+      }
+    }
+    __syncthreads();
+    const int p = s_p;
+    const int ap = p / TR, rp = p - ap * TR, ak = k / TR, rk = k - ak * TR;
+    if (tr == rp || tr == rk) {
+#pragma unroll
+      for (int b = 0; b < CB; b++) {
+        CT vp = 0, vk = 0;
+#pragma unroll
+        for (int a = 0; a < RA; a++) {
+          if (a == ap) vp = A[a][b];
+          if (a == ak) vk = A[a][b];
+        }
+        const int j = tc + TC * b;
+        if (j < n) {
+          if (tr == rp) s_row[j] = vp;
+          if (tr == rk) s_rowk[j] = vk;
+        }
+      }
+    }
+    __syncthreads();
+    const CT piv = s_row[k];
+    // a structurally empty column (a local pressure mode with no velocity
+    // coupling inside the element) leaves that unknown untouched
+    const CT d = fabs(piv) > (CT)1e-300 ? (CT)1 / piv : (CT)0;
+    const CT fk = s_col[k];
+    CT rw[CB];
+#pragma unroll
+    for (int b = 0; b < CB; b++) {
+      const int j = tc + TC * b;
+      rw[b] = j < n ? s_row[j] : (CT)0;
+    }
+#pragma unroll
+    for (int a = 0; a < RA; a++) {
+      const int i = tr + TR * a;
+      if (i >= n) continue;
+      const CT f = i == p ? fk : s_col[i];
+      const bool take_k = (i == p) && (p != k);
+#pragma unroll
+      for (int b = 0; b < CB; b++) {
+        const int j = tc + TC * b;
+        if (j >= n) continue;
+        const CT cur = take_k ? s_rowk[j] : A[a][b];
+        CT nv;
+        if (i == k) nv = j == k ? d : rw[b] * d;
+        else nv = j == k ? -f * d : cur - f * (rw[b] * d);
+        A[a][b] = nv;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    for (int j = 0; j < n; j++) s_idx[j] = j;
+    for (int k = n - 1; k >= 0; k--) {
+      const int p = s_piv[k];
+      const int t = s_idx[k];
+      s_idx[k] = s_idx[p];
+      s_idx[p] = t;
+    }
+    for (int j = 0; j < n; j++) s_pos[s_idx[j]] = j;
+  }
+  __syncthreads();
+  AT *out = ainv + (size_t)e * n * n;
+#pragma unroll
+  for (int a = 0; a < RA; a++)
+#pragma unroll
+    for (int b = 0; b < CB; b++) {
+      const int i = tr + TR * a, c = tc + TC * b;
+      if (i < n && c < n) out[(size_t)s_pos[c] * n + i] = (AT)A[a][b];
+    }
+}
+
 // r = b - A x over all rows (MSR), G lanes per row
 template <typename VT, typename BT, int G>
 __global__ __launch_bounds__(256) void k_mg_resid(int N, const VT *__restrict__ a,
@@ -465,13 +603,17 @@ static int alloc_level_vectors(MgLevel &L, const gm_ctx *lc, int prec, bool coar
 
 template <int DIM, int EN, typename VT, typename AT>
 static void launch_invert(gm_ctx *prof, gm_ctx *lc, hipStream_t s, const VT *val, AT *ainv) {
-  // 3D blocks (n = 85 / 112) fill one CU's LDS in FP64: eight waves per block
-  constexpr int NT = DIM == 2 ? 128 : 512;
   constexpr int n = Loc<DIM, EN>::N;
   // gathered entries + map + inverses written
   const double bytes = (double)lc->num_elems * n * n * (sizeof(VT) + 4 + sizeof(AT));
-  gm_launch(prof, GM_K_VANKA_INV, bytes, k_vanka_invert<DIM, EN, VT, AT, AT, NT>,
-            dim3(lc->num_elems), dim3(NT), 0, s, level_dev(lc), val, ainv);
+  if constexpr (DIM == 3) {
+    // 3D blocks (n = 85 / 112): in registers, 32 x 16 threads, at most 4 x 7 entries each
+    gm_launch(prof, GM_K_VANKA_INV, bytes, k_vanka_invert_reg<DIM, EN, VT, AT, AT, 32, 16>,
+              dim3(lc->num_elems), dim3(512), 0, s, level_dev(lc), val, ainv);
+  } else {
+    gm_launch(prof, GM_K_VANKA_INV, bytes, k_vanka_invert<DIM, EN, VT, AT, AT, 128>,
+              dim3(lc->num_elems), dim3(128), 0, s, level_dev(lc), val, ainv);
+  }
 }
 
 template <typename VT>

@@ -199,8 +199,10 @@ def sweep_bench(args, torch, capi):
     from goma_amd import continuation, learned, problems
 
     n = args.n or 128
-    # the shear-thinning flow defeats deep re-discretised hierarchies: stop at 16^2 (solved exactly)
-    args.mg_coarse = max(args.mg_coarse, 16)
+    gal = 1 if args.mg_galerkin is None else args.mg_galerkin
+    if not gal:
+        # the shear-thinning flow defeats deep re-discretised hierarchies: stop at 16^2 (solved exactly)
+        args.mg_coarse = max(args.mg_coarse, 16)
     prob = problems.natural_convection(n=n, Ra=1e3, carreau=True, jitter=0.15)
     ctx = capi.Context(prob)
     nlev = ctx.attach_multigrid(args.mg_coarse)
@@ -208,7 +210,8 @@ def sweep_bench(args, torch, capi):
     # shear-thinning flow weakens the V(1,1) cycle as Ra grows: V(2,2), larger Krylov space
     opts = dict(kspace=max(args.kspace, 100), max_its=max(args.max_its, 400), precision=prec,
                 max_newton=25, precond=2, mg_nu=max(args.mg_nu, 2), mg_omega=args.mg_omega,
-                line_search=args.line_search, pc_reuse=args.pc_reuse, mg_smoother=0)
+                line_search=args.line_search, pc_reuse=args.pc_reuse, mg_smoother=0,
+                mg_galerkin=gal)
     log = lambda s: print(s, file=sys.stderr, flush=True)
     ras = problems.sweep_ra(args.sweep_count, 1e3, 1e6)
     log(f"[sweep] {prob.name}: N={ctx.N} levels {nlev}, {len(ras)} Ra values")
@@ -457,6 +460,8 @@ def main(argv=None):
     ap.add_argument("--mg-coarse", type=int, default=2, help="coarsest elements per side")
     ap.add_argument("--mg-smoother", type=int, default=1,
                     help="element-block smoother: 0 multiplicative, 1 additive (one residual per sweep)")
+    ap.add_argument("--mg-galerkin", type=int, default=None,
+                    help="1: Galerkin R A P coarse operators (default: sweep 1, others 0)")
     ap.add_argument("--pc-reuse", type=int, default=0,
                     help="preconditioner reuse across Newton passes: 0 recalc, 1 reuse, 2 adaptive")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -508,7 +513,8 @@ def main(argv=None):
     x = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
     opts = dict(kspace=args.kspace, max_its=args.max_its, patch_bins=bins, precision=prec,
                 max_newton=40, verbose=1 if args.verbose else 0, precond=pc, mg_nu=args.mg_nu,
-                mg_omega=args.mg_omega, line_search=args.line_search, pc_reuse=args.pc_reuse, mg_smoother=args.mg_smoother)
+                mg_omega=args.mg_omega, line_search=args.line_search, pc_reuse=args.pc_reuse, mg_smoother=args.mg_smoother,
+                mg_galerkin=args.mg_galerkin or 0)
     for w in range(args.warmup):
         x.zero_()
         torch.cuda.synchronize()

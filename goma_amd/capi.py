@@ -30,7 +30,8 @@ GM_PC_ILU0 = 1
 GM_PC_MG = 2
 KERNEL_NAMES = ["k_fill", "k_spmv_msr", "k_ilu_lower", "k_ilu_upper", "k_ilu_factor", "k_mdot",
                 "k_orth_update", "k_mg_resid", "k_vanka_update", "k_csr_apply", "k_vanka_invert",
-                "k_fill(residual only)", "k_mg_resid_rows", "k_dense_gemv"]
+                "k_fill(residual only)", "k_mg_resid_rows", "k_dense_gemv", "k_galerkin",
+                "k_elem_scatter"]
 
 
 class GmMesh(C.Structure):
@@ -57,7 +58,8 @@ class GmSolverOpts(C.Structure):
                 ("lin_tol", C.c_double), ("precond", C.c_int), ("patch_bins", C.c_int),
                 ("precision", C.c_int), ("verbose", C.c_int), ("mg_nu", C.c_int),
                 ("mg_omega", C.c_double), ("mg_coarse_sweeps", C.c_int), ("line_search", C.c_int),
-                ("min_damp", C.c_double), ("pc_reuse", C.c_int), ("mg_smoother", C.c_int)]
+                ("min_damp", C.c_double), ("pc_reuse", C.c_int), ("mg_smoother", C.c_int),
+                ("mg_galerkin", C.c_int)]
 
 
 MAX_REPORT = 64
@@ -114,7 +116,7 @@ EXPORTS = [
     "gm_matrix_fill_full_dev", "gm_matrix_fill_full", "gm_row_sum_scale_dev", "gm_msr_matvec_dev",
     "gm_gmres_dev", "gm_ilu0_factor_dev", "gm_solve_nonlinear_dev", "gm_solve_nonlinear",
     "gm_default_solver_opts", "gm_prof_enable", "gm_prof_get", "gm_mg_add_level",
-    "gm_mg_num_levels", "gm_mg_setup_dev", "gm_solve_nonlinear_guess_dev",
+    "gm_mg_num_levels", "gm_mg_setup_dev", "gm_mg_copy_level_matrix", "gm_solve_nonlinear_guess_dev",
     "gm_mlp_create", "gm_mlp_destroy", "gm_mlp_num_params", "gm_mlp_get_params",
     "gm_mlp_set_params", "gm_mlp_default_train_opts", "gm_mlp_set_data", "gm_mlp_train",
     "gm_mlp_loss_grad", "gm_mlp_predict_dev", "gm_mlp_predict", "gm_mlp_guess_dev",
@@ -160,6 +162,7 @@ def lib():
     L.gm_mg_add_level.argtypes = [vp, vp, vp, vp, vp, vp]
     L.gm_mg_num_levels.argtypes = [vp]
     L.gm_mg_setup_dev.argtypes = [vp, vp, vp, vp]
+    L.gm_mg_copy_level_matrix.argtypes = [vp, ip, vp]
     L.gm_prof_enable.argtypes = [vp, ip]
     L.gm_prof_get.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.gm_solve_nonlinear_guess_dev.argtypes = [vp, vp, vp, C.POINTER(GmSolverOpts), ip,
@@ -357,6 +360,13 @@ class Context:
             self._coarse.append(cc)
             above = cc
         return lib().gm_mg_num_levels(self.h)
+
+    def mg_level_matrix(self, level: int) -> np.ndarray:
+        """FP64 host copy of the Jacobian of multigrid level ``level`` (1 = first
+        coarse level) from the last set-up, in that level's MSR layout."""
+        out = np.zeros(self._coarse[level - 1].nval)
+        _check(lib().gm_mg_copy_level_matrix(self.h, level, out.ctypes.data), "gm_mg_copy_level_matrix")
+        return out
 
     def mg_setup_dev(self, x_ptr, val_ptr, scale_ptr=None):
         _check(lib().gm_mg_setup_dev(self.h, x_ptr, val_ptr, scale_ptr), "gm_mg_setup_dev")

@@ -95,6 +95,7 @@ extern "C" void gm_default_solver_opts(gm_solver_opts *o) {
   o->min_damp = 0.05;
   o->pc_reuse = 0;
   o->mg_smoother = 0;
+  o->mg_galerkin = 0;
 }
 
 static void free_ctx(gm_ctx *c) {
@@ -104,7 +105,7 @@ static void free_ctx(gm_ctx *c) {
   void *ptrs[] = {c->d_coor, c->d_conn,   c->d_fu,    c->d_centroid, c->d_dbc,   c->d_dbc_val,
                   c->d_bindx, c->d_ecol_elems, c->d_e2nz, c->d_flag, c->d_tables, c->d_val,
                   c->d_resid, c->d_dx,    c->d_scale, c->d_val32,    c->d_red,   c->d_xwork,
-                  c->d_xsave, c->d_rtrial, c->d_w, c->d_mg_rows, c->d_mg_wmul};
+                  c->d_xsave, c->d_rtrial, c->d_w, c->d_mg_rows, c->d_mg_wmul, c->d_kel};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->h_red) hipHostFree(c->h_red);
@@ -412,6 +413,8 @@ static int solve_impl(gm_ctx *c, double *x, const gm_solver_opts *o, gm_newton_r
   int converged = 0, inewton = 0;
   bool have_pc = false, last_ok = true;  // preconditioner reuse state
   int its_ref = 0, last_its = 0;
+  const bool galerkin = o->precond == GM_PC_MG && o->mg_galerkin && !c->mg.empty();
+  if (galerkin && (rc = gm_mg_kel_alloc(c))) return rc;
   while (!converged && inewton < o->max_newton) {
     double ta = now_s();
     if ((rc = gm_zero(c, c->d_resid, c->N))) return rc;
@@ -419,7 +422,9 @@ static int solve_impl(gm_ctx *c, double *x, const gm_solver_opts *o, gm_newton_r
     if (prec == GM_PREC_MIXED) rc = gm_zero32(c, c->d_val32, c->nval);
     else rc = gm_zero(c, c->d_val, c->nval);
     if (rc) return rc;
+    c->kel_on = galerkin;
     rc = gm_fill_launch(c, x, c->d_resid, c->d_val, prec == GM_PREC_MIXED ? c->d_val32 : nullptr);
+    c->kel_on = 0;
     if (rc) return rc;
     double un, unoo;
     if ((rc = gm_norms(c, c->d_resid, c->N, &un, &unoo))) return rc;
@@ -467,7 +472,7 @@ static int solve_impl(gm_ctx *c, double *x, const gm_solver_opts *o, gm_newton_r
         if ((rc = gm_ilu_factor(c, c->d_val, c->d_val32, prec))) return rc;
       } else if (rebuild && o->precond == GM_PC_MG) {
         if ((rc = gm_mg_setup(c, x, c->d_val, c->d_val32, prec, c->d_scale,
-                              o->mg_coarse_sweeps <= 0)))
+                              o->mg_coarse_sweeps <= 0, galerkin)))
           return rc;
         GM_HIP(hipStreamSynchronize(c->stream));
       }

@@ -50,6 +50,8 @@ struct FillArgs {
   const double *x;
   double *resid;
   void *val;
+  double *kel;  // optional [num_elems][E]: element matrices before the Dirichlet rows
+                // (for Galerkin multigrid coarse operators), in the e2nz layout
   int *flag;
   int num_nodes;
   int resid_only;  // 1: residual only (Assemble_Jacobian = FALSE)
@@ -347,6 +349,13 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
         acc[DIM][DIM] += dA * (-rCp * phi_i * advj - kc * GG);
       }
     }
+    if (A.kel) {
+      double *__restrict__ ke = A.kel + (size_t)e * A.E;
+#pragma unroll
+      for (int a = 0; a < NV; a++)
+#pragma unroll
+        for (int b = 0; b < NV; b++) ke[(a * NV + b) * NVE2 + t] = acc[a][b];
+    }
 #pragma unroll
     for (int a = 0; a < NV; a++) {
       const int row = s_gun[a][i];
@@ -377,6 +386,7 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
     }
 #pragma unroll
     for (int a = 0; a < DIM; a++) {
+      if (A.kel) A.kel[(size_t)e * A.E + A.off_mp + a * NVE * NP + t] = acc[a];
       if (A.dbc[s_gun[a][i]]) continue;
       const int idx = map[A.off_mp + a * NVE * NP + t];
       out[idx] += (JT)acc[a];
@@ -386,7 +396,8 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
   // 5d. continuity x velocity (J_c_v): int psi_k grad_b phi_j dA
   for (int t = tid; t < NP * NVE; t += NT) {
     const int k = t / NVE, j = t - k * NVE;
-    if (A.dbc[s_gp[k]]) continue;
+    const bool dir = A.dbc[s_gp[k]] != 0;
+    if (dir && !A.kel) continue;
     double acc[DIM];
 #pragma unroll
     for (int b = 0; b < DIM; b++) acc[b] = 0.0;
@@ -397,6 +408,8 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
     }
 #pragma unroll
     for (int b = 0; b < DIM; b++) {
+      if (A.kel) A.kel[(size_t)e * A.E + A.off_cv + b * NP * NVE + t] = acc[b];
+      if (dir) continue;
       const int idx = map[A.off_cv + b * NP * NVE + t];
       out[idx] += (JT)acc[b];
     }
@@ -406,6 +419,7 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
   // Dirichlet (datum) row
   if (tid < NP * NP) {
     const int k = tid / NP, k2 = tid - k * NP;
+    if (A.kel) A.kel[(size_t)e * A.E + A.off_cp + tid] = 0.0;
     if (k == k2 && A.dbc[s_gp[k]]) {
       const int idx = map[A.off_cp + tid];
       out[idx] += (JT)1.0;
@@ -627,6 +641,7 @@ int gm_fill_launch_ex(gm_ctx *c, const double *x, double *resid, double *val, fl
   a.x = x;
   a.resid = resid;
   a.val = val32 ? (void *)val32 : (void *)val;
+  a.kel = c->kel_on ? c->d_kel : nullptr;
   a.flag = c->d_flag;
   a.num_nodes = c->num_nodes;
   a.E = c->lay.total;

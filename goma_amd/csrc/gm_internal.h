@@ -78,6 +78,14 @@ struct MgLevel {
   int *d_Prow = nullptr, *d_Pcol = nullptr, *d_Rrow = nullptr, *d_Rcol = nullptr;
   double *d_Pval = nullptr, *d_Rval = nullptr;
   int *d_inject = nullptr;    // coarse node -> finer node at the same position
+  // element-local form of P, for Galerkin coarse operators: the finer elements
+  // inside each coarse element, and per finer element the nodal weights
+  // W[i][m] (finer local node i <- coarse local node m) and P1 weights Q[k][l]
+  int *d_child = nullptr;     // [coarse elems][2^dim]
+  double *d_W = nullptr;      // [finer elems][nve * nve]
+  double *d_Q = nullptr;      // [finer elems][np * np]
+  bool gal_ok = false;
+  std::string gal_err;
   double *d_state = nullptr;  // injected state for re-assembly
   double *d_rdummy = nullptr; // residual written by the coarse assembly (unused)
   void *d_x = nullptr, *d_b = nullptr, *d_r = nullptr;  // level vectors
@@ -127,6 +135,8 @@ struct gm_ctx {
 
   // Newton work vectors
   double *d_val = nullptr, *d_resid = nullptr, *d_dx = nullptr, *d_scale = nullptr;
+  double *d_kel = nullptr;  // element matrices before Dirichlet rows [num_elems][lay.total]
+  int kel_on = 0;           // the next assembly also writes d_kel
   float *d_val32 = nullptr;
   double *d_red = nullptr;      // reduction scratch
   double *h_red = nullptr;      // pinned host mirror
@@ -189,7 +199,8 @@ void gm_free_linalg(gm_ctx *c);
 
 // multigrid (gm_mg.hip)
 int gm_mg_setup(gm_ctx *c, const double *x, const double *val, const float *val32, int prec,
-                const double *scale, int coarse_direct);
+                const double *scale, int coarse_direct, int galerkin);
+int gm_mg_kel_alloc(gm_ctx *c);  // element-matrix buffer of the finest level
 int gm_mg_apply_d(gm_ctx *c, const double *a0, const double *v, double *z, const gm_solver_opts *o);
 int gm_mg_apply_f(gm_ctx *c, const float *a0, const double *v, double *z, const gm_solver_opts *o);
 void gm_mg_free(gm_ctx *c);

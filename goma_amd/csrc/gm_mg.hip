@@ -530,6 +530,287 @@ static int upv(T **d, const T *h, size_t n) {
   return GM_OK;
 }
 
+// Galerkin coarse operators.  When every finer element lies inside one
+// coarser element (its parent) and P maps the parent's unknowns to the
+// element's unknowns, R A P = sum over finer elements of P_e^T K_e P_e, with K_e
+// the element matrix before Dirichlet rows and P_e the element-local block of
+// P.  The coarse element matrix of element E is then the sum over its 2^dim
+// children, and the coarse level assembles it like any element matrix
+// (the same element-to-nonzero map, its own Dirichlet rows).  P_e is one
+// nodal weight matrix W (the same for every nodal variable) and one P1 matrix Q.
+static std::string build_galerkin_maps(const gm_ctx *f, const gm_ctx *c, const int *Prow,
+                                       const int *Pcol, const double *Pval, MgLevel &L) {
+  const int nve = f->nve, np = f->np, nv = f->nv, nch = 1 << f->dim;
+  std::vector<int> node_of(c->N, -1);
+  for (int n = 0; n < c->num_nodes; n++)
+    for (int k = c->first_unknown[n]; k < c->first_unknown[n + 1]; k++) node_of[k] = n;
+  std::vector<int> elem_of_centroid(c->num_nodes, -1);
+  for (int e = 0; e < c->num_elems; e++) elem_of_centroid[c->elem_centroid[e]] = e;
+  std::vector<int> child((size_t)c->num_elems * nch, -1), nchild(c->num_elems, 0);
+  std::vector<double> W((size_t)f->num_elems * nve * nve, 0.0);
+  std::vector<double> Q((size_t)f->num_elems * np * np, 0.0);
+  for (int e = 0; e < f->num_elems; e++) {
+    // parent: the coarse element whose centroid node enters the interpolation
+    // of this element's centroid
+    const int ur = f->first_unknown[f->elem_centroid[e]];
+    int parent = -1;
+    for (int k = Prow[ur]; k < Prow[ur + 1]; k++) {
+      const int pe = elem_of_centroid[node_of[Pcol[k]]];
+      if (pe < 0 || Pval[k] == 0.0) continue;
+      if (parent >= 0 && parent != pe) return "an element's centroid interpolates from two coarse elements";
+      parent = pe;
+    }
+    if (parent < 0) return "an element has no parent coarse element";
+    if (nchild[parent] >= nch) return "a coarse element has more than 2^dim children";
+    child[(size_t)parent * nch + nchild[parent]++] = e;
+    const int *cn = &c->conn[(size_t)parent * nve];
+    for (int i = 0; i < nve; i++) {
+      const int u = f->first_unknown[f->conn[(size_t)e * nve + i]];  // first nodal variable
+      for (int k = Prow[u]; k < Prow[u + 1]; k++) {
+        const int n = node_of[Pcol[k]];
+        if (Pcol[k] != c->first_unknown[n]) return "P couples different variables";
+        int m = 0;
+        while (m < nve && cn[m] != n) m++;
+        if (m == nve) return "P couples an element to unknowns outside its parent";
+        W[(size_t)e * nve * nve + i * nve + m] = Pval[k];
+      }
+    }
+    const int pf = f->first_unknown[f->elem_centroid[e]] + nv;
+    const int pc = c->first_unknown[c->elem_centroid[parent]] + nv;
+    for (int kk = 0; kk < np; kk++)
+      for (int k = Prow[pf + kk]; k < Prow[pf + kk + 1]; k++) {
+        const int l = Pcol[k] - pc;
+        if (l < 0 || l >= np) return "P couples pressure outside the parent element";
+        Q[(size_t)e * np * np + kk * np + l] = Pval[k];
+      }
+  }
+  for (int e = 0; e < c->num_elems; e++)
+    if (nchild[e] != nch) return "a coarse element without 2^dim children";
+  if (upv(&L.d_child, child.data(), child.size()) || upv(&L.d_W, W.data(), W.size()) ||
+      upv(&L.d_Q, Q.data(), Q.size()))
+    return "device allocation failed";
+  return "";
+}
+
+// Coarse element matrices P_e^T K_e P_e summed over the children of each coarse
+// element (one workgroup per coarse element), in the e2nz layout: nodal blocks
+// (a, b) as [i][j], then momentum x P, continuity x velocity, P x P.
+template <int DIM, int EN>
+__global__ __launch_bounds__(256) void k_galerkin(const double *__restrict__ kf,
+                                                  const int *__restrict__ child,
+                                                  const double *__restrict__ W,
+                                                  const double *__restrict__ Q, long long E,
+                                                  long long off_mp, double *__restrict__ kc) {
+  constexpr int NVE = DIM == 2 ? 9 : 27, NP = DIM + 1, NV = DIM + EN, NCH = 1 << DIM, NT = 256;
+  constexpr int NVE2 = NVE * NVE;
+  constexpr int RN = (NVE2 + NT - 1) / NT;
+  constexpr int NMP = DIM * NVE * NP;          // momentum x P (= continuity x velocity) entries
+  constexpr int NPC = 2 * NMP + NP * NP;       // pressure couplings, contiguous from off_mp
+  constexpr int RP = (NPC + NT - 1) / NT;
+  __shared__ double sW[NVE2], sK[NVE2], sT[NVE2], sQ[NP * NP], sKp[NPC];
+  const int ec = blockIdx.x, tid = threadIdx.x;
+  const int *ch = child + (size_t)ec * NCH;
+  double *__restrict__ out = kc + (size_t)ec * E;
+  for (int blk = 0; blk < NV * NV; blk++) {
+    double acc[RN];
+#pragma unroll
+    for (int r = 0; r < RN; r++) acc[r] = 0.0;
+    for (int sidx = 0; sidx < NCH; sidx++) {
+      const int f = ch[sidx];
+      const double *Kf = kf + (size_t)f * E + (size_t)blk * NVE2;
+      const double *Wf = W + (size_t)f * NVE2;
+      for (int t = tid; t < NVE2; t += NT) {
+        sK[t] = Kf[t];
+        sW[t] = Wf[t];
+      }
+      __syncthreads();
+      for (int t = tid; t < NVE2; t += NT) {  // T = K W
+        const int i = t / NVE, n = t - i * NVE;
+        double v = 0.0;
+        for (int j = 0; j < NVE; j++) v += sK[i * NVE + j] * sW[j * NVE + n];
+        sT[t] = v;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < RN; r++) {  // acc += W^T T
+        const int t = tid + r * NT;
+        if (t < NVE2) {
+          const int m = t / NVE, n = t - m * NVE;
+          double v = 0.0;
+          for (int i = 0; i < NVE; i++) v += sW[i * NVE + m] * sT[i * NVE + n];
+          acc[r] += v;
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < RN; r++) {
+      const int t = tid + r * NT;
+      if (t < NVE2) out[(size_t)blk * NVE2 + t] = acc[r];
+    }
+  }
+  double accp[RP];
+#pragma unroll
+  for (int r = 0; r < RP; r++) accp[r] = 0.0;
+  for (int sidx = 0; sidx < NCH; sidx++) {
+    const int f = ch[sidx];
+    const double *Kf = kf + (size_t)f * E + off_mp;
+    for (int t = tid; t < NVE2; t += NT) sW[t] = W[(size_t)f * NVE2 + t];
+    if (tid < NP * NP) sQ[tid] = Q[(size_t)f * NP * NP + tid];
+    for (int t = tid; t < NPC; t += NT) sKp[t] = Kf[t];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RP; r++) {
+      const int t = tid + r * NT;
+      if (t >= NPC) continue;
+      double v = 0.0;
+      if (t < NMP) {  // row (a, m) momentum, column l pressure
+        const int a = t / (NVE * NP), rem = t - a * NVE * NP, m = rem / NP, l = rem - m * NP;
+        for (int i = 0; i < NVE; i++) {
+          double q = 0.0;
+#pragma unroll
+          for (int k = 0; k < NP; k++) q += sKp[a * NVE * NP + i * NP + k] * sQ[k * NP + l];
+          v += sW[i * NVE + m] * q;
+        }
+      } else if (t < 2 * NMP) {  // row l continuity, column (b, n) velocity
+        const int t2 = t - NMP, b = t2 / (NP * NVE), rem = t2 - b * NP * NVE;
+        const int l = rem / NVE, n = rem - l * NVE;
+#pragma unroll
+        for (int k = 0; k < NP; k++) {
+          double q = 0.0;
+          for (int j = 0; j < NVE; j++) q += sKp[NMP + b * NP * NVE + k * NVE + j] * sW[j * NVE + n];
+          v += sQ[k * NP + l] * q;
+        }
+      } else {  // P x P
+        const int t3 = t - 2 * NMP, l = t3 / NP, l2 = t3 - l * NP;
+#pragma unroll
+        for (int k = 0; k < NP; k++)
+#pragma unroll
+          for (int k2 = 0; k2 < NP; k2++)
+            v += sQ[k * NP + l] * sKp[2 * NMP + k * NP + k2] * sQ[k2 * NP + l2];
+      }
+      accp[r] += v;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < RP; r++) {
+    const int t = tid + r * NT;
+    if (t < NPC) out[off_mp + t] = accp[r];
+  }
+}
+
+// Adds the element matrices of one element colour into an MSR matrix with
+// Goma's per-element Dirichlet rows (a Dirichlet row keeps only a unit
+// diagonal, src/bc_dirich.c:44-153), as the assembly kernel does.
+template <int DIM, typename JT>
+__global__ __launch_bounds__(256) void k_elem_scatter(const double *__restrict__ kel,
+                                                      const int *__restrict__ elems, int nv,
+                                                      const int *__restrict__ conn,
+                                                      const int *__restrict__ fu,
+                                                      const int *__restrict__ centroid,
+                                                      const signed char *__restrict__ dbc,
+                                                      const int *__restrict__ e2nz, long long E,
+                                                      long long off_mp, long long off_cv,
+                                                      long long off_cp, JT *__restrict__ val) {
+  constexpr int NVE = DIM == 2 ? 9 : 27, NP = DIM + 1, NVE2 = NVE * NVE;
+  const int e = elems[blockIdx.x];
+  __shared__ int s_f[NVE];
+  __shared__ int s_pc;
+  if (threadIdx.x < NVE) s_f[threadIdx.x] = fu[conn[(size_t)e * NVE + threadIdx.x]];
+  if (threadIdx.x == 0) s_pc = fu[centroid[e]] + nv;
+  __syncthreads();
+  const double *__restrict__ K = kel + (size_t)e * E;
+  const int *__restrict__ map = e2nz + (size_t)e * E;
+  for (long long idx = threadIdx.x; idx < E; idx += blockDim.x) {
+    int row;
+    bool diag;
+    if (idx < off_mp) {
+      const int blk = (int)(idx / NVE2), t = (int)(idx - (long long)blk * NVE2);
+      const int a = blk / nv, b = blk - a * nv, i = t / NVE, j = t - i * NVE;
+      row = s_f[i] + a;
+      diag = i == j && a == b;
+    } else if (idx < off_cv) {
+      const int r = (int)(idx - off_mp);
+      const int a = r / (NVE * NP), t = r - a * NVE * NP;
+      row = s_f[t / NP] + a;
+      diag = false;
+    } else if (idx < off_cp) {
+      const int t = (int)((idx - off_cv) % (NP * NVE));
+      row = s_pc + t / NVE;
+      diag = false;
+    } else {
+      const int r = (int)(idx - off_cp);
+      row = s_pc + r / NP;
+      diag = r / NP == r % NP;
+    }
+    double v = K[idx];
+    if (dbc[row]) {
+      if (!diag) continue;
+      v = 1.0;
+    }
+    val[map[idx]] += (JT)v;
+  }
+}
+
+template <typename JT>
+static void galerkin_level(gm_ctx *prof, const gm_ctx *above, MgLevel &L, JT *val) {
+  gm_ctx *lc = L.ctx;
+  const ElemLayout &ly = lc->lay;
+  const int nch = 1 << lc->dim;
+  const double gbytes = ((double)above->num_elems * (ly.total + lc->nve * lc->nve + lc->np * lc->np) +
+                         (double)lc->num_elems * ly.total) * 8.0 + 4.0 * nch * lc->num_elems;
+  if (lc->dim == 2) {
+    if (lc->energy)
+      gm_launch(prof, GM_K_GALERKIN, gbytes, k_galerkin<2, 1>, dim3(lc->num_elems), dim3(256), 0,
+                lc->stream, above->d_kel, L.d_child, L.d_W, L.d_Q, ly.total, ly.off_mp, lc->d_kel);
+    else
+      gm_launch(prof, GM_K_GALERKIN, gbytes, k_galerkin<2, 0>, dim3(lc->num_elems), dim3(256), 0,
+                lc->stream, above->d_kel, L.d_child, L.d_W, L.d_Q, ly.total, ly.off_mp, lc->d_kel);
+  } else {
+    if (lc->energy)
+      gm_launch(prof, GM_K_GALERKIN, gbytes, k_galerkin<3, 1>, dim3(lc->num_elems), dim3(256), 0,
+                lc->stream, above->d_kel, L.d_child, L.d_W, L.d_Q, ly.total, ly.off_mp, lc->d_kel);
+    else
+      gm_launch(prof, GM_K_GALERKIN, gbytes, k_galerkin<3, 0>, dim3(lc->num_elems), dim3(256), 0,
+                lc->stream, above->d_kel, L.d_child, L.d_W, L.d_Q, ly.total, ly.off_mp, lc->d_kel);
+  }
+  for (int col = 0; col < lc->n_ecol; col++) {
+    const int cnt = lc->ecol_ptr[col + 1] - lc->ecol_ptr[col];
+    if (!cnt) continue;
+    const double sbytes = (double)cnt * ly.total * (8.0 + 4.0 + 2.0 * sizeof(JT));
+    auto kern = lc->dim == 2 ? k_elem_scatter<2, JT> : k_elem_scatter<3, JT>;
+    gm_launch(prof, GM_K_ELEM_SCATTER, sbytes, kern, dim3(cnt), dim3(256), 0, lc->stream,
+              (const double *)lc->d_kel, (const int *)(lc->d_ecol_elems + lc->ecol_ptr[col]), lc->nv,
+              (const int *)lc->d_conn, (const int *)lc->d_fu, (const int *)lc->d_centroid,
+              (const signed char *)lc->d_dbc, (const int *)lc->d_e2nz, ly.total, ly.off_mp,
+              ly.off_cv, ly.off_cp, val);
+  }
+}
+
+int gm_mg_kel_alloc(gm_ctx *c) {
+  if (!c->d_kel) GM_HIP(hipMalloc(&c->d_kel, (size_t)c->num_elems * c->lay.total * sizeof(double)));
+  return GM_OK;
+}
+
+extern "C" int gm_mg_copy_level_matrix(const gm_ctx *c, int level, double *val) {
+  if (!c || !val || level < 1 || level > (int)c->mg.size()) return GM_ERR_ARG;
+  const MgLevel &L = c->mg[level - 1];
+  if (!L.d_val) {
+    gm_set_error("multigrid level %d has no matrix yet", level);
+    return GM_ERR_ARG;
+  }
+  const size_t n = L.ctx->nval;
+  if (L.d_val == L.d_val_p[1]) {
+    std::vector<float> h(n);
+    GM_HIP(hipMemcpy(h.data(), L.d_val, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) val[i] = h[i];
+  } else {
+    GM_HIP(hipMemcpy(val, L.d_val, n * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return GM_OK;
+}
+
 extern "C" int gm_mg_add_level(gm_ctx *fine, gm_ctx *coarse, const int *P_rowptr, const int *P_col,
                                const double *P_val, const int *inject) {
   if (!fine || !coarse || !P_rowptr || !P_col || !P_val || !inject) return GM_ERR_ARG;
@@ -571,6 +852,8 @@ extern "C" int gm_mg_add_level(gm_ctx *fine, gm_ctx *coarse, const int *P_rowptr
       (rc = upv(&L.d_inject, inject, coarse->num_nodes)))
     return rc;
   L.nnz = nnz;
+  L.gal_err = build_galerkin_maps(above, coarse, P_rowptr, P_col, P_val, L);
+  L.gal_ok = L.gal_err.empty();
   GM_HIP(hipMalloc(&L.d_state, Nc * sizeof(double)));
   GM_HIP(hipMalloc(&L.d_rdummy, Nc * sizeof(double)));
   fine->mg.push_back(L);
@@ -718,7 +1001,7 @@ static int coarse_inverse(gm_ctx *c, MgLevel &L, int prec) {
 // exactly (coarse_direct).  val / val32 is the level-0 (scaled) matrix, x the
 // state.
 int gm_mg_setup(gm_ctx *c, const double *x, const double *val, const float *val32, int prec,
-                const double *scale, int coarse_direct) {
+                const double *scale, int coarse_direct, int galerkin) {
   c->mg_scale = scale;
   if (c->mg.empty()) {
     gm_set_error("multigrid preconditioner requested but no coarse level attached");
@@ -731,7 +1014,30 @@ int gm_mg_setup(gm_ctx *c, const double *x, const double *val, const float *val3
     if ((rc = build_colour_rows(L.ctx))) return rc;
   const double *xs = x;
   gm_ctx *above = c;
+  if (galerkin && !c->d_kel) {
+    gm_set_error("Galerkin coarse operators need the element matrices of the last assembly");
+    return GM_ERR_ARG;
+  }
   for (MgLevel &L : c->mg) {
+    if (galerkin) {
+      gm_ctx *lc = L.ctx;
+      if (!L.gal_ok) {
+        gm_set_error("Galerkin coarse operators unavailable: %s", L.gal_err.c_str());
+        return GM_ERR_ARG;
+      }
+      if ((rc = alloc_level_vectors(L, lc, prec, true)) || (rc = gm_mg_kel_alloc(lc))) return rc;
+      if (prec == GM_PREC_MIXED) {
+        k_set<float><<<2048, 256, 0, lc->stream>>>((float *)L.d_val, lc->nval, 0.0f);
+        galerkin_level<float>(c, above, L, (float *)L.d_val);
+      } else {
+        k_set<double><<<2048, 256, 0, lc->stream>>>((double *)L.d_val, lc->nval, 0.0);
+        galerkin_level<double>(c, above, L, (double *)L.d_val);
+      }
+      GM_HIP(hipGetLastError());
+      GM_HIP(hipStreamSynchronize(lc->stream));
+      above = lc;
+      continue;
+    }
     gm_ctx *lc = L.ctx;
     if ((rc = alloc_level_vectors(L, lc, prec, true))) return rc;
     k_inject<<<1024, 256, 0, c->stream>>>(lc->num_nodes, L.d_inject, above->d_fu, lc->d_fu, lc->nv,
@@ -918,7 +1224,7 @@ void gm_mg_free(gm_ctx *c) {
     void *p[] = {L.d_Prow, L.d_Pcol, L.d_Pval, L.d_Rrow, L.d_Rcol, L.d_Rval, L.d_inject,
                  L.d_state, L.d_rdummy, L.d_x, L.d_b, L.d_r, L.d_val_p[0], L.d_val_p[1],
                  L.d_ainv_p[0], L.d_ainv_p[1],
-                 L.d_dense, L.d_pcol, L.d_prow, L.d_perm};
+                 L.d_dense, L.d_pcol, L.d_prow, L.d_perm, L.d_child, L.d_W, L.d_Q};
     for (void *q : p)
       if (q) (void)hipFree(q);
   };
@@ -930,7 +1236,7 @@ void gm_mg_free(gm_ctx *c) {
 
 extern "C" int gm_mg_setup_dev(gm_ctx *c, const double *x, const double *val, const double *scale) {
   if (!c || !x || !val) return GM_ERR_ARG;
-  int rc = gm_mg_setup(c, x, val, nullptr, GM_PREC_FP64, scale, 1);
+  int rc = gm_mg_setup(c, x, val, nullptr, GM_PREC_FP64, scale, 1, 0);
   if (rc) return rc;
   GM_HIP(hipStreamSynchronize(c->stream));
   return GM_OK;

@@ -107,6 +107,10 @@ typedef struct gm_solver_opts {
                           Gauss-Seidel, a residual on each colour's rows); 1 additive (one
                           residual per sweep, corrections weighted by 1 / elements sharing
                           the unknown) */
+  int mg_galerkin;     /* multigrid coarse operators: 0 re-assembled on each coarse mesh at the
+                          injected state; 1 Galerkin R A P of the finer level's element
+                          matrices (P element-local: each finer element inside one coarser
+                          one), with each coarse level's own Dirichlet rows */
 } gm_solver_opts;
 
 #define GM_MAX_REPORT 64
@@ -208,6 +212,9 @@ int gm_mg_num_levels(const gm_ctx *ctx);
 /* Coarse Jacobians at the injected state x_dev and element-block inverses for
  * the level-0 matrix val_dev (row-sum scaled by scale_dev, or NULL). */
 int gm_mg_setup_dev(gm_ctx *ctx, const double *x_dev, const double *val_dev, const double *scale_dev);
+/* Host copy (as FP64) of the Jacobian of multigrid level `level` (1 = first
+ * coarse level) from the last set-up, in that level's MSR layout. */
+int gm_mg_copy_level_matrix(const gm_ctx *ctx, int level, double *val);
 
 /* Kernel timing: when enabled, launches of the kernels below carry HIP
  * events that their own dispatch stamps (hipExtLaunchKernel), so a duration
@@ -228,7 +235,9 @@ int gm_mg_setup_dev(gm_ctx *ctx, const double *x_dev, const double *val_dev, con
 #define GM_K_FILL_RESID 11 /* k_fill, residual only (line-search trial), one colour */
 #define GM_K_MG_RESID_ROWS 12 /* k_mg_resid_rows: smoother residual on one element colour's rows */
 #define GM_K_MG_DENSE 13  /* k_dense_gemv: exact solve on the coarsest level (dense inverse) */
-#define GM_K_COUNT 14
+#define GM_K_GALERKIN 14  /* k_galerkin: coarse element matrices P^T K P of one level */
+#define GM_K_ELEM_SCATTER 15 /* k_elem_scatter: element matrices of one colour into MSR */
+#define GM_K_COUNT 16
 /* on = 0: off; 1: time every launch; k > 1: time a pseudo-random one launch
  * in k of each kernel (lower overhead), durations scaled to all launches by
  * the launch count. */

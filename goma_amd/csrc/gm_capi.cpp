@@ -102,6 +102,7 @@ static void free_ctx(gm_ctx *c) {
   if (!c) return;
   gm_free_linalg(c);
   gm_mg_free(c);
+  gm_band_free(c);
   void *ptrs[] = {c->d_coor, c->d_conn,   c->d_fu,    c->d_centroid, c->d_dbc,   c->d_dbc_val,
                   c->d_bindx, c->d_ecol_elems, c->d_e2nz, c->d_flag, c->d_tables, c->d_val,
                   c->d_resid, c->d_dx,    c->d_scale, c->d_val32,    c->d_red,   c->d_xwork,
@@ -470,6 +471,8 @@ static int solve_impl(gm_ctx *c, double *x, const gm_solver_opts *o, gm_newton_r
         const int bins = o->patch_bins > 0 ? o->patch_bins : gm_default_bins(c);
         if ((rc = gm_ilu_plan(c, bins))) return rc;
         if ((rc = gm_ilu_factor(c, c->d_val, c->d_val32, prec))) return rc;
+      } else if (rebuild && o->precond == GM_PC_BAND) {
+        if ((rc = gm_band_factor(c, c->d_val, c->d_val32, prec))) return rc;
       } else if (rebuild && o->precond == GM_PC_MG) {
         if ((rc = gm_mg_setup(c, x, c->d_val, c->d_val32, prec, c->d_scale,
                               o->mg_coarse_sweeps <= 0, galerkin)))

@@ -71,6 +71,15 @@ struct KrylovWork {
   float *d_w32 = nullptr, *d_z32 = nullptr;
 };
 
+// banded LU (gm_band.hip)
+struct BandLU {
+  int kl = -1, ku = -1;       // lower / upper bandwidth of the MSR graph
+  long long ld = 0;           // entries per stored column
+  int prec = -1;              // precision of the last factorisation (0 FP64, 1 FP32)
+  void *d_ab[2] = {nullptr, nullptr};  // [N][ld] per precision
+  double *d_w1 = nullptr, *d_w2 = nullptr;  // solve work vectors
+};
+
 struct gm_ctx;
 struct MgLevel {
   gm_ctx *ctx = nullptr;  // the level's context (not owned)
@@ -145,6 +154,7 @@ struct gm_ctx {
 
   IluPlan ilu;
   KrylovWork kry;
+  BandLU band;
   std::vector<MgLevel> mg;  // coarse levels, finest first
   MgLevel mg0;              // level-0 vectors and element inverses
   const double *mg_scale = nullptr;  // row-sum scale of the level-0 matrix
@@ -204,6 +214,11 @@ int gm_mg_kel_alloc(gm_ctx *c);  // element-matrix buffer of the finest level
 int gm_mg_apply_d(gm_ctx *c, const double *a0, const double *v, double *z, const gm_solver_opts *o);
 int gm_mg_apply_f(gm_ctx *c, const float *a0, const double *v, double *z, const gm_solver_opts *o);
 void gm_mg_free(gm_ctx *c);
+
+// banded LU (gm_band.hip)
+int gm_band_factor(gm_ctx *c, const double *val, const float *val32, int prec);
+int gm_band_apply(gm_ctx *c, const double *in, double *out);
+void gm_band_free(gm_ctx *c);
 
 // timing helper (used through gm_launch in gm_launch.h): adds `bytes` to the
 // family's total; returns true, with the event pair that the launch must

@@ -883,6 +883,8 @@ static int gmres_t(gm_ctx *c, const VT *a, const VT *lu, VT *ywork, const double
     if (o->precond == GM_PC_MG) {
       if constexpr (sizeof(VT) == 8) gm_mg_apply_d(c, (const double *)a, (const double *)in, (double *)out, o);
       else gm_mg_apply_f(c, (const float *)a, (const double *)in, (double *)out, o);
+    } else if (o->precond == GM_PC_BAND) {
+      gm_band_apply(c, (const double *)in, (double *)out);
     } else if (pc) {
       ilu_apply<VT, BT, BT>(c, lu, ywork, in, out);
     } else {
@@ -993,6 +995,8 @@ int gm_gmres_run(gm_ctx *c, const double *val, const float *val32, const double 
     int bins = o->patch_bins > 0 ? o->patch_bins : gm_default_bins(c);
     if ((rc = gm_ilu_plan(c, bins))) return rc;
     if ((rc = gm_ilu_factor(c, val, val32, prec))) return rc;
+  } else if (o->precond == GM_PC_BAND) {
+    if ((rc = gm_band_factor(c, val, val32, prec))) return rc;
   } else if (o->precond == GM_PC_MG &&
              (!c->mg0.d_ainv || c->mg0.d_ainv != c->mg0.d_ainv_p[prec == GM_PREC_MIXED ? 1 : 0])) {
     gm_set_error("multigrid preconditioner not set up (gm_mg_setup_dev)");

@@ -51,6 +51,10 @@ extern "C" {
 #define GM_PC_NONE 0
 #define GM_PC_ILU0 1     /* ILU(0) in patch order (see gm_solver_opts.patch_bins) */
 #define GM_PC_MG 2       /* geometric multigrid V-cycle, element-block (Vanka) smoothing */
+#define GM_PC_BAND 3     /* banded LU of the scaled matrix in Goma's unknown order, without
+                            pivoting: a direct solve (the role of Goma's direct solvers,
+                            src/mm_sol_nonlinear.c:1646-1690); memory N (kl + ku + 129) x 8
+                            bytes (x 4 in mixed precision), kl / ku the graph's bandwidths */
 
 typedef struct gm_mesh {
   int dim;             /* 2: QUAD9, 3: HEX27 */
@@ -88,7 +92,7 @@ typedef struct gm_solver_opts {
   int kspace;          /* GMRES restart length */
   int max_its;         /* GMRES iteration limit per Newton step */
   double lin_tol;      /* ||r||/||r0|| tolerance */
-  int precond;         /* GM_PC_NONE | GM_PC_ILU0 */
+  int precond;         /* GM_PC_NONE | GM_PC_ILU0 | GM_PC_MG | GM_PC_BAND */
   int patch_bins;      /* ILU(0) patches per axis (0: library default) */
   int precision;       /* GM_PREC_FP64 | GM_PREC_MIXED */
   int verbose;         /* 1: one line per Newton step on stderr */
@@ -237,7 +241,10 @@ int gm_mg_copy_level_matrix(const gm_ctx *ctx, int level, double *val);
 #define GM_K_MG_DENSE 13  /* k_dense_gemv: exact solve on the coarsest level (dense inverse) */
 #define GM_K_GALERKIN 14  /* k_galerkin: coarse element matrices P^T K P of one level */
 #define GM_K_ELEM_SCATTER 15 /* k_elem_scatter: element matrices of one colour into MSR */
-#define GM_K_COUNT 16
+#define GM_K_BAND_PANEL 16 /* k_band_a11 / k_band_panel: one 64-column panel of the banded LU */
+#define GM_K_BAND_UPDATE 17 /* k_band_update: trailing update of one panel */
+#define GM_K_BAND_SOLVE 18 /* k_band_fwd / k_band_bwd: one 64-row block of the triangular solves */
+#define GM_K_COUNT 19
 /* on = 0: off; 1: time every launch; k > 1: time a pseudo-random one launch
  * in k of each kernel (lower overhead), durations scaled to all launches by
  * the launch count. */

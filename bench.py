@@ -189,32 +189,40 @@ def sweep_bench(args, torch, capi):
     """Carreau-Yasuda Ra continuation and the learned initial guess.
 
     Sweep: zeroth-order continuation over 64 log-spaced Ra in [1e3, 1e6] on
-    the 128^2 jittered mesh (previous-step guess, FP64 Newton to 1e-10).
+    the 128^2 jittered mesh (previous-step guess, FP64 Newton to 1e-10, each
+    Newton system solved by the banded LU, or by multigrid-GMRES).
     Network: trained on the GPU on the converged sweep solutions.
-    Evaluation: at the 32 held-out geometric midpoints of sweep intervals
-    (0,1), (2,3), ..., Newton from the previous-step guess (the solution at
-    the interval's lower end) against Newton from the learned guess (delta
-    form from the same solution, with fallback), same tolerance.
+    Evaluation: at up to 32 held-out geometric midpoints of sweep intervals
+    spread over the converged range, Newton from the previous-step guess (the
+    solution at the interval's lower end) against Newton from the learned
+    guess (delta form from the same solution, with fallback), same tolerance.
     """
     from goma_amd import continuation, learned, problems
 
     n = args.n or 128
-    gal = 1 if args.mg_galerkin is None else args.mg_galerkin
-    if not gal:
+    gal = args.mg_galerkin
+    band = args.sweep_precond == "band"
+    if not band and not gal:
         # the shear-thinning flow defeats deep re-discretised hierarchies: stop at 16^2 (solved exactly)
         args.mg_coarse = max(args.mg_coarse, 16)
     prob = problems.natural_convection(n=n, Ra=1e3, carreau=True, jitter=0.15)
     ctx = capi.Context(prob)
-    nlev = ctx.attach_multigrid(args.mg_coarse)
+    nlev = 0 if band else ctx.attach_multigrid(args.mg_coarse)
     prec = 1 if args.precision == "mixed" else 0
-    # shear-thinning flow weakens the V(1,1) cycle as Ra grows: V(2,2), larger Krylov space
-    opts = dict(kspace=max(args.kspace, 100), max_its=max(args.max_its, 400), precision=prec,
-                max_newton=25, precond=2, mg_nu=max(args.mg_nu, 2), mg_omega=args.mg_omega,
-                line_search=args.line_search, pc_reuse=args.pc_reuse, mg_smoother=0,
-                mg_galerkin=gal)
+    if band:
+        # element Reynolds numbers reach the hundreds: each Newton system is solved directly
+        opts = dict(kspace=20, max_its=40, precision=prec, max_newton=15, precond=3,
+                    line_search=args.line_search, pc_reuse=args.pc_reuse)
+    else:
+        # shear-thinning flow weakens the V(1,1) cycle as Ra grows: V(2,2), larger Krylov space
+        opts = dict(kspace=max(args.kspace, 100), max_its=max(args.max_its, 400), precision=prec,
+                    max_newton=25, precond=2, mg_nu=max(args.mg_nu, 2), mg_omega=args.mg_omega,
+                    line_search=args.line_search, pc_reuse=args.pc_reuse, mg_smoother=0,
+                    mg_galerkin=gal)
     log = lambda s: print(s, file=sys.stderr, flush=True)
     ras = problems.sweep_ra(args.sweep_count, 1e3, 1e6)
-    log(f"[sweep] {prob.name}: N={ctx.N} levels {nlev}, {len(ras)} Ra values")
+    log(f"[sweep] {prob.name}: N={ctx.N} " + ("banded LU" if band else f"multigrid, {nlev} levels")
+        + f", {len(ras)} Ra values")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     steps, sols = continuation.sweep(ctx, prob, ras, opts, log=log if args.verbose else None)
@@ -248,7 +256,9 @@ def sweep_bench(args, torch, capi):
     nb = lg.set_basis(sols_ok)
     log(f"[sweep] snapshot subspace of dimension {nb} for the delta-form guess")
     rows = []
-    for k in range(0, ok - 1, 2):
+    # held-out values: geometric midpoints of sweep intervals, spread over the converged range
+    picks = np.unique(np.linspace(0, ok - 2, min(args.held_out, ok - 1)).round().astype(int)) if ok > 1 else []
+    for k in picks:
         Ra = float(np.sqrt(ras_ok[k] * ras_ok[k + 1]))
         ctx.set_physics(problems.physics_at(prob, Ra))
         xprev = torch.tensor(sols_ok[k], device="cuda")
@@ -281,7 +291,8 @@ def sweep_bench(args, torch, capi):
         "data": "synthetic",
         "config": {"workload": f"{prob.name} Carreau-Yasuda natural convection, jitter 0.15h, "
                                f"Ra sweep [1e3, 1e6] x {len(ras)}", "unknowns": ctx.N,
-                   "held_out": len(rows), "network": "3-256-256-256-256-4 tanh, BF16 MFMA"},
+                   "held_out": len(rows), "network": "3-256-256-256-256-4 tanh, BF16 MFMA",
+                   "linear_solver": "banded LU (direct)" if band else "GMRES + multigrid"},
         "sweep": {"converged_steps": ok, "seconds": round(t_sweep, 2), "cell_reynolds": diag[::4] + diag[-1:],
                   "newton_its": sum(s.newton_its for s in steps),
                   "retries": sum(s.retries for s in steps)},
@@ -326,7 +337,7 @@ def learned_bench(args, torch, capi):
     nlev = ctx.attach_multigrid(args.mg_coarse)
     base = dict(kspace=args.kspace, max_its=args.max_its, max_newton=40, precond=2,
                 mg_nu=args.mg_nu, mg_omega=args.mg_omega, line_search=args.line_search,
-                pc_reuse=args.pc_reuse, mg_smoother=args.mg_smoother)
+                pc_reuse=args.pc_reuse, mg_smoother=args.mg_smoother, mg_galerkin=args.mg_galerkin)
     fp64, mixed = dict(base, precision=0), dict(base, precision=1)
     groups = field_groups(ctx, prob)
     log(f"[learned] {prob.name}: N={ctx.N} levels {nlev}, target Ra={target:g}")
@@ -452,7 +463,7 @@ def main(argv=None):
     ap.add_argument("--kspace", type=int, default=50)
     ap.add_argument("--max-its", type=int, default=200)
     ap.add_argument("--patch-bins", type=int, default=0)
-    ap.add_argument("--precond", choices=["mg", "ilu"], default="mg")
+    ap.add_argument("--precond", choices=["mg", "ilu", "band"], default="mg")
     ap.add_argument("--mg-nu", type=int, default=None,
                     help="smoothing sweeps per level (default: 2 in 2D, 1 in 3D)")
     ap.add_argument("--mg-omega", type=float, default=0.7)
@@ -460,8 +471,11 @@ def main(argv=None):
     ap.add_argument("--mg-coarse", type=int, default=2, help="coarsest elements per side")
     ap.add_argument("--mg-smoother", type=int, default=1,
                     help="element-block smoother: 0 multiplicative, 1 additive (one residual per sweep)")
-    ap.add_argument("--mg-galerkin", type=int, default=None,
-                    help="1: Galerkin R A P coarse operators (default: sweep 1, others 0)")
+    ap.add_argument("--mg-galerkin", type=int, default=1,
+                    help="1: Galerkin R A P coarse operators; 0: re-assembled on each coarse mesh")
+    ap.add_argument("--sweep-precond", choices=["band", "mg"], default="band",
+                    help="sweep workload: banded LU direct solve or multigrid")
+    ap.add_argument("--held-out", type=int, default=32, help="sweep workload: held-out Ra values")
     ap.add_argument("--pc-reuse", type=int, default=0,
                     help="preconditioner reuse across Newton passes: 0 recalc, 1 reuse, 2 adaptive")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -505,7 +519,7 @@ def main(argv=None):
     ctx = capi.Context(prob)
     t_setup = time.perf_counter() - t0
     bins = args.patch_bins or 0
-    pc = 2 if args.precond == "mg" else 1
+    pc = {"ilu": 1, "mg": 2, "band": 3}[args.precond]
     nlev = ctx.attach_multigrid(args.mg_coarse) if pc == 2 else 0
     t_setup = time.perf_counter() - t0
     print(f"[bench] {prob.name}: N={ctx.N} nnz={ctx.nval - 1} levels {nlev} setup {t_setup:.2f}s",
@@ -514,7 +528,7 @@ def main(argv=None):
     opts = dict(kspace=args.kspace, max_its=args.max_its, patch_bins=bins, precision=prec,
                 max_newton=40, verbose=1 if args.verbose else 0, precond=pc, mg_nu=args.mg_nu,
                 mg_omega=args.mg_omega, line_search=args.line_search, pc_reuse=args.pc_reuse, mg_smoother=args.mg_smoother,
-                mg_galerkin=args.mg_galerkin or 0)
+                mg_galerkin=args.mg_galerkin)
     for w in range(args.warmup):
         x.zero_()
         torch.cuda.synchronize()
@@ -597,7 +611,7 @@ def main(argv=None):
         "data": "synthetic",
         "config": {"workload": f"{prob.name} ({args.workload}) " + ", ".join(f"{k}={v}" for k, v in params.items()),
                    "unknowns": ctx.N, "nnz": ctx.nval - 1, "elements": prob.mesh.num_elems,
-                   "precision": args.precision, "solver": f"GMRES({args.kspace}) + " + ("multigrid V(%d,%d) %s element-block smoothing, %d levels, exact coarsest solve" % (args.mg_nu, args.mg_nu, ("multiplicative", "additive")[args.mg_smoother], nlev) if pc == 2 else "ILU(0) patch order") + ", preconditioner reuse " + ("recalc", "reuse", "adaptive")[args.pc_reuse],
+                   "precision": args.precision, "solver": f"GMRES({args.kspace}) + " + ("multigrid V(%d,%d) %s element-block smoothing, %s coarse operators, %d levels, exact coarsest solve" % (args.mg_nu, args.mg_nu, ("multiplicative", "additive")[args.mg_smoother], ("re-assembled", "Galerkin")[args.mg_galerkin], nlev) if pc == 2 else "ILU(0) patch order" if pc == 1 else "banded LU") + ", preconditioner reuse " + ("recalc", "reuse", "adaptive")[args.pc_reuse],
                    "initial_guess": "zero"},
         "newton_its": newton,
         "gmres_its": gm_its,

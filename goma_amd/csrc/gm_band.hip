@@ -1,8 +1,9 @@
 // Banded LU factorisation and solve of the row-sum-scaled Jacobian, without
 // pivoting, in Goma's unknown order: a direct solver on the GPU for the
-// Newton systems where iterative preconditioners stall (the role of Goma's
-// direct solvers, UMFPACK / Sparse lu / SuperLU, src/mm_sol_nonlinear.c:
-// 1646-1690, behind the same GMRES interface as the other preconditioners).
+// Newton systems where iterative preconditioners stall -- the role of Goma's
+// direct solvers (Solution Algorithm = lu, Goma's default, src/mm_input.c:5904;
+// umf / lu dispatch in src/mm_sol_nonlinear.c:1624-1681) -- behind the same
+// GMRES interface as the other preconditioners.
 //
 // Storage.  With kl / ku the lower / upper bandwidths of the MSR graph, the
 // band lives column-major in ld = kl + ku + 1 + 2 NB entries per column,
@@ -213,19 +214,26 @@ __global__ __launch_bounds__(256) void k_band_update(int j0, int kb, int R, int 
 // Forward substitution of one block: y[blk] = L11^-1 w[blk] (w holds b minus
 // the earlier blocks' contributions), then w[j0+kb .. j0+kb+R) -= L21 y[blk].
 // w[blk] is only read in this launch, so every workgroup solves the block
-// itself; workgroup 0 writes it to y.
+// itself (from L11 staged in LDS); workgroup 0 writes it to y.
 template <typename T>
 __global__ __launch_bounds__(256) void k_band_fwd(int j0, int kb, int R, const T *__restrict__ ab,
                                                   long long off, long long lda, double *__restrict__ w,
                                                   double *__restrict__ y) {
+  __shared__ T sT[NB][NB + 1];
   __shared__ double sy[NB];
   const T *__restrict__ base = ab + off;
   const int tid = threadIdx.x, lane = tid & 63;
+  for (int t = tid; t < NB * NB; t += 256) {
+    const int j = t >> 6, i = t & 63;
+    sT[i][j] = (i < kb && j < kb && i > j) ? base[(j0 + i) + (size_t)(j0 + j) * lda] : T(0);
+  }
+  if (tid < NB) sy[tid] = tid < kb ? w[j0 + tid] : 0.0;
+  __syncthreads();
   if (tid < 64) {
-    double v = lane < kb ? w[j0 + lane] : 0.0;
+    double v = sy[lane];
     for (int k = 0; k < kb; k++) {
       const double vk = __shfl(v, k);
-      if (lane > k && lane < kb) v -= (double)base[(j0 + lane) + (size_t)(j0 + k) * lda] * vk;
+      v -= (double)sT[lane][k] * vk;  // zero for lane <= k
     }
     sy[lane] = v;
     if (blockIdx.x == 0 && lane < kb) y[j0 + lane] = v;
@@ -234,9 +242,16 @@ __global__ __launch_bounds__(256) void k_band_fwd(int j0, int kb, int R, const T
   const int rr = blockIdx.x * 256 + tid;
   if (rr >= R) return;
   const size_t i = (size_t)j0 + kb + rr;
-  double s = 0.0;
-  for (int k = 0; k < kb; k++) s += (double)base[i + (size_t)(j0 + k) * lda] * sy[k];
-  w[i] -= s;
+  const T *__restrict__ col = base + i + (size_t)j0 * lda;
+  double s0 = 0.0, s1 = 0.0;
+  int k = 0;
+#pragma unroll 8
+  for (; k + 1 < kb; k += 2) {
+    s0 += (double)col[(size_t)k * lda] * sy[k];
+    s1 += (double)col[(size_t)(k + 1) * lda] * sy[k + 1];
+  }
+  if (k < kb) s0 += (double)col[(size_t)k * lda] * sy[k];
+  w[i] -= s0 + s1;
 }
 
 // Backward substitution of one block: x[blk] = U11^-1 w[blk], then
@@ -245,16 +260,23 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_band_bwd(int j0, int kb, int R, const T *__restrict__ ab,
                                                   long long off, long long lda, double *__restrict__ w,
                                                   double *__restrict__ x) {
+  __shared__ T sT[NB][NB + 1];
   __shared__ double sx[NB];
   const T *__restrict__ base = ab + off;
   const int tid = threadIdx.x, lane = tid & 63;
+  for (int t = tid; t < NB * NB; t += 256) {
+    const int j = t >> 6, i = t & 63;
+    sT[i][j] = (i < kb && j < kb && i <= j) ? base[(j0 + i) + (size_t)(j0 + j) * lda] : T(0);
+  }
+  if (tid < NB) sx[tid] = tid < kb ? w[j0 + tid] : 0.0;
+  __syncthreads();
   if (tid < 64) {
-    double v = lane < kb ? w[j0 + lane] : 0.0;
+    double v = sx[lane];
     for (int k = kb - 1; k >= 0; k--) {
-      const double d = (double)base[(j0 + k) + (size_t)(j0 + k) * lda];
+      const double d = (double)sT[k][k];
       if (lane == k) v /= (d == 0.0 ? 1.0 : d);
       const double vk = __shfl(v, k);
-      if (lane < k) v -= (double)base[(j0 + lane) + (size_t)(j0 + k) * lda] * vk;
+      if (lane < k) v -= (double)sT[lane][k] * vk;
     }
     sx[lane] = v;
     if (blockIdx.x == 0 && lane < kb) x[j0 + lane] = v;
@@ -263,9 +285,16 @@ __global__ __launch_bounds__(256) void k_band_bwd(int j0, int kb, int R, const T
   const int rr = blockIdx.x * 256 + tid;
   if (rr >= R) return;
   const size_t i = (size_t)j0 - R + rr;
-  double s = 0.0;
-  for (int k = 0; k < kb; k++) s += (double)base[i + (size_t)(j0 + k) * lda] * sx[k];
-  w[i] -= s;
+  const T *__restrict__ col = base + i + (size_t)j0 * lda;
+  double s0 = 0.0, s1 = 0.0;
+  int k = 0;
+#pragma unroll 8
+  for (; k + 1 < kb; k += 2) {
+    s0 += (double)col[(size_t)k * lda] * sx[k];
+    s1 += (double)col[(size_t)(k + 1) * lda] * sx[k + 1];
+  }
+  if (k < kb) s0 += (double)col[(size_t)k * lda] * sx[k];
+  w[i] -= s0 + s1;
 }
 
 __global__ __launch_bounds__(256) void k_band_copy(int n, const double *__restrict__ a, double *__restrict__ b) {

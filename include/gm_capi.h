@@ -53,8 +53,9 @@ extern "C" {
 #define GM_PC_MG 2       /* geometric multigrid V-cycle, element-block (Vanka) smoothing */
 #define GM_PC_BAND 3     /* banded LU of the scaled matrix in Goma's unknown order, without
                             pivoting: a direct solve (the role of Goma's direct solvers,
-                            src/mm_sol_nonlinear.c:1646-1690); memory N (kl + ku + 129) x 8
-                            bytes (x 4 in mixed precision), kl / ku the graph's bandwidths */
+                            Solution Algorithm = lu / umf, src/mm_sol_nonlinear.c:1624-1681);
+                            memory N (kl + ku + 129) x 8 bytes (x 4 in mixed precision),
+                            kl / ku the graph's bandwidths */
 
 typedef struct gm_mesh {
   int dim;             /* 2: QUAD9, 3: HEX27 */

@@ -225,7 +225,8 @@ def sweep_bench(args, torch, capi):
         + f", {len(ras)} Ra values")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    steps, sols = continuation.sweep(ctx, prob, ras, opts, log=log if args.verbose else None)
+    steps, sols = continuation.sweep(ctx, prob, ras, opts, log=log if args.verbose else None,
+                                     order=args.sweep_order)
     torch.cuda.synchronize()
     t_sweep = time.perf_counter() - t0
     ok = 0
@@ -253,8 +254,11 @@ def sweep_bench(args, torch, capi):
     train_flops = 3.0 * fpp * lg.num_points * args.epochs  # forward + backward (2x forward)
     log(f"[sweep] trained on {lg.num_points} points x {args.epochs} epochs in {t_train:.1f}s "
         f"({train_flops / t_train / 1e12:.1f} TFLOP/s), loss {hist[0]:.3e} -> {hist[-1]:.3e}")
-    nb = lg.set_basis(sols_ok)
-    log(f"[sweep] snapshot subspace of dimension {nb} for the delta-form guess")
+    # the delta-form guess is projected onto the span of nearby sweep solutions: a wide sweep's
+    # full snapshot span filters little of the network's mesh-scale noise
+    half = args.basis_half
+    log(f"[sweep] snapshot subspace from the {2 * half} sweep solutions nearest each held-out value")
+    nb = 0
     rows = []
     # held-out values: geometric midpoints of sweep intervals, spread over the converged range
     picks = np.unique(np.linspace(0, ok - 2, min(args.held_out, ok - 1)).round().astype(int)) if ok > 1 else []
@@ -268,6 +272,7 @@ def sweep_bench(args, torch, capi):
         _, rb = ctx.solve_dev(x.data_ptr(), **opts)
         torch.cuda.synchronize()
         tb = time.perf_counter() - t0
+        nb = lg.set_basis(sols_ok[max(0, k - half + 1): min(ok, k + half + 1)])
         x = torch.empty_like(xprev)
         t0 = time.perf_counter()
         lg.guess(x.data_ptr(), Ra, float(ras_ok[k]), xprev.data_ptr())
@@ -292,11 +297,12 @@ def sweep_bench(args, torch, capi):
         "config": {"workload": f"{prob.name} Carreau-Yasuda natural convection, jitter 0.15h, "
                                f"Ra sweep [1e3, 1e6] x {len(ras)}", "unknowns": ctx.N,
                    "held_out": len(rows), "network": "3-256-256-256-256-4 tanh, BF16 MFMA",
-                   "linear_solver": "banded LU (direct)" if band else "GMRES + multigrid"},
+                   "linear_solver": "banded LU (direct)" if band else "GMRES + multigrid",
+                   "predictor": ("zeroth", "first")[args.sweep_order] + "-order continuation"},
         "sweep": {"converged_steps": ok, "seconds": round(t_sweep, 2), "cell_reynolds": diag[::4] + diag[-1:],
                   "newton_its": sum(s.newton_its for s in steps),
                   "retries": sum(s.retries for s in steps)},
-        "subspace_dim": nb,
+        "subspace_dim": nb, "subspace_snapshots": 2 * half,
         "training": {"points": lg.num_points, "epochs": args.epochs, "seconds": round(t_train, 2),
                      "algorithmic_TFLOPs": round(train_flops / t_train / 1e12, 2),
                      "bf16_peak_TFLOPs": 2500.0, "loss_first": hist[0], "loss_last": hist[-1]},
@@ -476,6 +482,11 @@ def main(argv=None):
     ap.add_argument("--sweep-precond", choices=["band", "mg"], default="band",
                     help="sweep workload: banded LU direct solve or multigrid")
     ap.add_argument("--held-out", type=int, default=32, help="sweep workload: held-out Ra values")
+    ap.add_argument("--sweep-order", type=int, default=1,
+                    help="sweep workload: continuation predictor order (0: previous solution, "
+                         "1: first-order, with the sensitivity dx/dRa)")
+    ap.add_argument("--basis-half", type=int, default=3,
+                    help="sweep workload: sweep solutions on each side of a held-out value in its subspace")
     ap.add_argument("--pc-reuse", type=int, default=0,
                     help="preconditioner reuse across Newton passes: 0 recalc, 1 reuse, 2 adaptive")
     ap.add_argument("--cpu-budget", type=float, default=20.0)

@@ -11,6 +11,11 @@ for one parameter:
 * a step whose Newton fails is retried from the last converged solution with
   half the parameter increment, up to five times (ac_conti.c:968-1045).
 
+First-order continuation (``order=1``, Goma's ALC_FIRST, ac_conti.c:766-773,
+1035-1041) predicts x_{n+1}^0 = x_n + (Ra_{n+1} - Ra_n) dx/dRa, with the
+sensitivity from J dx/dRa = -dR/dRa at the converged x_n, dR/dRa by central
+differences of the residual (soln_sens, src/mm_sol_nonlinear.c:3638-3930).
+
 Steps are uniform in log10 Ra, the natural spacing of the sweep.
 """
 from __future__ import annotations
@@ -72,6 +77,36 @@ class Step:
     retries: int
 
 
+def sensitivity(ctx, prob, x, Ra, opts):
+    """dx/dRa at the converged state x (float64 CUDA tensor): J t = -dR/dRa on
+    the row-sum-scaled system, solved with the solver options ``opts``."""
+    import torch
+
+    z = lambda n: torch.zeros(n, dtype=torch.float64, device="cuda")
+    N, nv = ctx.N, ctx.nval
+    r, rp, rm, sc, t = z(N), z(N), z(N), z(N), z(N)
+    v = z(nv)
+    dRa = 1e-3 * Ra
+    for R, ra in ((rp, Ra + dRa), (rm, Ra - dRa)):
+        ctx.set_physics(problems.physics_at(prob, ra))
+        v.zero_()
+        if ctx.fill_dev(x.data_ptr(), R.data_ptr(), v.data_ptr()) != 0:
+            raise RuntimeError("assembly failed in the sensitivity")
+    ctx.set_physics(problems.physics_at(prob, Ra))
+    v.zero_()
+    if ctx.fill_dev(x.data_ptr(), r.data_ptr(), v.data_ptr()) != 0:
+        raise RuntimeError("assembly failed in the sensitivity")
+    ctx.row_sum_scale_dev(v.data_ptr(), r.data_ptr(), sc.data_ptr())
+    b = (rp - rm) / (2.0 * dRa) / sc
+    lin = {k: opts[k] for k in ("precond", "precision", "kspace", "patch_bins", "mg_nu", "mg_omega",
+                                "mg_smoother", "mg_galerkin") if k in opts}
+    if lin.get("precision", 0) == 1:
+        lin["precision"] = 0  # the public GMRES entry point takes FP64 values
+    ctx.gmres_dev(v.data_ptr(), b.data_ptr(), t.data_ptr(), max_its=opts.get("max_its", 200),
+                  lin_tol=1e-8, **lin)
+    return -t
+
+
 def _solve(ctx, prob, x, Ra, opts):
     ctx.set_physics(problems.physics_at(prob, Ra))
     t0 = time.perf_counter()
@@ -80,13 +115,14 @@ def _solve(ctx, prob, x, Ra, opts):
 
 
 def sweep(ctx, prob, ras, opts: dict, x=None, max_retries: int = 5, keep: bool = True,
-          log=None):
+          log=None, order: int = 0):
     """Continuation through the values ``ras`` (increasing or decreasing).
 
     Returns (steps, solutions): one Step per requested value and, with
     ``keep``, the converged solutions as host float64 arrays.  ``x`` (a
     float64 CUDA tensor of ctx.N entries, zero if omitted) is the predictor
-    of the first step and holds the last solution on return.
+    of the first step and holds the last solution on return.  ``order`` 1
+    uses the first-order predictor from the second step on.
     """
     import torch
 
@@ -95,6 +131,7 @@ def sweep(ctx, prob, ras, opts: dict, x=None, max_retries: int = 5, keep: bool =
     steps, sols = [], []
     last_ok = x.clone()
     lam_prev = None
+    tan = None  # dx/dRa at last_ok
     for Ra in ras:
         target = float(Ra)
         retries = 0
@@ -103,6 +140,8 @@ def sweep(ctx, prob, ras, opts: dict, x=None, max_retries: int = 5, keep: bool =
         lam = target
         while True:
             x.copy_(last_ok)
+            if tan is not None:
+                x.add_(tan, alpha=lam - lam_prev)
             rep, dt = _solve(ctx, prob, x, lam, opts)
             its += rep.newton_its
             lin += rep.lin_its_total
@@ -110,6 +149,10 @@ def sweep(ctx, prob, ras, opts: dict, x=None, max_retries: int = 5, keep: bool =
             if rep.converged:
                 last_ok.copy_(x)
                 lam_prev = lam
+                if order == 1:
+                    t0 = time.perf_counter()
+                    tan = sensitivity(ctx, prob, last_ok, lam, opts)
+                    secs += time.perf_counter() - t0
                 if lam == target:
                     break
                 lam = target  # reduced step succeeded: continue to the requested value

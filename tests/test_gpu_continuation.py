@@ -1,0 +1,68 @@
+"""Continuation on the GPU: the first-order predictor's sensitivity dx/dRa
+(Goma's soln_sens) against a difference quotient of two converged solutions,
+and a short first-order sweep against the zeroth-order one."""
+import numpy as np
+import pytest
+
+from goma_amd import problems
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from goma_amd import capi as M
+
+    M.lib()
+    assert torch.cuda.is_available()
+    return M
+
+
+OPTS = dict(kspace=20, max_its=40, precond=3, line_search=1, max_newton=25)
+
+
+def test_sensitivity_matches_difference_quotient(capi):
+    from goma_amd import continuation
+
+    prob = problems.natural_convection(n=8, Ra=1e4)
+    print("[case] sensitivity natconv n=8 Ra=1e4", flush=True)
+    ctx = capi.Context(prob)
+    Ra, h = 1e4, 1.0
+    xs = []
+    for ra in (Ra - h, Ra + h):
+        ctx.set_physics(problems.physics_at(prob, ra))
+        x = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+        _, rep = ctx.solve_dev(x.data_ptr(), eps_resid=1e-12, **OPTS)
+        assert rep.converged
+        xs.append(x)
+    ctx.set_physics(problems.physics_at(prob, Ra))
+    x0 = 0.5 * (xs[0] + xs[1])
+    _, rep = ctx.solve_dev(x0.data_ptr(), eps_resid=1e-12, **OPTS)
+    assert rep.converged
+    t = continuation.sensitivity(ctx, prob, x0, Ra, OPTS).cpu().numpy()
+    fd = ((xs[1] - xs[0]) / (2 * h)).cpu().numpy()
+    assert np.abs(t - fd).max() <= 1e-5 * np.abs(fd).max()
+    # a wrong sign or scale would be far off
+    assert np.abs(-t - fd).max() > 0.5 * np.abs(fd).max()
+    ctx.close()
+
+
+def test_first_order_sweep_saves_newton_passes(capi):
+    from goma_amd import continuation
+
+    prob = problems.natural_convection(n=16, Ra=1e4, carreau=True, jitter=0.15)
+    print("[case] first- vs zeroth-order sweep carreau n=16", flush=True)
+    ras = problems.sweep_ra(8, 1e3, 1e4)
+    its = {}
+    sols = {}
+    for order in (0, 1):
+        ctx = capi.Context(prob)
+        steps, s = continuation.sweep(ctx, prob, ras, dict(OPTS), order=order)
+        assert all(st.converged for st in steps)
+        its[order] = sum(st.newton_its for st in steps[1:])
+        sols[order] = s[-1]
+        ctx.close()
+    assert its[1] < its[0]
+    assert np.abs(sols[1] - sols[0]).max() <= 1e-8 * np.abs(sols[0]).max()

@@ -6,9 +6,9 @@
 // GMRES interface as the other preconditioners.
 //
 // Storage.  With kl / ku the lower / upper bandwidths of the MSR graph, the
-// band lives column-major in ld = kl + ku + 1 + 2 NB entries per column,
-// A(i, j) at ab[off + i + j * lda] with off = NB + ku and lda = ld - 1, for
-// j - ku - NB <= i <= j + kl + NB.  The NB-entry margins above and below the
+// band lives column-major in ld = kl + ku + 1 + 2 MG entries per column,
+// A(i, j) at ab[off + i + j * lda] with off = MG + ku and lda = ld - 1, for
+// j - ku - MG <= i <= j + kl + MG.  The MG-entry margins above and below the
 // band let every panel step treat its window as a plain dense matrix: window
 // entries outside the band start at zero and stay exactly zero (each of their
 // updates has a zero factor).  Without pivoting no fill leaves the band.
@@ -31,7 +31,8 @@
 
 namespace {
 
-constexpr int NB = 64;
+constexpr int NB = 64;  // panel width
+constexpr int MG = NB;  // storage margin above and below the band
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_band_zero(size_t n, T *__restrict__ p) {
@@ -54,7 +55,8 @@ __global__ __launch_bounds__(256) void k_band_load(int N, const VT *__restrict__
 // lower, unit diagonal implied) and U.
 template <typename T>
 __device__ inline void block_lu(T (*s)[NB + 1], int kb, int *flag) {
-  const int tid = threadIdx.x;
+  // thread (i = tid / 4, q = tid % 4) owns row i, columns q, q + 4, ..., q + 60
+  const int tid = threadIdx.x, i = tid >> 2, q = tid & 3;
   for (int k = 0; k < kb; k++) {
     __syncthreads();
     T piv = s[k][k];
@@ -62,12 +64,17 @@ __device__ inline void block_lu(T (*s)[NB + 1], int kb, int *flag) {
       if (tid == 0) atomicOr(flag, 1);
       piv = T(1);
     }
-    const int m = kb - k - 1;
-    for (int i = tid; i < m; i += 256) s[k + 1 + i][k] /= piv;
-    __syncthreads();
-    for (int t = tid; t < m * m; t += 256) {
-      const int i = k + 1 + t / m, j = k + 1 + t % m;
-      s[i][j] -= s[i][k] * s[k][j];
+    const bool act = i > k && i < kb;
+    T l = T(0);
+    if (act) l = s[i][k] / piv;
+    __syncthreads();  // every thread has read s[i][k] and row k before they change
+    if (act) {
+      if (q == 0) s[i][k] = l;
+#pragma unroll
+      for (int c = 0; c < NB / 4; c++) {
+        const int j = q + 4 * c;
+        if (j > k && j < kb) s[i][j] -= l * s[k][j];
+      }
     }
   }
   __syncthreads();
@@ -98,11 +105,16 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_band_panel(int j0, int kb, int R, int C, int nrb,
                                                     T *__restrict__ ab, long long off, long long lda) {
   __shared__ T sA[NB][NB + 1];
+  __shared__ T sR[NB];  // 1 / U11 diagonal
   const int tid = threadIdx.x;
   T *__restrict__ base = ab + off;
   for (int t = tid; t < NB * NB; t += 256) {
     const int j = t >> 6, i = t & 63;
     sA[i][j] = (i < kb && j < kb) ? base[(j0 + i) + (size_t)(j0 + j) * lda] : T(0);
+  }
+  if (tid < NB) {
+    const T d = tid < kb ? base[(j0 + tid) + (size_t)(j0 + tid) * lda] : T(1);
+    sR[tid] = T(1) / (d == T(0) ? T(1) : d);
   }
   __syncthreads();
   const int b = blockIdx.x;
@@ -117,8 +129,7 @@ __global__ __launch_bounds__(256) void k_band_panel(int j0, int kb, int R, int C
 #pragma unroll
     for (int k = 0; k < NB; k++) {
       if (k < kb) {
-        const T d = sA[k][k];
-        a[k] /= (d == T(0) ? T(1) : d);
+        a[k] *= sR[k];
 #pragma unroll
         for (int j = k + 1; j < NB; j++) a[j] -= a[k] * sA[k][j];
       }
@@ -147,67 +158,73 @@ __global__ __launch_bounds__(256) void k_band_panel(int j0, int kb, int R, int C
   }
 }
 
-// A22 -= L21 U12 on rows [j0+kb, j0+kb+R) x columns [j0+kb, j0+kb+C).  With
-// next_kb > 0, workgroup (0, 0) -- whose tile holds the next panel's A11, now
-// final -- also factors that block in place (from its own tile in LDS).
+// A[rs..re) x [cs..ce) -= L(rows, k0..k0+K) U(k0..k0+K, cols), K taken in
+// chunks of NB; 64 x 64 tiles, 4 x 4 per thread (measured faster than 128 x 128
+// tiles with 8 x 8 per thread, whose LDS allows one workgroup per CU).  With
+// fuse > 0, workgroup (0, 0) -- whose tile then holds the next panel's
+// diagonal block, now final -- also factors that block's fuse x fuse corner in
+// place (from its own tile in LDS).
 template <typename T>
-__global__ __launch_bounds__(256) void k_band_update(int j0, int kb, int R, int C, int next_kb,
-                                                     T *__restrict__ ab, long long off, long long lda,
-                                                     int *flag) {
+__global__ __launch_bounds__(256) void k_band_update(int k0, int K, int rs, int re, int cs, int ce,
+                                                     int fuse, T *__restrict__ ab, long long off,
+                                                     long long lda, int *flag) {
   __shared__ T sL[NB][NB + 1];  // [k][row]; then the updated tile of workgroup (0, 0), [row][col]
   __shared__ T sU[NB][NB + 1];  // [k][column]
   T *__restrict__ base = ab + off;
   const int tid = threadIdx.x;
-  const int rb = blockIdx.x * 64, cb = blockIdx.y * 64;  // tile origin within the window
-  const size_t r0 = (size_t)j0 + kb + rb, c0 = (size_t)j0 + kb + cb;
-  for (int t = tid; t < 64 * 64; t += 256) {
-    const int k = t >> 6, i = t & 63;
-    sL[k][i] = (k < kb && rb + i < R) ? base[(r0 + i) + (size_t)(j0 + k) * lda] : T(0);
-  }
-  for (int t = tid; t < 64 * 64; t += 256) {
-    const int j = t >> 6, k = t & 63;
-    sU[k][j] = (k < kb && cb + j < C) ? base[(j0 + k) + (c0 + j) * lda] : T(0);
-  }
-  __syncthreads();
+  const int r0 = rs + blockIdx.x * NB, c0 = cs + blockIdx.y * NB;
   const int tr = tid & 15, tc = tid >> 4;
   T acc[4][4];
 #pragma unroll
   for (int u = 0; u < 4; u++)
 #pragma unroll
     for (int v = 0; v < 4; v++) acc[u][v] = T(0);
-  for (int k = 0; k < kb; k++) {
-    T a[4], bv[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      a[u] = sL[k][tr + 16 * u];
-      bv[u] = sU[k][tc + 16 * u];
+  for (int kc = 0; kc < K; kc += NB) {
+    const int kn = min(NB, K - kc);
+    if (kc) __syncthreads();
+    for (int t = tid; t < NB * NB; t += 256) {
+      const int k = t >> 6, i = t & 63;
+      sL[k][i] = (k < kn && r0 + i < re) ? base[(size_t)(r0 + i) + (size_t)(k0 + kc + k) * lda] : T(0);
     }
+    for (int t = tid; t < NB * NB; t += 256) {
+      const int j = t >> 6, k = t & 63;
+      sU[k][j] = (k < kn && c0 + j < ce) ? base[(size_t)(k0 + kc + k) + (size_t)(c0 + j) * lda] : T(0);
+    }
+    __syncthreads();
+    for (int k = 0; k < kn; k++) {
+      T a[4], bv[4];
 #pragma unroll
-    for (int u = 0; u < 4; u++)
+      for (int u = 0; u < 4; u++) {
+        a[u] = sL[k][tr + 16 * u];
+        bv[u] = sU[k][tc + 16 * u];
+      }
 #pragma unroll
-      for (int v = 0; v < 4; v++) acc[u][v] += a[u] * bv[v];
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[u][v] += a[u] * bv[v];
+    }
   }
-  const bool fuse = next_kb > 0 && blockIdx.x == 0 && blockIdx.y == 0;
-  if (fuse) __syncthreads();  // uniform per workgroup: sL is reused below
+  const bool fz = fuse > 0 && blockIdx.x == 0 && blockIdx.y == 0;
+  if (fz) __syncthreads();  // uniform per workgroup: sL is reused below
 #pragma unroll
   for (int v = 0; v < 4; v++) {
-    const int cj = cb + tc + 16 * v;
+    const int cj = c0 + tc + 16 * v;
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int ri = rb + tr + 16 * u;
-      if (ri < R && cj < C) {
-        T *p = base + (r0 + tr + 16 * u) + (c0 + tc + 16 * v) * lda;
+      const int ri = r0 + tr + 16 * u;
+      if (ri < re && cj < ce) {
+        T *p = base + (size_t)ri + (size_t)cj * lda;
         const T nv = *p - acc[u][v];
         *p = nv;
-        if (fuse) sL[tr + 16 * u][tc + 16 * v] = nv;
+        if (fz) sL[tr + 16 * u][tc + 16 * v] = nv;
       }
     }
   }
-  if (!fuse) return;
-  block_lu<T>(sL, next_kb, flag);
+  if (!fz) return;
+  block_lu<T>(sL, fuse, flag);
   for (int t = tid; t < NB * NB; t += 256) {
     const int j = t >> 6, i = t & 63;
-    if (i < next_kb && j < next_kb) base[(r0 + i) + (c0 + j) * lda] = sL[i][j];
+    if (i < fuse && j < fuse) base[(size_t)(r0 + i) + (size_t)(c0 + j) * lda] = sL[i][j];
   }
 }
 
@@ -261,20 +278,23 @@ __global__ __launch_bounds__(256) void k_band_bwd(int j0, int kb, int R, const T
                                                   long long off, long long lda, double *__restrict__ w,
                                                   double *__restrict__ x) {
   __shared__ T sT[NB][NB + 1];
-  __shared__ double sx[NB];
+  __shared__ double sx[NB], sr[NB];
   const T *__restrict__ base = ab + off;
   const int tid = threadIdx.x, lane = tid & 63;
   for (int t = tid; t < NB * NB; t += 256) {
     const int j = t >> 6, i = t & 63;
     sT[i][j] = (i < kb && j < kb && i <= j) ? base[(j0 + i) + (size_t)(j0 + j) * lda] : T(0);
   }
-  if (tid < NB) sx[tid] = tid < kb ? w[j0 + tid] : 0.0;
+  if (tid < NB) {
+    sx[tid] = tid < kb ? w[j0 + tid] : 0.0;
+    const double d = tid < kb ? (double)base[(j0 + tid) + (size_t)(j0 + tid) * lda] : 1.0;
+    sr[tid] = 1.0 / (d == 0.0 ? 1.0 : d);
+  }
   __syncthreads();
   if (tid < 64) {
     double v = sx[lane];
     for (int k = kb - 1; k >= 0; k--) {
-      const double d = (double)sT[k][k];
-      if (lane == k) v /= (d == 0.0 ? 1.0 : d);
+      if (lane == k) v *= sr[k];
       const double vk = __shfl(v, k);
       if (lane < k) v -= (double)sT[lane][k] * vk;
     }
@@ -315,7 +335,7 @@ static int band_plan(gm_ctx *c) {
     }
   B.kl = kl;
   B.ku = ku;
-  B.ld = (long long)kl + ku + 1 + 2 * NB;
+  B.ld = (long long)kl + ku + 1 + 2 * MG;
   GM_HIP(hipMalloc(&B.d_w1, (size_t)c->N * sizeof(double)));
   GM_HIP(hipMalloc(&B.d_w2, (size_t)c->N * sizeof(double)));
   return GM_OK;
@@ -325,34 +345,44 @@ template <typename VT, typename T>
 static int band_factor_t(gm_ctx *c, const VT *val, T *ab) {
   BandLU &B = c->band;
   const int N = c->N, kl = B.kl, ku = B.ku;
-  const long long off = NB + ku, lda = B.ld - 1;
+  const long long off = MG + ku, lda = B.ld - 1;
   const size_t total = (size_t)N * B.ld;
   hipStream_t s = c->stream;
   k_band_zero<T><<<4096, 256, 0, s>>>(total, ab);
   k_band_load<VT, T><<<(N + 255) / 256, 256, 0, s>>>(N, val, c->d_bindx, ab, off, lda);
   GM_HIP(hipMemsetAsync(c->d_flag + 4, 0, sizeof(int), s));
+  int *d_pivflag = c->d_flag + 4;
   bool a11_done = false;  // the next panel's A11 was factored by the last update
-  for (int j0 = 0; j0 < N; j0 += NB) {
-    const int kb = std::min(NB, N - j0);
+  auto panel = [&](int j0, int kb) {
     const int R = std::min(kl, N - j0 - kb), C = std::min(ku, N - j0 - kb);
     const int nrb = (R + 255) / 256, ncb = (C + 255) / 256;
     if (!a11_done)
       gm_launch(c, GM_K_BAND_PANEL, 2.0 * kb * kb * sizeof(T), k_band_a11<T>, dim3(1), dim3(256), 0, s,
-                j0, kb, ab, off, lda, c->d_flag + 4);
+                j0, kb, ab, off, lda, d_pivflag);
+    a11_done = false;
     if (nrb + ncb > 0) {
       const double pbytes = ((double)kb * kb + (double)(R + C) * kb * 2.0) * sizeof(T);
       gm_launch(c, GM_K_BAND_PANEL, pbytes, k_band_panel<T>, dim3(nrb + ncb), dim3(256), 0, s, j0, kb,
                 R, C, nrb, ab, off, lda);
     }
+  };
+  auto update = [&](int k0, int K, int rs, int re, int cs, int ce, int fuse) {
+    if (re <= rs || ce <= cs) return;
+    const double ubytes = ((double)(re - rs + ce - cs) * K + 2.0 * (re - rs) * (ce - cs)) * sizeof(T);
+    gm_launch(c, GM_K_BAND_UPDATE, ubytes, k_band_update<T>,
+              dim3((re - rs + NB - 1) / NB, (ce - cs + NB - 1) / NB), dim3(256), 0, s, k0, K, rs, re, cs, ce,
+              fuse, ab, off, lda, d_pivflag);
+    if (fuse) a11_done = true;
+  };
+  for (int j0 = 0; j0 < N;) {
+    const int kb = std::min(NB, N - j0);
+    const int R = std::min(kl, N - j0 - kb), C = std::min(ku, N - j0 - kb);
     const int next_kb = std::min(NB, std::max(0, N - j0 - kb));
-    a11_done = false;
-    if (R > 0 && C > 0) {
-      const int fuse = (next_kb > 0 && R >= next_kb && C >= next_kb) ? next_kb : 0;
-      const double ubytes = ((double)(R + C) * kb + 2.0 * R * C) * sizeof(T);
-      gm_launch(c, GM_K_BAND_UPDATE, ubytes, k_band_update<T>, dim3((R + 63) / 64, (C + 63) / 64),
-                dim3(256), 0, s, j0, kb, R, C, fuse, ab, off, lda, c->d_flag + 4);
-      a11_done = fuse > 0;
-    }
+    panel(j0, kb);
+    if (R > 0 && C > 0)
+      update(j0, kb, j0 + kb, j0 + kb + R, j0 + kb, j0 + kb + C,
+             (next_kb > 0 && R >= next_kb && C >= next_kb) ? next_kb : 0);
+    j0 += kb;
   }
   GM_HIP(hipGetLastError());
   int flag = 0;
@@ -396,7 +426,7 @@ template <typename T>
 static void band_apply_t(gm_ctx *c, const T *ab, const double *in, double *out) {
   BandLU &B = c->band;
   const int N = c->N, kl = B.kl, ku = B.ku;
-  const long long off = NB + ku, lda = B.ld - 1;
+  const long long off = MG + ku, lda = B.ld - 1;
   hipStream_t s = c->stream;
   k_band_copy<<<1024, 256, 0, s>>>(N, in, B.d_w1);
   for (int j0 = 0; j0 < N; j0 += NB) {

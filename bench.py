@@ -343,7 +343,8 @@ def learned_bench(args, torch, capi):
     nlev = ctx.attach_multigrid(args.mg_coarse)
     base = dict(kspace=args.kspace, max_its=args.max_its, max_newton=40, precond=2,
                 mg_nu=args.mg_nu, mg_omega=args.mg_omega, line_search=args.line_search,
-                pc_reuse=args.pc_reuse, mg_smoother=args.mg_smoother, mg_galerkin=args.mg_galerkin)
+                pc_reuse=args.pc_reuse, mg_smoother=args.mg_smoother, mg_galerkin=args.mg_galerkin,
+                lin_tol_adapt=args.lin_adapt)
     fp64, mixed = dict(base, precision=0), dict(base, precision=1)
     groups = field_groups(ctx, prob)
     log(f"[learned] {prob.name}: N={ctx.N} levels {nlev}, target Ra={target:g}")
@@ -479,6 +480,8 @@ def main(argv=None):
                     help="element-block smoother: 0 multiplicative, 1 additive (one residual per sweep)")
     ap.add_argument("--mg-galerkin", type=int, default=1,
                     help="1: Galerkin R A P coarse operators; 0: re-assembled on each coarse mesh")
+    ap.add_argument("--lin-adapt", type=int, default=1,
+                    help="1: Eisenstat-Walker linear tolerances; 0: every solve to 1e-6 (Goma's fixed)")
     ap.add_argument("--sweep-precond", choices=["band", "mg"], default="band",
                     help="sweep workload: banded LU direct solve or multigrid")
     ap.add_argument("--held-out", type=int, default=32, help="sweep workload: held-out Ra values")
@@ -539,7 +542,7 @@ def main(argv=None):
     opts = dict(kspace=args.kspace, max_its=args.max_its, patch_bins=bins, precision=prec,
                 max_newton=40, verbose=1 if args.verbose else 0, precond=pc, mg_nu=args.mg_nu,
                 mg_omega=args.mg_omega, line_search=args.line_search, pc_reuse=args.pc_reuse, mg_smoother=args.mg_smoother,
-                mg_galerkin=args.mg_galerkin)
+                mg_galerkin=args.mg_galerkin, lin_tol_adapt=args.lin_adapt)
     for w in range(args.warmup):
         x.zero_()
         torch.cuda.synchronize()
@@ -622,7 +625,7 @@ def main(argv=None):
         "data": "synthetic",
         "config": {"workload": f"{prob.name} ({args.workload}) " + ", ".join(f"{k}={v}" for k, v in params.items()),
                    "unknowns": ctx.N, "nnz": ctx.nval - 1, "elements": prob.mesh.num_elems,
-                   "precision": args.precision, "solver": f"GMRES({args.kspace}) + " + ("multigrid V(%d,%d) %s element-block smoothing, %s coarse operators, %d levels, exact coarsest solve" % (args.mg_nu, args.mg_nu, ("multiplicative", "additive")[args.mg_smoother], ("re-assembled", "Galerkin")[args.mg_galerkin], nlev) if pc == 2 else "ILU(0) patch order" if pc == 1 else "banded LU") + ", preconditioner reuse " + ("recalc", "reuse", "adaptive")[args.pc_reuse],
+                   "precision": args.precision, "solver": ("Eisenstat-Walker forcing terms (>= 1e-6), " if args.lin_adapt else "") + f"GMRES({args.kspace}) + " + ("multigrid V(%d,%d) %s element-block smoothing, %s coarse operators, %d levels, exact coarsest solve" % (args.mg_nu, args.mg_nu, ("multiplicative", "additive")[args.mg_smoother], ("re-assembled", "Galerkin")[args.mg_galerkin], nlev) if pc == 2 else "ILU(0) patch order" if pc == 1 else "banded LU") + ", preconditioner reuse " + ("recalc", "reuse", "adaptive")[args.pc_reuse],
                    "initial_guess": "zero"},
         "newton_its": newton,
         "gmres_its": gm_its,

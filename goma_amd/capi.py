@@ -59,7 +59,7 @@ class GmSolverOpts(C.Structure):
                 ("precision", C.c_int), ("verbose", C.c_int), ("mg_nu", C.c_int),
                 ("mg_omega", C.c_double), ("mg_coarse_sweeps", C.c_int), ("line_search", C.c_int),
                 ("min_damp", C.c_double), ("pc_reuse", C.c_int), ("mg_smoother", C.c_int),
-                ("mg_galerkin", C.c_int)]
+                ("mg_galerkin", C.c_int), ("lin_tol_adapt", C.c_int)]
 
 
 MAX_REPORT = 64

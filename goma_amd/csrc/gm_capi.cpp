@@ -96,6 +96,7 @@ extern "C" void gm_default_solver_opts(gm_solver_opts *o) {
   o->pc_reuse = 0;
   o->mg_smoother = 0;
   o->mg_galerkin = 0;
+  o->lin_tol_adapt = 0;
 }
 
 static void free_ctx(gm_ctx *c) {
@@ -414,6 +415,7 @@ static int solve_impl(gm_ctx *c, double *x, const gm_solver_opts *o, gm_newton_r
   int converged = 0, inewton = 0;
   bool have_pc = false, last_ok = true;  // preconditioner reuse state
   int its_ref = 0, last_its = 0;
+  double eta = 0.1, L2_prev = -1.0;  // Eisenstat-Walker forcing term and the last scaled norm
   const bool galerkin = o->precond == GM_PC_MG && o->mg_galerkin && !c->mg.empty();
   if (galerkin && (rc = gm_mg_kel_alloc(c))) return rc;
   while (!converged && inewton < o->max_newton) {
@@ -464,6 +466,20 @@ static int solve_impl(gm_ctx *c, double *x, const gm_solver_opts *o, gm_newton_r
       int its = 0;
       double rr = 0.0;
       gm_solver_opts oo = *o;
+      if (o->lin_tol_adapt) {
+        // Eisenstat-Walker choice 2 with its safeguard (SIAM J. Sci. Comput. 17, 1996)
+        const double emax = 0.1, gam = 0.9;
+        double e = emax;
+        if (L2_prev > 0.0) {
+          const double r = L2 / L2_prev;
+          e = gam * r * r;
+          const double safe = gam * eta * eta;
+          if (safe > 0.1) e = std::max(e, safe);
+        }
+        eta = std::min(emax, std::max(e, o->lin_tol));
+        oo.lin_tol = eta;
+      }
+      L2_prev = L2;
       const bool rebuild = !have_pc || o->pc_reuse == 0 ||
                            (o->pc_reuse == 2 && (!last_ok || last_its > 1.5 * its_ref + 2));
       // preconditioner set-up timed separately, then GMRES

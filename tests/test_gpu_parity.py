@@ -253,3 +253,27 @@ def test_newton_line_search_matches_oracle(capi, oracle_mod):
     x = xd.cpu().numpy()
     assert np.abs(x - x_ref).max() <= 1e-8 * np.abs(x_ref).max()
     ctx.close()
+
+
+def test_newton_adaptive_tolerance_matches_oracle(capi, oracle_mod):
+    """Eisenstat-Walker linear tolerances change the inner solves only: the
+    Newton fixed point stays the oracle's."""
+    prob = problems.natural_convection(n=8, Ra=1e4)
+    print("[case] newton multigrid, Eisenstat-Walker tolerances, natconv n=8", flush=True)
+    ctx = capi.Context(prob)
+    ctx.attach_multigrid(2)
+    orc = oracle_mod.Oracle(prob)
+    rc, x_ref, rep_ref = orc.newton(precond=1, kspace=50, max_its=1000)
+    assert rep_ref.converged
+    its = {}
+    for adapt in (0, 1):
+        xd = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        rc, rep = ctx.solve_dev(xd.data_ptr(), kspace=50, max_its=200, precond=2, line_search=1,
+                                mg_smoother=1, mg_nu=2, mg_galerkin=1, lin_tol_adapt=adapt)
+        assert rep.converged
+        x = xd.cpu().numpy()
+        assert np.abs(x - x_ref).max() <= 1e-8 * np.abs(x_ref).max()
+        its[adapt] = rep.lin_its_total
+    assert its[1] < its[0]
+    ctx.close()

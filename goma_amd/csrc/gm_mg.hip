@@ -172,10 +172,11 @@ __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const V
   using Lc = Loc<DIM, EN>;
   constexpr int n = Lc::N;
   constexpr int RA = (n + TR - 1) / TR, CB = (n + TC - 1) / TC;
-  __shared__ CT s_col[n], s_row[n], s_rowk[n];
+  // pivot column and the two rows of each step, double-buffered by step parity
+  // so that a step needs two barriers: after the column, and after the rows
+  __shared__ CT s_col[2][n], s_row[2][n], s_rowk[2][n];
   __shared__ int s_piv[n], s_idx[n], s_pos[n];
-  __shared__ int s_p;
-  const int e = blockIdx.x, tid = threadIdx.x;
+  const int e = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
   const int tr = tid % TR, tc = tid / TR;
   const int *map = L.e2nz + (size_t)e * L.E;
   CT A[RA][CB];
@@ -188,6 +189,8 @@ __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const V
       A[a][b] = p >= 0 ? (CT)val[map[p]] : (CT)0;
     }
   for (int k = 0; k < n; k++) {
+    const int q = k & 1;
+    CT *const col = s_col[q], *const row = s_row[q], *const rowk = s_rowk[q];
     const int bk = k / TC, ck = k - bk * TC;
     if (tc == ck) {
 #pragma unroll
@@ -197,15 +200,17 @@ __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const V
         for (int b = 0; b < CB; b++)
           if (b == bk) v = A[a][b];
         const int i = tr + TR * a;
-        if (i < n) s_col[i] = v;
+        if (i < n) col[i] = v;
       }
     }
     __syncthreads();
-    if (tid < 64) {
+    // every wave finds the pivot row itself (same data, same order: same row)
+    int p;
+    {
       CT best = (CT)-1;
       int bi = n;
-      for (int i = k + tid; i < n; i += 64) {
-        const CT v = fabs(s_col[i]);
+      for (int i = k + lane; i < n; i += 64) {
+        const CT v = fabs(col[i]);
         if (v > best) {
           best = v;
           bi = i;
@@ -220,14 +225,9 @@ __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const V
           bi = oi;
         }
       }
-      if (tid == 0) {
-        if (bi >= n) bi = k;  // no comparable candidate (NaN column): keep the row
-        s_p = bi;
-        s_piv[k] = bi;
-      }
+      p = bi >= n ? k : bi;  // no comparable candidate (NaN column): keep the row
     }
-    __syncthreads();
-    const int p = s_p;
+    if (tid == 0) s_piv[k] = p;
     const int ap = p / TR, rp = p - ap * TR, ak = k / TR, rk = k - ak * TR;
     if (tr == rp || tr == rk) {
 #pragma unroll
@@ -240,42 +240,63 @@ __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const V
         }
         const int j = tc + TC * b;
         if (j < n) {
-          if (tr == rp) s_row[j] = vp;
-          if (tr == rk) s_rowk[j] = vk;
+          if (tr == rp) row[j] = vp;
+          if (tr == rk) rowk[j] = vk;
         }
       }
     }
     __syncthreads();
-    const CT piv = s_row[k];
+    const CT piv = row[k];
     // a structurally empty column (a local pressure mode with no velocity
     // coupling inside the element) leaves that unknown untouched
     const CT d = fabs(piv) > (CT)1e-300 ? (CT)1 / piv : (CT)0;
-    const CT fk = s_col[k];
-    CT rw[CB];
+    const CT fk = col[k];
+    // Step k of in-place Gauss-Jordan with rows k and p swapped: the new row k
+    // is the pivot row scaled by d (column k: d); every other row i subtracts
+    // f_i times it (f_i its column-k entry; row p's is old row k's, fk) and
+    // takes -f_i d in column k.  Written as one FMA per entry: column k is
+    // zeroed and row p replaced by old row k first, row k overwritten last.
+    CT rb[CB], fa[RA];
 #pragma unroll
     for (int b = 0; b < CB; b++) {
       const int j = tc + TC * b;
-      rw[b] = j < n ? s_row[j] : (CT)0;
+      rb[b] = j < n ? (j == k ? d : row[j] * d) : (CT)0;
     }
 #pragma unroll
     for (int a = 0; a < RA; a++) {
       const int i = tr + TR * a;
-      if (i >= n) continue;
-      const CT f = i == p ? fk : s_col[i];
-      const bool take_k = (i == p) && (p != k);
+      fa[a] = (i < n && i != k) ? (i == p ? fk : col[i]) : (CT)0;
+    }
+    if (tc == ck) {
+#pragma unroll
+      for (int a = 0; a < RA; a++)
+#pragma unroll
+        for (int b = 0; b < CB; b++)
+          if (b == bk) A[a][b] = (CT)0;
+    }
+    if (tr == rp && p != k) {
 #pragma unroll
       for (int b = 0; b < CB; b++) {
         const int j = tc + TC * b;
-        if (j >= n) continue;
-        const CT cur = take_k ? s_rowk[j] : A[a][b];
-        CT nv;
-        if (i == k) nv = j == k ? d : rw[b] * d;
-        else nv = j == k ? -f * d : cur - f * (rw[b] * d);
-        A[a][b] = nv;
+        const CT v = j < n ? rowk[j] : (CT)0;
+#pragma unroll
+        for (int a = 0; a < RA; a++)
+          if (a == ap) A[a][b] = b == bk && tc == ck ? (CT)0 : v;
       }
     }
-    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < RA; a++)
+#pragma unroll
+      for (int b = 0; b < CB; b++) A[a][b] -= fa[a] * rb[b];
+    if (tr == rk) {
+#pragma unroll
+      for (int b = 0; b < CB; b++)
+#pragma unroll
+        for (int a = 0; a < RA; a++)
+          if (a == ak) A[a][b] = rb[b];
+    }
   }
+  __syncthreads();
   if (tid == 0) {
     for (int j = 0; j < n; j++) s_idx[j] = j;
     for (int k = n - 1; k >= 0; k--) {

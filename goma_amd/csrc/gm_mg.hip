@@ -175,8 +175,8 @@ __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const V
   // pivot column and the two rows of each step, double-buffered by step parity
   // so that a step needs two barriers: after the column, and after the rows
   __shared__ CT s_col[2][n], s_row[2][n], s_rowk[2][n];
-  __shared__ int s_piv[n], s_idx[n], s_pos[n];
-  const int e = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+  __shared__ int s_piv[n], s_idx[n], s_pos[n], s_pp[2];
+  const int e = blockIdx.x, tid = threadIdx.x;
   const int tr = tid % TR, tc = tid / TR;
   const int *map = L.e2nz + (size_t)e * L.E;
   CT A[RA][CB];
@@ -193,6 +193,10 @@ __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const V
     CT *const col = s_col[q], *const row = s_row[q], *const rowk = s_rowk[q];
     const int bk = k / TC, ck = k - bk * TC;
     if (tc == ck) {
+      // the half-wave owning column k stages it and finds the pivot row from
+      // its registers (largest |entry| at or below k, first on ties)
+      CT best = (CT)-1;
+      int bi = n;
 #pragma unroll
       for (int a = 0; a < RA; a++) {
         CT v = 0;
@@ -200,33 +204,27 @@ __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const V
         for (int b = 0; b < CB; b++)
           if (b == bk) v = A[a][b];
         const int i = tr + TR * a;
-        if (i < n) col[i] = v;
-      }
-    }
-    __syncthreads();
-    // every wave finds the pivot row itself (same data, same order: same row)
-    int p;
-    {
-      CT best = (CT)-1;
-      int bi = n;
-      for (int i = k + lane; i < n; i += 64) {
-        const CT v = fabs(col[i]);
-        if (v > best) {
-          best = v;
-          bi = i;
+        if (i < n) {
+          col[i] = v;
+          if (i >= k && fabs(v) > best) {
+            best = fabs(v);
+            bi = i;
+          }
         }
       }
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        const CT ov = __shfl_xor(best, o, 64);
-        const int oi = __shfl_xor(bi, o, 64);
+      for (int o = TR / 2; o > 0; o >>= 1) {
+        const CT ov = __shfl_xor(best, o, TR);
+        const int oi = __shfl_xor(bi, o, TR);
         if (ov > best || (ov == best && oi < bi)) {
           best = ov;
           bi = oi;
         }
       }
-      p = bi >= n ? k : bi;  // no comparable candidate (NaN column): keep the row
+      if (tr == 0) s_pp[q] = bi >= n ? k : bi;  // no comparable candidate (NaN column): keep the row
     }
+    __syncthreads();
+    const int p = s_pp[q];
     if (tid == 0) s_piv[k] = p;
     const int ap = p / TR, rp = p - ap * TR, ak = k / TR, rk = k - ak * TR;
     if (tr == rp || tr == rk) {

@@ -58,3 +58,36 @@ def test_coarse_dirichlet_masks():
     lat = c.mesh.lattice()
     bnd = np.nonzero(np.any((lat == 0) | (lat == 2 * c.mesh.n), axis=1))[0]
     assert set(np.unique(c.dbc_node)) == set(bnd)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_galerkin_operator_fits_coarse_graph(dim, oracle_mod):
+    """R A P of the Dirichlet-free Jacobian has no entry outside the coarse
+    mesh's own MSR graph: P is element-local (every finer element lies in one
+    coarser element and interpolates only from its unknowns), which is what
+    lets the Galerkin coarse operator be assembled per coarse element."""
+    from dataclasses import replace
+
+    import scipy.sparse as sp
+
+    from goma_amd import multigrid, problems
+
+    prob = (problems.natural_convection(n=8, Ra=1e4, carreau=True, jitter=0.15) if dim == 2
+            else problems.lid_cavity(n=4, Re=400.0, dim=3, energy=True))
+    nat = replace(prob, dbc_node=prob.dbc_node[:0], dbc_var=prob.dbc_var[:0],
+                  dbc_value=prob.dbc_value[:0], datum_elem=-1)
+    of = oracle_mod.Oracle(nat)
+    x = np.random.default_rng(0).standard_normal(of.N) * 0.3
+    rc, _, v = of.fill(x)
+    assert rc == 0
+    A = oracle_mod.msr_to_scipy(of.ija, v, of.N)
+    cp = multigrid.coarse_problem(prob, multigrid.coarsen(prob.mesh))
+    oc = oracle_mod.Oracle(cp)
+    rp, ci, cv, _ = multigrid.prolongation(of.first_unknown, oc.first_unknown, prob.mesh, cp.mesh,
+                                           prob.phys.energy)
+    P = sp.csr_matrix((cv, ci, rp), shape=(of.N, oc.N))
+    rap = (P.T @ A @ P).tocsr()
+    rap.eliminate_zeros()
+    G = oracle_mod.msr_to_scipy(oc.ija, np.ones(oc.nval), oc.N)
+    assert rap.nnz > 0
+    assert abs(rap - rap.multiply(G)).sum() == 0.0

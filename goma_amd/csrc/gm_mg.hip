@@ -488,22 +488,30 @@ __global__ __launch_bounds__(NT) void k_vanka_update(LevelDev L, int nelem, cons
   }
 }
 
-// y = M v (CSR), optionally scaling v by sc and masking Dirichlet rows
-template <typename BT>
-__global__ void k_csr_apply(int nrows, const int *__restrict__ rp, const int *__restrict__ ci,
-                            const double *__restrict__ cv, const BT *__restrict__ v,
-                            const double *__restrict__ sc, const signed char *__restrict__ mask,
-                            BT *__restrict__ y, int accumulate) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nrows; i += gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int k = rp[i]; k < rp[i + 1]; k++) {
-      const int j = ci[k];
-      s += cv[k] * (double)v[j] * (sc ? sc[j] : 1.0);
-    }
-    if (mask && mask[i]) s = 0.0;
-    if (accumulate) y[i] += (BT)s;
-    else y[i] = (BT)s;
+// y = M v (CSR), optionally scaling v by sc and masking Dirichlet rows; G
+// lanes per row (consecutive lanes read consecutive entries of the row)
+template <typename BT, int G>
+__global__ __launch_bounds__(256) void k_csr_apply(int nrows, const int *__restrict__ rp,
+                                                   const int *__restrict__ ci,
+                                                   const double *__restrict__ cv,
+                                                   const BT *__restrict__ v,
+                                                   const double *__restrict__ sc,
+                                                   const signed char *__restrict__ mask,
+                                                   BT *__restrict__ y, int accumulate) {
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const int lane = threadIdx.x & (G - 1);
+  if (i >= nrows) return;  // whole groups of G lanes leave together
+  double s = 0.0;
+  for (int k = rp[i] + lane; k < rp[i + 1]; k += G) {
+    const int j = ci[k];
+    s += cv[k] * (double)v[j] * (sc ? sc[j] : 1.0);
   }
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
+  if (lane) return;
+  if (mask && mask[i]) s = 0.0;
+  if (accumulate) y[i] += (BT)s;
+  else y[i] = (BT)s;
 }
 
 // coarse state: nodal values injected from the finer level, pressure 0
@@ -1091,7 +1099,8 @@ int gm_mg_setup(gm_ctx *c, const double *x, const double *val, const float *val3
 
 static gm_ctx *g_prof_ctx = nullptr;  // context whose counters the V-cycle feeds
 
-// lanes per row: 2D rows hold ~100-250 entries, 3D rows ~500-2100
+// lanes per row: 2D rows hold ~50-250 entries (8 lanes: 33 vs 36 us per 256^2 residual
+// with 16), 3D rows ~500-2100
 template <typename VT, typename BT, int G>
 static void resid_g(gm_ctx *lc, hipStream_t s, const VT *a, const BT *x, const BT *b, BT *r) {
   const int blocks = (int)(((long long)lc->N * G + 255) / 256);
@@ -1104,7 +1113,7 @@ static void resid_g(gm_ctx *lc, hipStream_t s, const VT *a, const BT *x, const B
 template <typename VT, typename BT>
 static void resid(gm_ctx *lc, hipStream_t s, const VT *a, const BT *x, const BT *b, BT *r) {
   if (lc->dim == 3) resid_g<VT, BT, 64>(lc, s, a, x, b, r);
-  else resid_g<VT, BT, 16>(lc, s, a, x, b, r);
+  else resid_g<VT, BT, 8>(lc, s, a, x, b, r);
 }
 
 template <typename VT, typename BT, int G>
@@ -1173,8 +1182,14 @@ static void transfer(gm_ctx *c, hipStream_t s, int nrows, const int *rp, const i
                      const signed char *mask, BT *y, int acc) {
   const double bytes = (double)nnz * 12.0 + 4.0 * (nrows + 1) + (double)ncols * sizeof(BT) +
                        (double)nrows * (1 + (acc ? 2 : 1) * sizeof(BT));
-  gm_launch(c, GM_K_TRANSFER, bytes, k_csr_apply<BT>, dim3(1024), dim3(256), 0, s, nrows, rp, ci, cv,
-            v, sc, mask, y, acc);
+  const double avg = nrows ? (double)nnz / nrows : 0.0;  // entries per row
+  auto launch = [&](auto kern, int G) {
+    gm_launch(c, GM_K_TRANSFER, bytes, kern, dim3((int)(((long long)nrows * G + 255) / 256)), dim3(256),
+              0, s, nrows, rp, ci, cv, v, sc, mask, y, acc);
+  };
+  if (avg <= 4.0) launch(k_csr_apply<BT, 4>, 4);
+  else if (avg <= 12.0) launch(k_csr_apply<BT, 8>, 8);
+  else launch(k_csr_apply<BT, 16>, 16);
 }
 
 template <typename VT, typename BT>

@@ -80,6 +80,7 @@ __global__ __launch_bounds__(NT) void k_vanka_invert(LevelDev L, const VT *__res
                                                      AT *__restrict__ ainv) {
   using Lc = Loc<DIM, EN>;
   constexpr int n = Lc::N;
+  static_assert(n <= 32 && NT % 32 == 0, "2D element blocks: at most 32 unknowns, rows on 32 lanes");
   __shared__ CT A[n][n + 1];
   __shared__ CT s_col[n], s_row[n];
   __shared__ int s_piv[n];
@@ -130,13 +131,15 @@ __global__ __launch_bounds__(NT) void k_vanka_invert(LevelDev L, const VT *__res
     // a structurally empty column (a local pressure mode with no velocity
     // coupling inside the element) leaves that unknown untouched
     const CT d = fabs(piv) > (CT)1e-300 ? (CT)1 / piv : (CT)0;
-    for (int idx = tid; idx < n * n; idx += NT) {
-      const int i = idx / n, j = idx - i * n;
+    // thread (row ti, column group tg): no index division in the update
+    const int ti = tid & 31, tg = tid >> 5;
+    if (ti < n) {
+      const int i = ti;
       if (i == k) {
-        A[k][j] = j == k ? d : s_row[j] * d;
+        for (int j = tg; j < n; j += NT / 32) A[k][j] = j == k ? d : s_row[j] * d;
       } else {
         const CT f = i == p ? s_col[k] : s_col[i];
-        A[i][j] = j == k ? -f * d : A[i][j] - f * (s_row[j] * d);
+        for (int j = tg; j < n; j += NT / 32) A[i][j] = j == k ? -f * d : A[i][j] - f * (s_row[j] * d);
       }
     }
     __syncthreads();

@@ -107,7 +107,8 @@ static void free_ctx(gm_ctx *c) {
   void *ptrs[] = {c->d_coor, c->d_conn,   c->d_fu,    c->d_centroid, c->d_dbc,   c->d_dbc_val,
                   c->d_bindx, c->d_ecol_elems, c->d_e2nz, c->d_flag, c->d_tables, c->d_val,
                   c->d_resid, c->d_dx,    c->d_scale, c->d_val32,    c->d_red,   c->d_xwork,
-                  c->d_xsave, c->d_rtrial, c->d_w, c->d_mg_rows, c->d_mg_wmul, c->d_kel};
+                  c->d_xsave, c->d_rtrial, c->d_w, c->d_mg_rows, c->d_mg_wmul, c->d_kel,
+                  c->d_mg_gptr, c->d_mg_gent, c->d_mg_corr};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->h_red) hipHostFree(c->h_red);

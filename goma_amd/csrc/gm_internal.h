@@ -160,6 +160,9 @@ struct gm_ctx {
   const double *mg_scale = nullptr;  // row-sum scale of the level-0 matrix
   int *d_mg_rows = nullptr;          // unknowns of each element colour (smoother residuals)
   double *d_mg_wmul = nullptr;       // 1 / elements sharing each unknown (additive smoother)
+  int *d_mg_gptr = nullptr;          // [N+1] additive smoother: corrections of each unknown,
+  int *d_mg_gent = nullptr;          //   as indices e * n_loc + local dof into d_mg_corr
+  double *d_mg_corr = nullptr;       // [num_elems][n_loc] element corrections
   std::vector<int> mg_rows_ptr;
   std::vector<long long> mg_rows_nnz;
 

@@ -491,6 +491,47 @@ __global__ __launch_bounds__(NT) void k_vanka_update(LevelDev L, int nelem, cons
   }
 }
 
+// Additive smoother, first half: every element's correction c_e = A_e^-1 r_e
+// (all elements in one launch; nothing is written to x).
+template <int DIM, int EN, typename AT, typename BT, int NT>
+__global__ __launch_bounds__(NT) void k_vanka_corr(LevelDev L, int nelem, const AT *__restrict__ ainv,
+                                                   const BT *__restrict__ r, double *__restrict__ corr) {
+  using Lc = Loc<DIM, EN>;
+  constexpr int n = Lc::N;
+  constexpr int EPB = (2 * n <= NT) ? 2 : 1;
+  constexpr int HALF = NT / EPB;
+  __shared__ double s_r2[EPB][n];
+  const int tid = threadIdx.x, sub = tid / HALF, t = tid - sub * HALF;
+  const int e = xcd_block() * EPB + sub;
+  const bool act = e < nelem;
+  if (act)
+    for (int i = t; i < n; i += HALF) s_r2[sub][i] = (double)r[local_dof<DIM, EN>(L, e, i)];
+  __syncthreads();
+  if (act) {
+    const AT *M = ainv + (size_t)e * n * n;
+    for (int i = t; i < n; i += HALF) {
+      double s = 0.0;
+      for (int j = 0; j < n; j++) s += (double)M[j * n + i] * s_r2[sub][j];
+      corr[(size_t)e * n + i] = s;
+    }
+  }
+}
+
+// Second half: x_u += omega / (elements sharing u) * sum of u's corrections, in
+// a fixed order (deterministic).
+template <typename BT>
+__global__ __launch_bounds__(256) void k_vanka_gather(int N, const int *__restrict__ gptr,
+                                                      const int *__restrict__ gent,
+                                                      const double *__restrict__ corr,
+                                                      const double *__restrict__ wmul, double omega,
+                                                      BT *__restrict__ x) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= N) return;
+  double s = 0.0;
+  for (int k = gptr[u]; k < gptr[u + 1]; k++) s += corr[gent[k]];
+  x[u] += (BT)(omega * wmul[u] * s);
+}
+
 // y = M v (CSR), optionally scaling v by sc and masking Dirichlet rows; G
 // lanes per row (consecutive lanes read consecutive entries of the row)
 template <typename BT, int G>
@@ -977,6 +1018,28 @@ static int build_colour_rows(gm_ctx *lc) {
   for (double &v : w) v = v > 0.0 ? 1.0 / v : 0.0;
   int rc = upv(&lc->d_mg_wmul, w.data(), w.size());
   if (rc) return rc;
+  // gather lists of the additive smoother: local dof li of element e (nodal
+  // variables node-blocked by variable, then the element's P1 dofs) is unknown
+  // g; its correction sits at e * nl + li
+  const int nl = lc->nv * nve + lc->np;
+  std::vector<int> gptr(lc->N + 1, 0), gent((size_t)lc->num_elems * nl);
+  auto dof = [&](int e, int li) {
+    if (li < lc->nv * nve) {
+      const int a = li / nve, i = li - a * nve;
+      return lc->first_unknown[lc->conn[(size_t)e * nve + i]] + a;
+    }
+    return lc->first_unknown[lc->elem_centroid[e]] + lc->dim + lc->energy + (li - lc->nv * nve);
+  };
+  for (int e = 0; e < lc->num_elems; e++)
+    for (int li = 0; li < nl; li++) gptr[dof(e, li) + 1]++;
+  for (int u = 0; u < lc->N; u++) gptr[u + 1] += gptr[u];
+  std::vector<int> fill(gptr.begin(), gptr.end() - 1);
+  for (int e = 0; e < lc->num_elems; e++)
+    for (int li = 0; li < nl; li++) gent[fill[dof(e, li)]++] = e * nl + li;
+  if ((rc = upv(&lc->d_mg_gptr, gptr.data(), gptr.size())) ||
+      (rc = upv(&lc->d_mg_gent, gent.data(), gent.size())))
+    return rc;
+  GM_HIP(hipMalloc(&lc->d_mg_corr, gent.size() * sizeof(double)));
   return upv(&lc->d_mg_rows, rows.data(), rows.size());
 }
 
@@ -1155,18 +1218,46 @@ static void vanka_colour(gm_ctx *lc, hipStream_t s, int col, const AT *ainv, con
             omega, wmul);
 }
 
+template <int DIM, int EN, typename AT, typename BT>
+static void vanka_additive(gm_ctx *lc, hipStream_t s, const AT *ainv, const BT *r, BT *x, double omega) {
+  constexpr int NT = DIM == 2 ? 64 : 128;
+  constexpr int n = Loc<DIM, EN>::N;
+  constexpr int EPB = (2 * n <= NT) ? 2 : 1;
+  const int ne = lc->num_elems;
+  // inverses, connectivity, residual gathered, corrections written
+  const double bytes = (double)ne * (n * n * sizeof(AT) + 4.0 * (Loc<DIM, EN>::NVE + 1) +
+                                     n * (sizeof(BT) + 8.0));
+  gm_launch(g_prof_ctx, GM_K_VANKA, bytes, k_vanka_corr<DIM, EN, AT, BT, NT>, dim3((ne + EPB - 1) / EPB),
+            dim3(NT), 0, s, level_dev(lc), ne, ainv, r, lc->d_mg_corr);
+  // lists, corrections, weights, x read and written
+  const double gbytes = (double)ne * n * 12.0 + 4.0 * (lc->N + 1) + lc->N * (8.0 + 2.0 * sizeof(BT));
+  gm_launch(g_prof_ctx, GM_K_VANKA_GATHER, gbytes, k_vanka_gather<BT>, dim3((lc->N + 255) / 256),
+            dim3(256), 0, s, lc->N, (const int *)lc->d_mg_gptr, (const int *)lc->d_mg_gent,
+            (const double *)lc->d_mg_corr, (const double *)lc->d_mg_wmul, omega, x);
+}
+
 // Multiplicative: coloured block Gauss-Seidel, the residual recomputed on each
 // colour's rows.  Additive: one residual per sweep, every element's correction
-// from it, weighted by 1 / elements sharing the unknown (colours only keep the
-// writes apart).
+// from it (one launch), then each unknown sums its corrections weighted by
+// 1 / elements sharing it (a second launch; no colours needed).
 template <typename VT, typename BT>
 static void smooth(gm_ctx *lc, hipStream_t s, const VT *a, const VT *ainv, const BT *b, BT *x, BT *r,
                    int sweeps, double omega, int additive) {
-  const double *w = additive ? lc->d_mg_wmul : nullptr;
   for (int it = 0; it < sweeps; it++) {
-    if (additive) resid<VT, BT>(lc, s, a, x, b, r);
+    if (additive) {
+      resid<VT, BT>(lc, s, a, x, b, r);
+      if (lc->dim == 2) {
+        if (lc->energy) vanka_additive<2, 1, VT, BT>(lc, s, ainv, r, x, omega);
+        else vanka_additive<2, 0, VT, BT>(lc, s, ainv, r, x, omega);
+      } else {
+        if (lc->energy) vanka_additive<3, 1, VT, BT>(lc, s, ainv, r, x, omega);
+        else vanka_additive<3, 0, VT, BT>(lc, s, ainv, r, x, omega);
+      }
+      continue;
+    }
+    const double *w = nullptr;
     for (int col = 0; col < lc->n_ecol; col++) {
-      if (!additive) resid_rows<VT, BT>(lc, s, col, a, x, b, r);
+      resid_rows<VT, BT>(lc, s, col, a, x, b, r);
       if (lc->dim == 2) {
         if (lc->energy) vanka_colour<2, 1, VT, BT>(lc, s, col, ainv, r, x, omega, w);
         else vanka_colour<2, 0, VT, BT>(lc, s, col, ainv, r, x, omega, w);

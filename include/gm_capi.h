@@ -238,7 +238,8 @@ int gm_mg_copy_level_matrix(const gm_ctx *ctx, int level, double *val);
 #define GM_K_MDOT 5       /* k_mdot: Gram-Schmidt projections */
 #define GM_K_ORTH 6       /* k_orth_update: Gram-Schmidt update */
 #define GM_K_MG_RESID 7   /* k_mg_resid: multigrid residual r = b - A x, all rows of one level */
-#define GM_K_VANKA 8      /* k_vanka_update: element-block updates of one colour */
+#define GM_K_VANKA 8      /* k_vanka_update: element-block updates of one colour (multiplicative);
+                             k_vanka_corr: every element's correction (additive) */
 #define GM_K_TRANSFER 9   /* k_csr_apply: restriction or prolongation */
 #define GM_K_VANKA_INV 10 /* k_vanka_invert: element-block inverses of one level */
 #define GM_K_FILL_RESID 11 /* k_fill, residual only (line-search trial), one colour */
@@ -249,7 +250,8 @@ int gm_mg_copy_level_matrix(const gm_ctx *ctx, int level, double *val);
 #define GM_K_BAND_PANEL 16 /* k_band_a11 / k_band_panel: one 64-column panel of the banded LU */
 #define GM_K_BAND_UPDATE 17 /* k_band_update: trailing update of one panel */
 #define GM_K_BAND_SOLVE 18 /* k_band_fwd / k_band_bwd: one 64-row block of the triangular solves */
-#define GM_K_COUNT 19
+#define GM_K_VANKA_GATHER 19 /* k_vanka_gather: additive smoother, element corrections summed per unknown */
+#define GM_K_COUNT 20
 /* on = 0: off; 1: time every launch; k > 1: time a pseudo-random one launch
  * in k of each kernel (lower overhead), durations scaled to all launches by
  * the launch count. */

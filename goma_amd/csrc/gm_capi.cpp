@@ -468,13 +468,13 @@ static int solve_impl(gm_ctx *c, double *x, const gm_solver_opts *o, gm_newton_r
       double rr = 0.0;
       gm_solver_opts oo = *o;
       if (o->lin_tol_adapt) {
-        // Eisenstat-Walker choice 2 with its safeguard (SIAM J. Sci. Comput. 17, 1996)
-        const double emax = 0.1, gam = 0.9;
+        // Eisenstat-Walker choice 2 with its safeguard (SIAM J. Sci. Comput. 17, 1996),
+        // exponent (1 + sqrt 5) / 2 (measured: fewer GMRES iterations than 2 at 256^2 and 48^3)
+        const double emax = 0.1, gam = 0.9, alpha = 1.6180339887498949;
         double e = emax;
         if (L2_prev > 0.0) {
-          const double r = L2 / L2_prev;
-          e = gam * r * r;
-          const double safe = gam * eta * eta;
+          e = gam * std::pow(L2 / L2_prev, alpha);
+          const double safe = gam * std::pow(eta, alpha);
           if (safe > 0.1) e = std::max(e, safe);
         }
         eta = std::min(emax, std::max(e, o->lin_tol));

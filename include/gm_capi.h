@@ -118,7 +118,7 @@ typedef struct gm_solver_opts {
                           one), with each coarse level's own Dirichlet rows */
   int lin_tol_adapt;   /* 0: every linear solve to lin_tol (Goma's fixed Residual Ratio
                           Tolerance); 1: Eisenstat-Walker forcing terms (choice 2, gamma 0.9,
-                          alpha 2, at most 0.1, at least lin_tol) from the ratio of successive
+                          alpha (1+sqrt 5)/2, at most 0.1, at least lin_tol) from the ratio of successive
                           scaled residual norms -- loose solves far from the solution */
 } gm_solver_opts;
 

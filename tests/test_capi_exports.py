@@ -28,3 +28,6 @@ def test_default_options_are_gomas():
     o = capi.solver_opts()
     assert o.eps_resid == 1e-10 and o.eps_corr == 1e10 and o.min_damp == 0.05
     assert o.lin_tol == 1e-6 and o.max_its == 500 and o.line_search == 0
+    # additions without a Goma counterpart are off by default: Goma's fixed linear
+    # tolerance, re-assembled coarse operators, recalc preconditioners
+    assert o.lin_tol_adapt == 0 and o.mg_galerkin == 0 and o.pc_reuse == 0

@@ -32,7 +32,8 @@ from __future__ import annotations
 import numpy as np
 
 from . import netcdf3 as nc
-from .mesh import CENTROID_LOCAL, Mesh
+from .mesh import CENTROID_LOCAL, Mesh, band_order
+from .mesh import renumber as renumber_mesh
 from .problems import PRES, TEMP, Problem, _BC
 
 ELEM_TYPE = {2: "QUAD9", 3: "HEX27"}
@@ -149,9 +150,24 @@ def write(path, mesh: Mesh, node_sets: dict | None = None, times=None, nodal: di
 
 def write_solution(path, prob: Problem, x: np.ndarray, time: float = 0.0,
                    node_sets: dict | None = None, title: str | None = None) -> None:
-    """Mesh plus one time step of Goma's output fields for solution ``x``."""
+    """Mesh plus one time step of Goma's output fields for solution ``x``.
+
+    A problem that ``problem_from_exodus(..., renumber=True)`` built is
+    written in the numbering of the file it came from, with its node sets."""
     nodal, elem = solution_fields(prob, x)
-    write(path, prob.mesh, node_sets=node_sets, times=[time],
+    mesh = prob.mesh
+    if "exodus_node_order" in prob.meta:
+        no, eo = prob.meta["exodus_node_order"], prob.meta["exodus_elem_order"]
+        for k, v in nodal.items():
+            nodal[k] = np.empty_like(v)
+            nodal[k][no] = v
+        for k, v in elem.items():
+            elem[k] = np.empty_like(v)
+            elem[k][eo] = v
+        mesh = prob.meta["exodus_file_mesh"]
+        if node_sets is None:
+            node_sets = prob.meta.get("exodus_node_sets")
+    write(path, mesh, node_sets=node_sets, times=[time],
           nodal={k: v[None] for k, v in nodal.items()}, elem={k: v[None] for k, v in elem.items()},
           title=title or prob.name)
 
@@ -240,7 +256,8 @@ def read(path) -> ExodusFile:
 
 
 def problem_from_exodus(path, phys, bcs, datum_elem: int = 0, datum_value: float = 0.0,
-                        name: str | None = None, meta: dict | None = None) -> Problem:
+                        name: str | None = None, meta: dict | None = None,
+                        renumber: bool = False) -> Problem:
     """A Problem on the mesh of an Exodus II file.
 
     ``bcs`` lists Dirichlet conditions as Goma's input deck does
@@ -248,9 +265,27 @@ def problem_from_exodus(path, phys, bcs, datum_elem: int = 0, datum_value: float
     Goma's variable numbering (U1=0, U2=1, U3=2, T=3).  They are applied in
     order; where two conditions fix the same unknown of a node, the later one
     holds.  The pressure datum pins the first P1 coefficient of element
-    ``datum_elem``.  The mesh has no structured hierarchy (``mesh.n == 0``), so
-    the geometric multigrid preconditioner does not apply to it."""
+    ``datum_elem`` (numbered as in the file).  The mesh has no structured
+    hierarchy (``mesh.n == 0``), so the geometric multigrid preconditioner
+    does not apply to it.
+
+    ``renumber`` orders nodes and elements so that the Jacobian's band is
+    narrow (``mesh.band_order``), as the banded LU needs; ``write_solution`` then maps results back to the file's
+    numbering."""
     f = read(path)
+    file_mesh, file_sets = f.mesh, f.node_sets
+    if not 0 <= datum_elem < file_mesh.num_elems:
+        raise ValueError(f"datum element {datum_elem} is not in the mesh")
+    meta = dict(meta or {})
+    if renumber:
+        no, eo = band_order(file_mesh)
+        new_id = np.empty_like(no)
+        new_id[no] = np.arange(no.size)
+        f.mesh = renumber_mesh(file_mesh, no, eo)
+        f.node_sets = {k: np.sort(new_id[v]) for k, v in file_sets.items()}
+        datum_elem = int(np.flatnonzero(eo == datum_elem)[0])
+        meta.update(exodus_node_order=no, exodus_elem_order=eo, exodus_file_mesh=file_mesh,
+                    exodus_node_sets=file_sets)
     bc = _BC()
     for ns, var, value in bcs:
         if ns not in f.node_sets:
@@ -263,8 +298,6 @@ def problem_from_exodus(path, phys, bcs, datum_elem: int = 0, datum_value: float
             raise ValueError("a temperature condition needs the energy equation")
         bc.add(f.node_sets[ns], var, value)
     nn, vv, xx = bc.arrays()
-    if not 0 <= datum_elem < f.mesh.num_elems:
-        raise ValueError(f"datum element {datum_elem} is not in the mesh")
     return Problem(name=name or f"exodus_{f.title or 'mesh'}", mesh=f.mesh, phys=phys, dbc_node=nn,
                    dbc_var=vv, dbc_value=xx, datum_elem=datum_elem, datum_value=datum_value,
-                   meta=dict(meta or {}))
+                   meta=meta)

@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 from goma_amd import exodus, netcdf3 as nc, problems
-from goma_amd.mesh import Mesh, structured_mesh
+from goma_amd.mesh import Mesh, renumber, structured_mesh
 from goma_amd.problems import TEMP, U1, U2, U3
 
 scipy_io = pytest.importorskip("scipy.io")
@@ -289,19 +289,17 @@ def test_problem_from_exodus_rejects_bad_cards(tmp_path):
         _deck_problem(tmp_path, prob, [(1, U3, 0.0)])
 
 
-@pytest.mark.parametrize("dim", [2, 3])
-def test_renumbered_exodus_mesh_solves_to_the_same_fields(tmp_path, dim, oracle_mod):
-    """Exodus files carry arbitrary node and element numbering.  Random
-    renumbering of both, written and read back through Exodus, with the
-    conditions given as node-set cards: the oracle's Newton solution, mapped
-    back, equals the structured problem's field by field."""
-    spla = pytest.importorskip("scipy.sparse.linalg")
+def _renumbered_file(tmp_path, dim, seed=8):
+    """The structured problem, and an Exodus file of its mesh with nodes and
+    elements renumbered at random, node sets on the walls and the deck cards
+    for its conditions.  Returns (prob, path, bcs, datum, pn, pe): the file's
+    node ``pn[i]`` is node i, the file's element j is element ``pe[j]``."""
     prob = (problems.natural_convection(n=4, Ra=1e4, jitter=0.15) if dim == 2 else
             problems.lid_cavity(n=2, Re=100.0, dim=3, energy=True, Ra=1e3))
-    rng = np.random.default_rng(8)
+    rng = np.random.default_rng(seed)
     mesh = prob.mesh
-    pn = rng.permutation(mesh.num_nodes)  # new id of old node i is pn[i]
-    pe = rng.permutation(mesh.num_elems)  # old element pe[j] becomes element j
+    pn = rng.permutation(mesh.num_nodes)
+    pe = rng.permutation(mesh.num_elems)
     coor = np.empty_like(mesh.coor)
     coor[:, pn] = mesh.coor
     ren = Mesh(dim=dim, n=0, coor=coor, connect=pn[mesh.connect[pe]].astype(np.int32))
@@ -316,27 +314,107 @@ def test_renumbered_exodus_mesh_solves_to_the_same_fields(tmp_path, dim, oracle_
         bcs = [(4, U1, 1.0)] + [(4, v, 0.0) for v in vel[1:]]
         bcs += [(ns, v, 0.0) for ns in walls for v in vel] + [(4, TEMP, 0.5), (3, TEMP, -0.5)]
     datum = int(np.flatnonzero(pe == prob.datum_elem)[0])
+    return prob, p, bcs, datum, pn, pe
+
+
+def _direct_newton(oracle_mod, prob):
+    """Newton on the oracle's assembly with a SciPy direct solve (the linear
+    solver is not what these tests are about; ILU(0) in a random order is a
+    poor preconditioner).  Returns (x, oracle)."""
+    spla = pytest.importorskip("scipy.sparse.linalg")
+    orc = oracle_mod.Oracle(prob)
+    x = np.zeros(orc.N)
+    for it in range(20):
+        rc, R, V = orc.fill(x)
+        assert rc == 0
+        if np.abs(R).max() < 1e-11:
+            break
+        x -= spla.spsolve(oracle_mod.msr_to_scipy(orc.ija, V, orc.N).tocsc(), R)
+    assert np.abs(R).max() < 1e-11, f"no convergence: {np.abs(R).max()}"
+    return x, orc
+
+
+def _half_bandwidth(orc):
+    rows = np.repeat(np.arange(orc.N), np.diff(orc.ija[:orc.N + 1]))
+    cols = orc.ija[orc.ija[0]:orc.ija[orc.N]]
+    return int(np.abs(cols - rows).max())
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_renumbered_exodus_mesh_solves_to_the_same_fields(tmp_path, dim, oracle_mod):
+    """Exodus files carry arbitrary node and element numbering.  Random
+    renumbering of both, written and read back through Exodus, with the
+    conditions given as node-set cards: the oracle's Newton solution, mapped
+    back, equals the structured problem's field by field."""
+    prob, p, bcs, datum, pn, pe = _renumbered_file(tmp_path, dim)
     rp = exodus.problem_from_exodus(p, prob.phys, bcs, datum_elem=datum, datum_value=prob.datum_value)
     assert rp.mesh.n == 0
-    solved = []
-    for q in (prob, rp):
-        # Newton on the oracle's assembly with a direct solve: ILU(0) in the
-        # file's random order is a poor preconditioner, and the linear solver
-        # is not what this test is about
-        orc = oracle_mod.Oracle(q)
-        x = np.zeros(orc.N)
-        for it in range(20):
-            rc, R, V = orc.fill(x)
-            assert rc == 0
-            if np.abs(R).max() < 1e-11:
-                break
-            x -= spla.spsolve(oracle_mod.msr_to_scipy(orc.ija, V, orc.N).tocsc(), R)
-        assert np.abs(R).max() < 1e-11, f"no convergence: {np.abs(R).max()}"
-        solved.append(exodus.solution_fields(q, x))
-    (na, ea), (nb, eb) = solved
+    (na, ea), (nb, eb) = [exodus.solution_fields(q, _direct_newton(oracle_mod, q)[0]) for q in (prob, rp)]
     scale = max(np.abs(v).max() for v in na.values())
     for k in na:
         assert np.abs(nb[k][pn] - na[k]).max() <= 1e-9 * scale, k
     pscale = max(np.abs(v).max() for v in ea.values())
     for k in ea:
         assert np.abs(eb[k] - ea[k][pe]).max() <= 1e-9 * pscale, k
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_band_renumbering_narrows_the_band_and_writes_back_in_file_order(tmp_path, dim, oracle_mod):
+    """``renumber=True``: the Jacobian's half bandwidth on the random file
+    numbering is most of N; after ``band_order`` it is within 1.5x of the
+    structured numbering's.  The solution written back lands in the file's
+    numbering: read again, it equals the structured solution mapped by the
+    file's permutation."""
+    prob, p, bcs, datum, pn, pe = _renumbered_file(tmp_path, dim)
+    raw = exodus.problem_from_exodus(p, prob.phys, bcs, datum_elem=datum, datum_value=prob.datum_value)
+    rp = exodus.problem_from_exodus(p, prob.phys, bcs, datum_elem=datum, datum_value=prob.datum_value,
+                                    renumber=True)
+    x_ref, o_ref = _direct_newton(oracle_mod, prob)
+    x, o = _direct_newton(oracle_mod, rp)
+    bw_ref, bw_raw, bw = _half_bandwidth(o_ref), _half_bandwidth(oracle_mod.Oracle(raw)), _half_bandwidth(o)
+    assert bw_raw > 0.5 * o.N
+    assert bw <= 1.5 * bw_ref, (bw, bw_ref, bw_raw)
+    out = tmp_path / "solution.exo"
+    exodus.write_solution(out, rp, x)
+    f = exodus.read(out)
+    src = exodus.read(p)
+    assert np.array_equal(f.mesh.coor, src.mesh.coor) and np.array_equal(f.mesh.connect, src.mesh.connect)
+    assert all(np.array_equal(f.node_sets[k], src.node_sets[k]) for k in src.node_sets)
+    na, ea = exodus.solution_fields(prob, x_ref)
+    scale = max(np.abs(v).max() for v in na.values())
+    for k in na:
+        assert np.abs(f.nodal[k][0][pn] - na[k]).max() <= 1e-9 * scale, k
+    pscale = max(np.abs(v).max() for v in ea.values())
+    for k in ea:
+        assert np.abs(f.elem[k][0] - ea[k][pe]).max() <= 1e-9 * pscale, k
+
+
+def test_renumber_rejects_non_permutations():
+    mesh = structured_mesh(2, 2)
+    with pytest.raises(ValueError, match="node_order"):
+        renumber(mesh, np.zeros(mesh.num_nodes, np.int64), np.arange(mesh.num_elems))
+    with pytest.raises(ValueError, match="elem_order"):
+        renumber(mesh, np.arange(mesh.num_nodes), np.zeros(mesh.num_elems, np.int64))
+
+
+@pytest.mark.parametrize("dim,n", [(2, 8), (3, 3)])
+def test_band_order_formula_and_box_recovery(dim, n, oracle_mod):
+    """The per-element half-bandwidth formula equals the oracle's MSR
+    pattern's (an energy problem: every unknown of two element-sharing nodes
+    couples across the widest pair); on a randomly renumbered box without
+    jitter the coordinate sweep gives back exactly the row-by-row band."""
+    from goma_amd.mesh import _dof_half_bandwidth, band_order
+
+    prob = (problems.natural_convection(n=n, Ra=1e4) if dim == 2 else
+            problems.lid_cavity(n=n, dim=3, energy=True, Ra=1e3))
+    mesh = prob.mesh
+    ident = np.arange(mesh.num_nodes)
+    assert _dof_half_bandwidth(mesh, ident) == _half_bandwidth(oracle_mod.Oracle(prob))
+    pn = np.random.default_rng(12).permutation(mesh.num_nodes)
+    coor = np.empty_like(mesh.coor)
+    coor[:, pn] = mesh.coor
+    rm = Mesh(dim=dim, n=0, coor=coor, connect=pn[mesh.connect].astype(np.int32))
+    assert _dof_half_bandwidth(rm, np.arange(rm.num_nodes)) > 2 * _dof_half_bandwidth(mesh, ident)
+    no, eo = band_order(rm)
+    m2 = renumber(rm, no, eo)
+    assert _dof_half_bandwidth(m2, np.arange(m2.num_nodes)) == _dof_half_bandwidth(mesh, ident)

@@ -9,12 +9,14 @@ the classic netCDF family; netCDF-4/HDF5 Exodus files are not read).
 What is kept from the Exodus II layout Goma sees:
   * dimensions ``num_dim``, ``num_nodes``, ``num_elem``, ``num_el_blk``,
     ``num_el_in_blk<b>``, ``num_nod_per_el<b>``, ``num_node_sets``,
-    ``num_nod_ns<k>``, ``time_step`` (unlimited), ``len_name``, ``len_string``,
+    ``num_nod_ns<k>``, ``num_side_sets``, ``num_side_ss<k>``, ``time_step`` (unlimited), ``len_name``, ``len_string``,
     ``len_line``, ``four``;
   * coordinates ``coordx/coordy[/coordz]`` (or the older ``coord[num_dim]``),
     block connectivity ``connect<b>`` (1-based, attribute ``elem_type``
     QUAD9 / HEX27), block and node-set IDs in ``eb_prop1`` / ``ns_prop1``
-    (attribute ``name = "ID"``), node-set members ``node_ns<k>`` (1-based);
+    (attribute ``name = "ID"``), node-set members ``node_ns<k>`` (1-based),
+    side sets ``ss_prop1``, ``elem_ss<k>`` (1-based) and ``side_ss<k>``
+    (Exodus side numbers, ``SIDE_CORNERS``);
   * results: ``time_whole``, nodal variables ``name_nod_var`` /
     ``vals_nod_var<i>`` and element variables ``name_elem_var`` /
     ``vals_elem_var<i>eb<b>`` with ``elem_var_tab``.
@@ -41,6 +43,16 @@ NODES_PER_ELEM = {2: 9, 3: 27}
 NODAL_NAMES = ("VX", "VY", "VZ")
 LEN_NAME = 33  # maximum_name_length 32 + NUL
 
+# Exodus II side numbering: local nodes (0-based, corner nodes only) of each
+# side.  QUAD: side 1 = nodes 1-2 (t = -1), 2 = 2-3 (s = +1), 3 = 3-4
+# (t = +1), 4 = 4-1 (s = -1).  HEX: side 1 = 1,2,6,5 (t = -1), 2 = 2,3,7,6
+# (s = +1), 3 = 3,4,8,7 (t = +1), 4 = 1,5,8,4 (s = -1), 5 = 1,4,3,2
+# (u = -1), 6 = 5,6,7,8 (u = +1).
+SIDE_CORNERS = {
+    2: ((0, 1), (1, 2), (2, 3), (3, 0)),
+    3: ((0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (0, 4, 7, 3), (0, 3, 2, 1), (4, 5, 6, 7)),
+}
+
 
 def wall_node_sets(mesh: Mesh) -> dict:
     """Node sets of the faces of a structured unit square / cube:
@@ -51,6 +63,48 @@ def wall_node_sets(mesh: Mesh) -> dict:
     for k in range(mesh.dim):
         out[2 * k + 1] = np.flatnonzero(lat[:, k] == 0)
         out[2 * k + 2] = np.flatnonzero(lat[:, k] == m)
+    return out
+
+
+def side_nodes(mesh: Mesh, elems, sides) -> np.ndarray:
+    """Nodes of element sides (``sides`` 1-based, Exodus numbering): the
+    side's corner nodes plus every Q2 node whose reference position lies on
+    the same face, as an array [len(elems), 3 (QUAD9) or 9 (HEX27)]."""
+    from .mesh import HEX27_POS, QUAD9_POS
+
+    dim = mesh.dim
+    pos = QUAD9_POS if dim == 2 else HEX27_POS
+    tables = []
+    for corners in SIDE_CORNERS[dim]:
+        c = pos[list(corners)]
+        axis = int(np.flatnonzero(np.all(c == c[0], axis=0))[0])  # the coordinate fixed on this side
+        tables.append(np.flatnonzero(pos[:, axis] == c[0, axis]))
+    tab = np.array(tables)
+    sides = np.asarray(sides, np.int64)
+    if sides.size and (sides.min() < 1 or sides.max() > len(tables)):
+        raise ValueError(f"side numbers run 1..{len(tables)}")
+    return mesh.connect[np.asarray(elems, np.int64)[:, None], tab[sides - 1]]
+
+
+def wall_side_sets(mesh: Mesh) -> dict:
+    """Side sets of the faces of a structured unit square / cube, with the
+    IDs of ``wall_node_sets``: ID -> (elements 0-based, sides 1-based)."""
+    lat = mesh.lattice()
+    m = 2 * mesh.n
+    out = {}
+    dim = mesh.dim
+    for sid in range(1, len(SIDE_CORNERS[dim]) + 1):
+        nodes = side_nodes(mesh, np.arange(mesh.num_elems), np.full(mesh.num_elems, sid))
+        for k in range(dim):
+            for face, val in ((2 * k + 1, 0), (2 * k + 2, m)):
+                on = np.all(lat[nodes, k] == val, axis=1)
+                if on.any():
+                    e = np.flatnonzero(on)
+                    prev = out.get(face, (np.zeros(0, np.int64), np.zeros(0, np.int64)))
+                    out[face] = (np.concatenate([prev[0], e]), np.concatenate([prev[1], np.full(e.size, sid)]))
+    for face, (e, sd) in out.items():
+        order = np.lexsort((sd, e))
+        out[face] = (e[order], sd[order])
     return out
 
 
@@ -86,10 +140,12 @@ def solution_fields(prob: Problem, x: np.ndarray):
 
 
 def write(path, mesh: Mesh, node_sets: dict | None = None, times=None, nodal: dict | None = None,
-          elem: dict | None = None, title: str = "goma_amd", version: int = 2) -> None:
-    """Write a one-block Exodus II file: mesh, node sets and (optionally)
-    results at ``times`` (``nodal[name]`` shaped [len(times), num_nodes],
-    ``elem[name]`` shaped [len(times), num_elem])."""
+          elem: dict | None = None, title: str = "goma_amd", version: int = 2,
+          side_sets: dict | None = None) -> None:
+    """Write a one-block Exodus II file: mesh, node sets, side sets (ID ->
+    (elements 0-based, sides 1-based)) and (optionally) results at ``times``
+    (``nodal[name]`` shaped [len(times), num_nodes], ``elem[name]`` shaped
+    [len(times), num_elem])."""
     dim, ne, nn = mesh.dim, mesh.num_elems, mesh.num_nodes
     if mesh.nodes_per_elem != NODES_PER_ELEM[dim]:
         raise ValueError(f"{dim}D meshes here are {ELEM_TYPE[dim]}")
@@ -129,6 +185,21 @@ def write(path, mesh: Mesh, node_sets: dict | None = None, times=None, nodal: di
                 raise ValueError(f"node set {i} is empty")
             dims[f"num_nod_ns{k}"] = nodes.size
             vs.append(V(f"node_ns{k}", (f"num_nod_ns{k}",), (nodes + 1).astype(np.int32)))
+    if side_sets:
+        ids = sorted(side_sets)
+        dims["num_side_sets"] = len(ids)
+        vs.append(V("ss_status", ("num_side_sets",), np.ones(len(ids), np.int32)))
+        vs.append(V("ss_prop1", ("num_side_sets",), np.asarray(ids, np.int32), {"name": "ID"}))
+        nsides = len(SIDE_CORNERS[dim])
+        for k, i in enumerate(ids, start=1):
+            e, sd = (np.asarray(a, np.int64) for a in side_sets[i])
+            if e.size == 0 or e.shape != sd.shape:
+                raise ValueError(f"side set {i}: needs matching, non-empty element and side lists")
+            if e.min() < 0 or e.max() >= ne or sd.min() < 1 or sd.max() > nsides:
+                raise ValueError(f"side set {i} refers to elements or sides outside the mesh")
+            dims[f"num_side_ss{k}"] = e.size
+            vs.append(V(f"elem_ss{k}", (f"num_side_ss{k}",), (e + 1).astype(np.int32)))
+            vs.append(V(f"side_ss{k}", (f"num_side_ss{k}",), sd.astype(np.int32)))
     if nodal:
         dims["num_nod_var"] = len(nodal)
         vs.append(V("name_nod_var", ("num_nod_var", "len_name"), nc.chars(list(nodal), LEN_NAME)))
@@ -174,7 +245,8 @@ def write_solution(path, prob: Problem, x: np.ndarray, time: float = 0.0,
 
 class ExodusFile:
     """Contents of an Exodus II file: ``mesh`` (0-based connectivity in
-    Goma's local node order), ``node_sets`` (ID -> 0-based nodes), ``times``,
+    Goma's local node order), ``node_sets`` (ID -> 0-based nodes),
+    ``side_sets`` (ID -> (0-based elements, 1-based Exodus sides)), ``times``,
     ``nodal`` (name -> [time_step, num_nodes]) and ``elem`` (name ->
     [time_step, num_elem], blocks concatenated in block order)."""
 
@@ -225,6 +297,16 @@ class ExodusFile:
                 if nodes.size and (nodes.min() < 0 or nodes.max() >= nn):
                     raise ValueError(f"node set {nsid[k - 1]} refers to nodes outside the mesh")
                 self.node_sets[int(nsid[k - 1])] = nodes
+        self.side_sets = {}
+        nss = int(d.get("num_side_sets", 0))
+        ssid = v["ss_prop1"].data if "ss_prop1" in v else np.arange(1, nss + 1)
+        for k in range(1, nss + 1):
+            if f"elem_ss{k}" in v:
+                e = np.asarray(v[f"elem_ss{k}"].data, np.int64) - 1
+                sd = np.asarray(v[f"side_ss{k}"].data, np.int64)
+                if e.size and (e.min() < 0 or e.max() >= connect.shape[0]):
+                    raise ValueError(f"side set {ssid[k - 1]} refers to elements outside the mesh")
+                self.side_sets[int(ssid[k - 1])] = (e, sd)
         self.times = np.asarray(v["time_whole"].data, np.float64) if "time_whole" in v else np.zeros(0)
         nt = self.times.size
         self.nodal = {}

@@ -217,7 +217,7 @@ class _Reader:
             nb = m * _DTYPE[t].itemsize
             raw = self.take(_pad4(nb))[:nb]
             if t == NC_CHAR:
-                out[k] = raw.rstrip(b"\0").decode("utf-8")
+                out[k] = raw.rstrip(b"\0").decode("utf-8", errors="replace")
             else:
                 a = np.frombuffer(raw, dtype=_DTYPE[t]).astype(_DTYPE[t].newbyteorder("="))
                 out[k] = a[0].item() if a.size == 1 else a
@@ -311,4 +311,5 @@ def chars(strings, width: int) -> np.ndarray:
 
 def strings(a: np.ndarray) -> list:
     """Inverse of ``chars``: rows of an NC_CHAR array as Python strings."""
-    return [b"".join(row.tolist()).split(b"\0", 1)[0].decode("utf-8") for row in np.atleast_2d(a)]
+    return [b"".join(row.tolist()).split(b"\0", 1)[0].decode("utf-8", errors="replace")
+            for row in np.atleast_2d(a)]

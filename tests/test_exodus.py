@@ -418,3 +418,49 @@ def test_band_order_formula_and_box_recovery(dim, n, oracle_mod):
     no, eo = band_order(rm)
     m2 = renumber(rm, no, eo)
     assert _dof_half_bandwidth(m2, np.arange(m2.num_nodes)) == _dof_half_bandwidth(mesh, ident)
+
+
+# Node sets of each element side in Goma's get_side_info
+# (src/mm_fill_fill.c:3049-3072 QUAD9, 3182-3250 HEX27), sides 1..
+GOMA_SIDE_NODES = {
+    2: [{0, 1, 4}, {1, 2, 5}, {2, 3, 6}, {3, 0, 7}],
+    3: [{0, 8, 1, 13, 5, 16, 4, 12, 25}, {1, 9, 2, 14, 6, 17, 5, 13, 24},
+        {2, 10, 3, 15, 7, 18, 6, 14, 26}, {0, 12, 4, 19, 7, 15, 3, 11, 23},
+        {3, 10, 2, 9, 1, 8, 0, 11, 21}, {4, 16, 5, 17, 6, 18, 7, 19, 22}],
+}
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_side_numbering_matches_goma(dim):
+    npe = 9 if dim == 2 else 27
+    one = Mesh(dim=dim, n=1, coor=np.zeros((dim, npe)), connect=np.arange(npe, dtype=np.int32)[None])
+    for sid, want in enumerate(GOMA_SIDE_NODES[dim], start=1):
+        assert set(exodus.side_nodes(one, [0], [sid])[0].tolist()) == want, sid
+    with pytest.raises(ValueError, match="side numbers"):
+        exodus.side_nodes(one, [0], [len(GOMA_SIDE_NODES[dim]) + 1])
+
+
+@pytest.mark.parametrize("dim,n", [(2, 3), (3, 2)])
+def test_side_sets_round_trip_and_cover_the_walls(tmp_path, dim, n):
+    mesh = structured_mesh(dim, n, jitter=0.1)
+    ss, ns = exodus.wall_side_sets(mesh), exodus.wall_node_sets(mesh)
+    assert sorted(ss) == sorted(ns)
+    for k, (e, sd) in ss.items():
+        assert e.size == n ** (dim - 1)  # one side per boundary element on each wall
+        nodes = exodus.side_nodes(mesh, e, sd)
+        assert np.array_equal(np.unique(nodes), np.sort(ns[k]))
+    p = tmp_path / "sides.exo"
+    exodus.write(p, mesh, node_sets=ns, side_sets=ss)
+    f = exodus.read(p)
+    assert sorted(f.side_sets) == sorted(ss)
+    for k in ss:
+        assert np.array_equal(f.side_sets[k][0], ss[k][0]) and np.array_equal(f.side_sets[k][1], ss[k][1])
+    g = scipy_io.netcdf_file(p, "r", mmap=False)
+    try:
+        assert g.dimensions["num_side_sets"] == len(ss) and g.variables["ss_prop1"].name == b"ID"
+        assert np.array_equal(g.variables["elem_ss1"][:] - 1, ss[1][0])
+        assert np.array_equal(g.variables["side_ss1"][:], ss[1][1])
+    finally:
+        g.close()
+    with pytest.raises(ValueError, match="outside the mesh"):
+        exodus.write(p, mesh, side_sets={1: (np.array([0]), np.array([7]))})

@@ -114,8 +114,8 @@ def write(path, dims: dict, variables: list, attrs: dict | None = None, version:
                 raise NetCDFError(f"variable {v.name}: unknown dimension {d}")
         if ulim in v.dims[1:]:
             raise NetCDFError(f"variable {v.name}: the unlimited dimension must come first")
-        t = _nc_type(v.data.dtype)
         a = np.asarray(v.data)
+        t = _nc_type(a.dtype)
         is_rec = bool(v.dims) and v.dims[0] == ulim
         shape = tuple(dims[d] for d in v.dims[is_rec:])
         if is_rec:

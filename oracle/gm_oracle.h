@@ -27,7 +27,10 @@
  *   row_sum_scale_MSR         src/sl_matrix_util.c:506-600
  *   L2/L1/Loo norms           src/mm_sol_nonlinear.c:3177-3420
  * GMRES(m)/ILU(0) restate the AztecOO behaviour Goma selects
- * (src/sl_util.c:392-960); AztecOO itself is not in the reference tree.
+ * (src/sl_util.c:392-960); AztecOO itself is not in the reference tree.  They
+ * are pinned by what defines them: each GMRES iterate is the minimal-residual
+ * point of its restart cycle's (right-preconditioned) Krylov space, and the
+ * ILU(0) factors reproduce the matrix on its pattern (test_oracle_gmres.py).
  *
  * The pins of this restatement are the tests in tests/test_oracle_*.py.
  */

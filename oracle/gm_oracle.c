@@ -345,12 +345,15 @@ typedef struct {
 #define DELTA(a, b) ((a) == (b) ? 1.0 : 0.0)
 
 /* Assembles one element into lec.  Returns 0, or -1 on detJ <= 0.
- * Follows matrix_fill's quadrature loop (src/mm_fill.c:1253-2791). */
-static int elem_fill(const om_problem *p, int e, const double *x, lec_t *L) {
+ * Follows matrix_fill's quadrature loop (src/mm_fill.c:1253-2791).  With
+ * xdot != NULL the transient mass terms are added: xdot is the time
+ * derivative of x and cmass = d xdot / d x = (1 + 2 theta) / dt. */
+static int elem_fill(const om_problem *p, int e, const double *x, const double *xdot,
+                     double cmass, lec_t *L) {
   const int d = p->dim, nve = p->nve, np = d + 1, en = p->ph.energy ? 1 : 0;
   const om_physics *ph = &p->ph;
   const int nq = (d == 2) ? 9 : 27;
-  double xc[3][MDE], vel[3][MDE], Tn[MDE], Pn[4];
+  double xc[3][MDE], vel[3][MDE], Tn[MDE], Pn[4], veld[3][MDE], Tdn[MDE];
   memset(L->ndof, 0, sizeof(L->ndof));
   for (int a = 0; a < d; a++) L->ndof[a] = nve;
   if (en) L->ndof[SLOT_T] = nve;
@@ -365,6 +368,8 @@ static int elem_fill(const om_problem *p, int e, const double *x, lec_t *L) {
     }
     Tn[i] = en ? x[f + d] : 0.0;
     if (en) L->gun[SLOT_T][i] = f + d;
+    for (int a = 0; a < d; a++) veld[a][i] = xdot ? xdot[f + a] : 0.0;
+    Tdn[i] = (xdot && en) ? xdot[f + d] : 0.0;
   }
   {
     int c = p->elem_centroid[e];
@@ -449,6 +454,12 @@ static int elem_fill(const om_problem *p, int e, const double *x, lec_t *L) {
       }
     }
     for (int k = 0; k < np; k++) Pq += Pn[k] * psi[k];
+    double v_dot[3] = {0, 0, 0}, T_dot = 0.0;
+    if (xdot)
+      for (int i = 0; i < nve; i++) {
+        for (int a = 0; a < d; a++) v_dot[a] += veld[a][i] * phi[i];
+        T_dot += Tdn[i] * phi[i];
+      }
     double div_v = 0.0;
     for (int pp = 0; pp < d; pp++) div_v += grad_v[pp][pp];
     const double d_area = detJ * wt; /* h3 = 1 (Cartesian), src/mm_fill_momentum.c:282 */
@@ -521,6 +532,12 @@ static int elem_fill(const om_problem *p, int e, const double *x, lec_t *L) {
         diffusion *= -d_area;
         double source = f[a] * phi_i * d_area;
         L->R[a][i] += advection + diffusion + source;
+        /* transient mass (src/mm_fill_momentum.c:534-539, J 1586-1590):
+         * R -= rho v_dot_a phi_i dA;  J_m_v[a][a] -= rho c phi_j phi_i dA */
+        if (xdot) {
+          L->R[a][i] += -ph->rho * v_dot[a] * phi_i * d_area;
+          for (int j = 0; j < nve; j++) L->J[a][a][i][j] += -ph->rho * cmass * phi[j] * phi_i * d_area;
+        }
 
         /* J_m_T (src/mm_fill_momentum.c:746-915): Boussinesq source */
         if (en)
@@ -588,6 +605,13 @@ static int elem_fill(const om_problem *p, int e, const double *x, lec_t *L) {
         diffusion *= detJ * wt;
         double source = phi_i * ph->hsrc * detJ * wt;
         L->R[SLOT_T][i] += advection + diffusion + source;
+        /* transient mass (src/mm_fill_energy.c:322-328, J 435-441), constant
+         * rho and Cp: R -= rho Cp T_dot phi_i dA;  J_e_T -= rho Cp c phi_j phi_i dA */
+        if (xdot) {
+          L->R[SLOT_T][i] += -rCp * T_dot * phi_i * detJ * wt;
+          for (int j = 0; j < nve; j++)
+            L->J[SLOT_T][SLOT_T][i][j] += -rCp * cmass * phi[j] * phi_i * detJ * wt;
+        }
         /* J_e_T (src/mm_fill_energy.c:425-487) */
         for (int j = 0; j < nve; j++) {
           double adv = 0.0, dif = 0.0;
@@ -653,12 +677,12 @@ static int load_lec(const om_problem *p, const lec_t *L, double *resid, double *
   return 0;
 }
 
-int om_fill_range(const om_problem *p, const double *x, double *resid, double *val, int e0,
-                  int e1) {
+static int fill_range(const om_problem *p, const double *x, const double *xdot, double cmass,
+                      double *resid, double *val, int e0, int e1) {
   lec_t *L = (lec_t *)malloc(sizeof(lec_t));
   int rc = 0;
   for (int e = e0; e < e1; e++) {
-    if (elem_fill(p, e, x, L) < 0) {
+    if (elem_fill(p, e, x, xdot, cmass, L) < 0) {
       rc = -1; /* negative / zero element volume: matrix_fill_full returns -1 */
       break;
     }
@@ -671,13 +695,23 @@ int om_fill_range(const om_problem *p, const double *x, double *resid, double *v
   return rc;
 }
 
+int om_fill_range(const om_problem *p, const double *x, double *resid, double *val, int e0,
+                  int e1) {
+  return fill_range(p, x, NULL, 0.0, resid, val, e0, e1);
+}
+
 int om_fill(const om_problem *p, const double *x, double *resid, double *val) {
   return om_fill_range(p, x, resid, val, 0, p->num_elems);
 }
 
+int om_fill_transient(const om_problem *p, const double *x, const double *xdot, double cmass,
+                      double *resid, double *val) {
+  return fill_range(p, x, xdot, cmass, resid, val, 0, p->num_elems);
+}
+
 int om_element(const om_problem *p, int e, const double *x, double *R, double *J, int *nloc) {
   lec_t *L = (lec_t *)malloc(sizeof(lec_t));
-  int rc = elem_fill(p, e, x, L);
+  int rc = elem_fill(p, e, x, NULL, 0.0, L);
   int off[NSLOT], n = 0;
   for (int A = 0; A < NSLOT; A++) {
     off[A] = n;

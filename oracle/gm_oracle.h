@@ -25,6 +25,8 @@
  *   find_MSR_problem_graph    src/mm_fill_util.c:3229-3445
  *   build_node_node           src/exo_conn.c:204-377
  *   row_sum_scale_MSR         src/sl_matrix_util.c:506-600
+ *   transient mass terms      src/mm_fill_momentum.c:534-539, 1586-1590;
+ *                             src/mm_fill_energy.c:322-328, 435-441
  *   L2/L1/Loo norms           src/mm_sol_nonlinear.c:3177-3420
  * GMRES(m)/ILU(0) restate the AztecOO behaviour Goma selects
  * (src/sl_util.c:392-960); AztecOO itself is not in the reference tree.  They
@@ -124,6 +126,13 @@ void om_copy_dbc(const om_problem *p, char *mask_out, double *val_out);
 int om_fill(const om_problem *p, const double *x, double *resid, double *val);
 /* Same, restricted to elements [e0, e1) (used for bounded CPU timing samples). */
 int om_fill_range(const om_problem *p, const double *x, double *resid, double *val, int e0, int e1);
+/* matrix_fill_full of a transient step (theta method): as om_fill, plus the
+ * mass terms of momentum and energy (src/mm_fill_momentum.c:534-539,
+ * 1586-1590; src/mm_fill_energy.c:322-328, 435-441).  xdot is the time
+ * derivative at x, cmass = (1 + 2 theta) / dt its derivative in x; the caller
+ * forms xdot = cmass (x - x_old) - 2 theta xdot_old (src/rf_solve.c:2848). */
+int om_fill_transient(const om_problem *p, const double *x, const double *xdot, double cmass,
+                      double *resid, double *val);
 /* One element's dense local residual and Jacobian in the oracle's local order:
  * local dofs = [var slot][dof], var slots U1,U2,(U3),(T),P.  R[nloc], J[nloc*nloc]. */
 int om_element(const om_problem *p, int elem, const double *x, double *R, double *J, int *nloc);

@@ -86,6 +86,7 @@ def lib():
         L.om_copy_dbc.argtypes = [C.c_void_p, _cp, _dp]
         L.om_fill.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.om_fill_range.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_int]
+        L.om_fill_transient.argtypes = [C.c_void_p, _dp, _dp, C.c_double, _dp, _dp]
         L.om_element.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_int)]
         L.om_row_sum_scale.argtypes = [C.c_int, _dp, _ip, _dp, _dp]
         for nm in ("om_norm_L2", "om_norm_L1", "om_norm_Loo"):
@@ -159,6 +160,16 @@ class Oracle:
         v = np.zeros(self.nval)
         e1 = self.prob.mesh.num_elems if e1 is None else e1
         rc = lib().om_fill_range(self.h, x, r, v, e0, e1)
+        return rc, r, v
+
+    def fill_transient(self, x, xdot, cmass):
+        """matrix_fill_full of a theta-method step: the steady terms plus the
+        mass terms for time derivative xdot (d xdot / d x = cmass)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        xd = np.ascontiguousarray(xdot, dtype=np.float64)
+        r = np.zeros(self.N)
+        v = np.zeros(self.nval)
+        rc = lib().om_fill_transient(self.h, x, xd, float(cmass), r, v)
         return rc, r, v
 
     def element(self, e, x):

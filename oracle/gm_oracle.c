@@ -34,6 +34,9 @@ struct om_problem {
   double *dbc_val;
   int *nn_ptr, *nn_list; /* node-node graph, sorted, self included */
   int *ija;              /* MSR */
+  int n_qs;              /* QSIDE heat-flux sides: element, side (1-based), flux */
+  int *qs_elem, *qs_side;
+  double *qs_q;
 };
 
 static double now_s(void) {
@@ -316,8 +319,34 @@ void om_destroy(om_problem *p) {
   free(p->nn_ptr);
   free(p->nn_list);
   free(p->ija);
+  free(p->qs_elem);
+  free(p->qs_side);
+  free(p->qs_q);
   free(p);
 }
+
+int om_add_qside(om_problem *p, int n, const int *elems, const int *sides, double q) {
+  const int nsides = p->dim == 2 ? 4 : 6;
+  if (!p->ph.energy || n < 0) return -1;
+  for (int k = 0; k < n; k++)
+    if (elems[k] < 0 || elems[k] >= p->num_elems || sides[k] < 1 || sides[k] > nsides) return -1;
+  int m = p->n_qs + n;
+  int *qe = (int *)realloc(p->qs_elem, (size_t)(m > 0 ? m : 1) * sizeof(int));
+  if (qe) p->qs_elem = qe;
+  int *qd = (int *)realloc(p->qs_side, (size_t)(m > 0 ? m : 1) * sizeof(int));
+  if (qd) p->qs_side = qd;
+  double *qv = (double *)realloc(p->qs_q, (size_t)(m > 0 ? m : 1) * sizeof(double));
+  if (qv) p->qs_q = qv;
+  if (!qe || !qd || !qv) return -1;
+  for (int k = 0; k < n; k++) {
+    p->qs_elem[p->n_qs + k] = elems[k];
+    p->qs_side[p->n_qs + k] = sides[k];
+    p->qs_q[p->n_qs + k] = q;
+  }
+  p->n_qs = m;
+  return 0;
+}
+
 
 int om_num_unknowns(const om_problem *p) { return p->N; }
 int om_ija_size(const om_problem *p) { return p->ija[p->N]; }
@@ -343,6 +372,54 @@ typedef struct {
 } lec_t;
 
 #define DELTA(a, b) ((a) == (b) ? 1.0 : 0.0)
+
+/* QSIDE (src/bc_integ.c:1518-1519 func = q; 2156-2157, 2207: R[T][i] +=
+ * w phi_i sdet q over the side).  Exodus side numbering, as Goma's
+ * get_side_info (src/mm_fill_fill.c:3049-3072, 3182-3250): the reference
+ * coordinate fixed on each side (axis 0 = s, 1 = t, 2 = u) and its value. */
+static void qside_elem(const om_problem *p, int e, double xc[3][MDE], lec_t *L) {
+  static const int side_axis[6] = {1, 0, 1, 0, 2, 2};
+  static const double side_val[6] = {-1.0, 1.0, 1.0, -1.0, -1.0, 1.0};
+  const double gp[3] = {-0.7745966692414834, 0.0, 0.7745966692414834};
+  const double gw[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+  const int d = p->dim, nve = p->nve;
+  for (int k = 0; k < p->n_qs; k++) {
+    if (p->qs_elem[k] != e) continue;
+    const int sd = p->qs_side[k] - 1, ax = side_axis[sd];
+    int fr[2], nf = 0;
+    for (int a = 0; a < d; a++)
+      if (a != ax) fr[nf++] = a;
+    const int n1 = 3, n2 = (d == 3) ? 3 : 1;
+    for (int g1 = 0; g1 < n1; g1++)
+      for (int g2 = 0; g2 < n2; g2++) {
+        double ref[3] = {0, 0, 0}, phi[MDE], dphi[MDE][3];
+        ref[ax] = side_val[sd];
+        ref[fr[0]] = gp[g1];
+        double w = gw[g1];
+        if (d == 3) {
+          ref[fr[1]] = gp[g2];
+          w *= gw[g2];
+        }
+        if (d == 2) shape_quad9(ref[0], ref[1], phi, dphi);
+        else shape_hex27(ref[0], ref[1], ref[2], phi, dphi);
+        double t1[3] = {0, 0, 0}, t2[3] = {0, 0, 0};
+        for (int j = 0; j < d; j++)
+          for (int i = 0; i < nve; i++) {
+            t1[j] += xc[j][i] * dphi[i][fr[0]];
+            if (d == 3) t2[j] += xc[j][i] * dphi[i][fr[1]];
+          }
+        double sdet;
+        if (d == 2) {
+          sdet = sqrt(t1[0] * t1[0] + t1[1] * t1[1]);
+        } else {
+          double c0 = t1[1] * t2[2] - t1[2] * t2[1], c1 = t1[2] * t2[0] - t1[0] * t2[2],
+                 c2 = t1[0] * t2[1] - t1[1] * t2[0];
+          sdet = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+        }
+        for (int i = 0; i < nve; i++) L->R[SLOT_T][i] += w * phi[i] * sdet * p->qs_q[k];
+      }
+  }
+}
 
 /* Assembles one element into lec.  Returns 0, or -1 on detJ <= 0.
  * Follows matrix_fill's quadrature loop (src/mm_fill.c:1253-2791).  With
@@ -631,6 +708,8 @@ static int elem_fill(const om_problem *p, int e, const double *x, const double *
       }
     }
   }
+
+  if (p->n_qs && en) qside_elem(p, e, xc, L);
 
   /* put_dirichlet_in_matrix (src/bc_dirich.c:44-153): zero the local row,
    * J_ii = DIRICHLET_PENALTY (1, include/rf_bc_const.h:152), R = x - V. */

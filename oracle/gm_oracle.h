@@ -25,6 +25,7 @@
  *   find_MSR_problem_graph    src/mm_fill_util.c:3229-3445
  *   build_node_node           src/exo_conn.c:204-377
  *   row_sum_scale_MSR         src/sl_matrix_util.c:506-600
+ *   QSIDE weak heat flux      src/bc_integ.c:1518-1519, 2156-2157, 2207
  *   transient mass terms      src/mm_fill_momentum.c:534-539, 1586-1590;
  *                             src/mm_fill_energy.c:322-328, 435-441
  *   L2/L1/Loo norms           src/mm_sol_nonlinear.c:3177-3420
@@ -133,6 +134,12 @@ int om_fill_range(const om_problem *p, const double *x, double *resid, double *v
  * forms xdot = cmass (x - x_old) - 2 theta xdot_old (src/rf_solve.c:2848). */
 int om_fill_transient(const om_problem *p, const double *x, const double *xdot, double cmass,
                       double *resid, double *val);
+/* Weak heat-flux condition QSIDE (src/bc_integ.c:1518-1519, 2207) on element
+ * sides (Exodus numbering, 1-based): the energy residual gains the integral
+ * of phi_i q over each side, q being the heat flux into the domain.  Calls
+ * append.  Returns 0, or -1 without the energy equation or for an element or
+ * side out of range. */
+int om_add_qside(om_problem *p, int n, const int *elems, const int *sides, double q);
 /* One element's dense local residual and Jacobian in the oracle's local order:
  * local dofs = [var slot][dof], var slots U1,U2,(U3),(T),P.  R[nloc], J[nloc*nloc]. */
 int om_element(const om_problem *p, int elem, const double *x, double *R, double *J, int *nloc);

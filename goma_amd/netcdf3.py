@@ -244,6 +244,8 @@ def read(path) -> Dataset:
         for _ in range(n):
             dnames.append(r.name())
             dlens.append(r.i32())
+            if dlens[-1] < 0:
+                raise NetCDFError(f"dimension {dnames[-1]}: negative length")
     unlimited = None
     for k, ln in zip(dnames, dlens):
         if ln == 0:
@@ -263,6 +265,10 @@ def read(path) -> Dataset:
             begin = struct.unpack(">i" if version == 1 else ">q", r.take(4 if version == 1 else 8))[0]
             if t not in _DTYPE:
                 raise NetCDFError(f"variable {name}: bad type {t}")
+            if any(i < 0 or i >= len(dnames) for i in ids):
+                raise NetCDFError(f"variable {name}: dimension id out of range")
+            if begin < 0:
+                raise NetCDFError(f"variable {name}: negative data offset")
             raw_vars.append((name, [dnames[i] for i in ids], vatts, t, vsize, begin))
     nrec_vars = sum(1 for v in raw_vars if v[1] and v[1][0] == unlimited)
     recsize = 0

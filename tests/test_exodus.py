@@ -108,6 +108,30 @@ def test_netcdf_rejects_bad_input(tmp_path):
     (tmp_path / "cut2.nc").write_bytes(raw[:20])
     with pytest.raises(nc.NetCDFError, match="truncated"):
         nc.read(tmp_path / "cut2.nc")
+    # corrupt headers: a negative dimension id, data offset or dimension length
+    # must not wrap around in Python indexing
+    import struct
+    tiny = tmp_path / "tiny.nc"
+    nc.write(tiny, {"n": 2}, [nc.Var("x", ("n",), np.arange(2.0))])
+    hdr = bytearray(tiny.read_bytes())
+    pos_dimid = hdr.index(b"x\0\0\0") + 8  # name "x" padded, then ndims, then the dim id
+    bad = bytearray(hdr)
+    bad[pos_dimid:pos_dimid + 4] = struct.pack(">i", -1)
+    (tmp_path / "dimid.nc").write_bytes(bytes(bad))
+    with pytest.raises(nc.NetCDFError, match="dimension id"):
+        nc.read(tmp_path / "dimid.nc")
+    bad = bytearray(hdr)
+    pos_begin = pos_dimid + 4 + 8 + 8  # empty attribute list, type, vsize
+    bad[pos_begin:pos_begin + 8] = struct.pack(">q", -8)
+    (tmp_path / "begin.nc").write_bytes(bytes(bad))
+    with pytest.raises(nc.NetCDFError, match="negative data offset"):
+        nc.read(tmp_path / "begin.nc")
+    bad = bytearray(hdr)
+    pos_len = hdr.index(b"n\0\0\0") + 4
+    bad[pos_len:pos_len + 4] = struct.pack(">i", -3)
+    (tmp_path / "len.nc").write_bytes(bytes(bad))
+    with pytest.raises(nc.NetCDFError, match="negative length"):
+        nc.read(tmp_path / "len.nc")
     with pytest.raises(nc.NetCDFError, match="does not fit"):
         nc.write(p, {"n": 3}, [nc.Var("x", ("n",), np.zeros(4))])
     with pytest.raises(nc.NetCDFError, match="unknown dimension"):

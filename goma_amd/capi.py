@@ -348,6 +348,10 @@ class Context:
         side and attaches them with their Q2/P1 prolongations."""
         from . import multigrid
 
+        if self.prob.mesh.n < 1:
+            raise ValueError("geometric multigrid needs the structured mesh hierarchy; this mesh "
+                             "has none (read from a file or renumbered, mesh.n == 0): use "
+                             "ILU(0) or the banded LU")
         self._coarse = []
         above = self
         for cp in multigrid.hierarchy(self.prob, min_elems_per_side):

@@ -488,3 +488,18 @@ def test_side_sets_round_trip_and_cover_the_walls(tmp_path, dim, n):
         g.close()
     with pytest.raises(ValueError, match="outside the mesh"):
         exodus.write(p, mesh, side_sets={1: (np.array([0]), np.array([7]))})
+
+
+def test_multigrid_is_refused_on_a_mesh_without_hierarchy(tmp_path):
+    """A mesh read from Exodus has no structured hierarchy: attaching the
+    geometric multigrid raises before any library call."""
+    from types import SimpleNamespace
+
+    from goma_amd import capi
+
+    prob = problems.natural_convection(n=2, Ra=1e4)
+    p = tmp_path / "m.exo"
+    exodus.write(p, prob.mesh, node_sets=exodus.wall_node_sets(prob.mesh))
+    rp = exodus.problem_from_exodus(p, prob.phys, [(1, TEMP, 0.5)])
+    with pytest.raises(ValueError, match="ILU"):
+        capi.Context.attach_multigrid(SimpleNamespace(prob=rp))

@@ -243,6 +243,39 @@ def write_solution(path, prob: Problem, x: np.ndarray, time: float = 0.0,
           title=title or prob.name)
 
 
+def append_solution(path, prob: Problem, x: np.ndarray, time: float) -> int:
+    """One more time step of Goma's output fields for solution ``x``,
+    appended to a file ``write_solution`` made for the same problem (Goma
+    writes each output step into the same results file, wr_soln.c:40-131).
+    Returns the number of time steps now in the file."""
+    nodal, elem = solution_fields(prob, x)
+    if "exodus_node_order" in prob.meta:
+        no, eo = prob.meta["exodus_node_order"], prob.meta["exodus_elem_order"]
+        for k, v in nodal.items():
+            nodal[k] = np.empty_like(v)
+            nodal[k][no] = v
+        for k, v in elem.items():
+            elem[k] = np.empty_like(v)
+            elem[k][eo] = v
+    ds = nc.read(path, only={"name_nod_var", "name_elem_var", "time_whole"})
+    v = ds.variables
+    times = v["time_whole"].data if "time_whole" in v else np.zeros(0)
+    if times.size and time <= times[-1]:
+        raise ValueError(f"time {time} does not follow the file's last time {times[-1]}")
+    rec = {"time_whole": np.array(float(time))}
+    names = nc.strings(v["name_nod_var"].data) if "name_nod_var" in v else []
+    if sorted(names) != sorted(nodal):
+        raise ValueError(f"the file's nodal variables {names} are not this problem's {list(nodal)}")
+    for i, name in enumerate(names, start=1):
+        rec[f"vals_nod_var{i}"] = nodal[name]
+    names = nc.strings(v["name_elem_var"].data) if "name_elem_var" in v else []
+    if sorted(names) != sorted(elem):
+        raise ValueError(f"the file's element variables {names} are not this problem's {list(elem)}")
+    for i, name in enumerate(names, start=1):
+        rec[f"vals_elem_var{i}eb1"] = elem[name]
+    return nc.append_record(path, rec)
+
+
 class ExodusFile:
     """Contents of an Exodus II file: ``mesh`` (0-based connectivity in
     Goma's local node order), ``node_sets`` (ID -> 0-based nodes),

@@ -224,10 +224,10 @@ class _Reader:
         return out
 
 
-def read(path) -> Dataset:
-    """Read a netCDF classic or 64-bit-offset file completely."""
-    with open(path, "rb") as f:
-        buf = memoryview(f.read())
+def _parse_header(buf):
+    """Header of a classic / 64-bit-offset file: (version, recs, dnames,
+    dlens, unlimited, global attributes, [(name, dims, attrs, type, vsize,
+    begin)], recsize)."""
     r = _Reader(buf)
     magic = r.take(4)
     if magic[:3] != b"CDF" or magic[3] not in (1, 2):
@@ -276,11 +276,22 @@ def read(path) -> Dataset:
         if vd and vd[0] == unlimited:
             slab = int(np.prod([dlens[dnames.index(d)] for d in vd[1:]], dtype=np.int64)) * _DTYPE[t].itemsize
             recsize += slab if nrec_vars == 1 else _pad4(slab)
+    return version, recs, dnames, dlens, unlimited, gatts, raw_vars, recsize
+
+
+def read(path, only=None) -> Dataset:
+    """Read a netCDF classic or 64-bit-offset file: every variable, or those
+    named in ``only``."""
+    with open(path, "rb") as f:
+        buf = memoryview(f.read())
+    version, recs, dnames, dlens, unlimited, gatts, raw_vars, recsize = _parse_header(buf)
     dims = dict(zip(dnames, dlens))
     if unlimited is not None:
         dims[unlimited] = recs
     out = {}
     for name, vd, vatts, t, vsize, begin in raw_vars:
+        if only is not None and name not in only:
+            continue
         dt = _DTYPE[t]
         is_rec = bool(vd) and vd[0] == unlimited
         shape = [dims[d] for d in vd[is_rec:]]
@@ -304,6 +315,45 @@ def read(path) -> Dataset:
             a = a.copy()
         out[name] = Var(name, tuple(vd), a, vatts)
     return Dataset(dims=dims, unlimited=unlimited, attrs=gatts, variables=out)
+
+
+def append_record(path, values: dict) -> int:
+    """Append one record to an existing file: ``values`` maps every record
+    variable to its data for the new record (shape = the variable's shape
+    without the record axis).  The record is written after the last one and
+    the header's record count updated in place.  Returns the new count."""
+    with open(path, "rb") as f:
+        buf = memoryview(f.read())
+    version, recs, dnames, dlens, unlimited, gatts, raw_vars, recsize = _parse_header(buf)
+    size = len(buf)
+    buf.release()
+    rvars = [v for v in raw_vars if v[1] and v[1][0] == unlimited]
+    if not rvars:
+        raise NetCDFError("the file has no record variables")
+    names = {v[0] for v in rvars}
+    if set(values) != names:
+        raise NetCDFError(f"a record needs exactly the record variables {sorted(names)}")
+    rec_start = min(v[5] for v in rvars)
+    if size < rec_start + recs * recsize:
+        raise NetCDFError("the file is shorter than its records")
+    pos0 = rec_start + recs * recsize
+    single = len(rvars) == 1
+    slabs = []  # every shape is checked before anything is written
+    for name, vd, vatts, t, vsize, begin in rvars:
+        shape = tuple(dlens[dnames.index(d)] for d in vd[1:])
+        a = np.array(values[name], dtype=_DTYPE[t], order="C")  # keeps 0-d for scalar records
+        if a.shape != shape:
+            raise NetCDFError(f"variable {name}: shape {a.shape}, the record needs {shape}")
+        raw = a.tobytes()
+        slabs.append((pos0 + (begin - rec_start), raw + (b"" if single else b"\0" * (_pad4(len(raw)) - len(raw)))))
+    with open(path, "r+b") as f:
+        for off, raw in slabs:
+            f.seek(off)
+            f.write(raw)
+        f.truncate(pos0 + recsize)
+        f.seek(4)
+        f.write(struct.pack(">i", recs + 1))
+    return recs + 1
 
 
 def chars(strings, width: int) -> np.ndarray:

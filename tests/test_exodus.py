@@ -503,3 +503,54 @@ def test_multigrid_is_refused_on_a_mesh_without_hierarchy(tmp_path):
     rp = exodus.problem_from_exodus(p, prob.phys, [(1, TEMP, 0.5)])
     with pytest.raises(ValueError, match="ILU"):
         capi.Context.attach_multigrid(SimpleNamespace(prob=rp))
+
+
+def test_netcdf_append_record_reads_in_scipy(tmp_path):
+    p = tmp_path / "rec.nc"
+    rng = np.random.default_rng(4)
+    a = rng.standard_normal((2, 5))
+    nc.write(p, {"t": None, "m": 5}, [nc.Var("time", ("t",), np.array([0.0, 1.0])),
+                                      nc.Var("a", ("t", "m"), a),
+                                      nc.Var("fixed", ("m",), np.arange(5, dtype=np.int32))])
+    b = rng.standard_normal(5)
+    assert nc.append_record(p, {"time": 2.0, "a": b}) == 3
+    with pytest.raises(nc.NetCDFError, match="exactly the record variables"):
+        nc.append_record(p, {"a": b})
+    with pytest.raises(nc.NetCDFError, match="shape"):
+        nc.append_record(p, {"time": 3.0, "a": np.zeros(4)})
+    f = scipy_io.netcdf_file(p, "r", mmap=False)
+    try:
+        assert np.array_equal(f.variables["time"][:], [0.0, 1.0, 2.0])
+        assert np.array_equal(f.variables["a"][:], np.vstack([a, b]))
+        assert np.array_equal(f.variables["fixed"][:], np.arange(5))
+    finally:
+        f.close()
+    ds = nc.read(p)
+    assert ds.dims["t"] == 3 and np.array_equal(ds.variables["a"].data, np.vstack([a, b]))
+
+
+@pytest.mark.parametrize("renumber", [False, True])
+def test_exodus_time_series_append(tmp_path, renumber, oracle_mod):
+    """Goma writes each output step into one results file: steps appended to
+    a solution file read back in order, in the file's numbering."""
+    prob, p, bcs, datum, pn, pe = _renumbered_file(tmp_path, 2)
+    rp = exodus.problem_from_exodus(p, prob.phys, bcs, datum_elem=datum, renumber=renumber)
+    orc = oracle_mod.Oracle(rp)
+    rng = np.random.default_rng(6)
+    xs = [rng.standard_normal(orc.N) for _ in range(3)]
+    out = tmp_path / "series.exo"
+    exodus.write_solution(out, rp, xs[0], time=0.0)
+    assert exodus.append_solution(out, rp, xs[1], 0.5) == 2
+    assert exodus.append_solution(out, rp, xs[2], 1.0) == 3
+    with pytest.raises(ValueError, match="does not follow"):
+        exodus.append_solution(out, rp, xs[2], 1.0)
+    f = exodus.read(out)
+    assert np.array_equal(f.times, [0.0, 0.5, 1.0])
+    for k, x in enumerate(xs):
+        one = tmp_path / f"one{k}.exo"
+        exodus.write_solution(one, rp, x, time=float(k))
+        g = exodus.read(one)
+        for name in g.nodal:
+            assert np.array_equal(f.nodal[name][k], g.nodal[name][0])
+        for name in g.elem:
+            assert np.array_equal(f.elem[name][k], g.elem[name][0])

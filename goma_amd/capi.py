@@ -113,7 +113,8 @@ EXPORTS = [
     "gm_ctx_create", "gm_ctx_destroy", "gm_last_error", "gm_num_unknowns", "gm_msr_size",
     "gm_copy_msr_graph", "gm_copy_first_unknown", "gm_copy_dirichlet", "gm_num_elem_colours",
     "gm_copy_elem_colours", "gm_copy_patch_permutation", "gm_set_physics",
-    "gm_matrix_fill_full_dev", "gm_matrix_fill_full", "gm_row_sum_scale_dev", "gm_msr_matvec_dev",
+    "gm_matrix_fill_full_dev", "gm_matrix_fill_full", "gm_matrix_fill_full32_dev",
+    "gm_row_sum_scale_dev", "gm_msr_matvec_dev",
     "gm_gmres_dev", "gm_ilu0_factor_dev", "gm_solve_nonlinear_dev", "gm_solve_nonlinear",
     "gm_default_solver_opts", "gm_prof_enable", "gm_prof_get", "gm_mg_add_level",
     "gm_mg_num_levels", "gm_mg_setup_dev", "gm_mg_copy_level_matrix", "gm_solve_nonlinear_guess_dev",
@@ -149,6 +150,7 @@ def lib():
     L.gm_set_physics.argtypes = [vp, C.POINTER(GmPhysics)]
     L.gm_matrix_fill_full_dev.argtypes = [vp, vp, vp, vp]
     L.gm_matrix_fill_full.argtypes = [vp, vp, vp, vp]
+    L.gm_matrix_fill_full32_dev.argtypes = [vp, vp, vp, vp]
     L.gm_row_sum_scale_dev.argtypes = [vp, vp, vp, vp]
     L.gm_msr_matvec_dev.argtypes = [vp, vp, vp, vp]
     L.gm_gmres_dev.argtypes = [vp, vp, vp, vp, C.POINTER(GmSolverOpts), C.POINTER(ip),
@@ -289,6 +291,10 @@ class Context:
     def fill_dev(self, x_ptr: int, resid_ptr: int, val_ptr: int) -> int:
         return lib().gm_matrix_fill_full_dev(self.h, x_ptr, resid_ptr, val_ptr)
 
+    def fill32_dev(self, x_ptr: int, resid_ptr: int, val32_ptr: int) -> int:
+        """The assembly with FP32 Jacobian values (mixed precision)."""
+        return lib().gm_matrix_fill_full32_dev(self.h, x_ptr, resid_ptr, val32_ptr)
+
     def fill_host(self, x: np.ndarray, resid: np.ndarray, val: np.ndarray) -> int:
         for a in (x, resid, val):
             assert a.dtype == np.float64 and a.flags.c_contiguous
@@ -352,6 +358,10 @@ class Context:
             raise ValueError("geometric multigrid needs the structured mesh hierarchy; this mesh "
                              "has none (read from a file or renumbered, mesh.n == 0): use "
                              "ILU(0) or the banded LU")
+        if getattr(self, "_coarse", None):
+            # the library keeps pointers to the attached levels: dropping them here
+            # would leave it with freed contexts
+            raise RuntimeError("a multigrid hierarchy is already attached to this context")
         self._coarse = []
         above = self
         for cp in multigrid.hierarchy(self.prob, min_elems_per_side):

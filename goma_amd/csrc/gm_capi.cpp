@@ -272,6 +272,11 @@ extern "C" int gm_matrix_fill_full_dev(gm_ctx *c, const double *x, double *resid
   return gm_fill_launch(c, x, resid, val, nullptr);
 }
 
+extern "C" int gm_matrix_fill_full32_dev(gm_ctx *c, const double *x, double *resid, float *val32) {
+  if (!c || !x || !resid || !val32) return GM_ERR_ARG;
+  return gm_fill_launch(c, x, resid, nullptr, val32);
+}
+
 extern "C" int gm_matrix_fill_full(gm_ctx *c, const double *x, double *resid, double *val) {
   if (!c || !x || !resid || !val) return GM_ERR_ARG;
   double *dx = nullptr, *dr = nullptr, *dv = nullptr;

@@ -171,6 +171,11 @@ int gm_set_physics(gm_ctx *ctx, const gm_physics *phys);
  * doubles).  Returns GM_OK or GM_ERR_ELEM. */
 int gm_matrix_fill_full_dev(gm_ctx *ctx, const double *x_dev, double *resid_dev, double *val_dev);
 int gm_matrix_fill_full(gm_ctx *ctx, const double *x, double *resid, double *val);
+/* The same assembly with the Jacobian values in FP32 (the mixed-precision
+ * path, GM_PREC_MIXED): val32 holds gm_msr_size floats and is accumulated
+ * (+=); the residual stays FP64.  Replaces matrix_fill_full
+ * (src/mm_fill.c:158-312) where the Jacobian is stored in single precision. */
+int gm_matrix_fill_full32_dev(gm_ctx *ctx, const double *x_dev, double *resid_dev, float *val32_dev);
 
 /* row_sum_scale_MSR (src/sl_matrix_util.c:506): scales val and b in place. */
 int gm_row_sum_scale_dev(gm_ctx *ctx, double *val_dev, double *b_dev, double *scale_dev);

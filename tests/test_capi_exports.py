@@ -31,3 +31,19 @@ def test_default_options_are_gomas():
     # additions without a Goma counterpart are off by default: Goma's fixed linear
     # tolerance, re-assembled coarse operators, recalc preconditioners
     assert o.lin_tol_adapt == 0 and o.mg_galerkin == 0 and o.pc_reuse == 0
+
+
+def test_second_multigrid_attach_is_refused():
+    """The library keeps pointers to the attached coarse levels, so a second
+    attach (which would drop and free them) is refused before any library call."""
+    import pytest
+
+    from goma_amd import problems
+
+    ctx = capi.Context.__new__(capi.Context)
+    ctx.h = None
+    ctx.prob = problems.lid_cavity(n=4, Re=1.0)
+    ctx._coarse = [object()]
+    with pytest.raises(RuntimeError, match="already attached"):
+        ctx.attach_multigrid(2)
+    ctx._coarse = []

@@ -1,0 +1,156 @@
+"""Config-3 physics on the GPU: 3D lid-driven cavity with energy on HEX27
+Q2/P1 elements, Re = 400, Ra = 1e4, Pr = 0.71.
+
+* n = 4 (3,172 unknowns): the converged GPU solution against the oracle's
+  FP64 fixed point, reached by a full Newton iteration with exact (sparse LU)
+  linear solves on the oracle's assembly; through (i) ILU(0)-GMRES(50), the
+  solver config 3 names, and (ii) the multigrid path the benchmark uses, each
+  in FP64 and in mixed precision.  Bar: 1e-9 of max|x| (ILU, GMRES to 1e-6
+  per pass, so the fixed point is reached to the Newton tolerance) and 1e-8
+  (multigrid, Eisenstat-Walker tolerances).
+* The FP32 Jacobian of the mixed path against the oracle's FP64 Jacobian,
+  entry by entry: each element contribution is summed in FP64 and rounded to
+  FP32 once, then up to 8 element contributions are added in FP32, so the
+  error is a few FP32 ulps of the largest contribution: within 2^-20 of max|J|
+  everywhere and 2^-22 of |J| on 99 % of the entries.
+* 48^3 (4,093,060 unknowns, the benchmark's size): size-independent
+  properties.  The FP64 and mixed solutions both have a scaled residual below
+  1e-10 in L2 and max norms when it is recomputed by a fresh FP64 assembly and
+  row-sum scaling through the C ABI, and the mixed solution is within 1e-8 of
+  the FP64 one in every field.
+"""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from goma_amd import problems
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _prob(n):
+    return problems.lid_cavity(n=n, Re=400.0, dim=3, energy=True, Ra=1e4, Pr=0.71)
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from goma_amd import capi as M
+
+    M.lib()
+    assert torch.cuda.is_available()
+    return M
+
+
+@pytest.fixture(scope="module")
+def ref4(oracle_mod):
+    """Oracle FP64 solution of the n = 4 problem: full Newton with exact
+    linear solves to ||D^-1 R|| < 1e-12."""
+    prob = _prob(4)
+    o = oracle_mod.Oracle(prob)
+    x = np.zeros(o.N)
+    for it in range(30):
+        rc, r, v = o.fill(x)
+        assert rc == 0
+        s = np.zeros(o.N)
+        oracle_mod.lib().om_row_sum_scale(o.N, v, o.ija, r, s)
+        if it > 0 and np.linalg.norm(r) < 1e-12 and np.abs(r).max() < 1e-12:
+            return prob, o, x
+        A = oracle_mod.msr_to_scipy(o.ija, v, o.N).tocsc()
+        x -= spla.spsolve(A, r)
+    raise AssertionError("oracle Newton did not converge")
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_config3_ilu_gmres50_matches_oracle(prec, capi, ref4):
+    prob, o, x_ref = ref4
+    print(f"[case] config-3 physics n=4, ILU(0)-GMRES(50), prec={prec}", flush=True)
+    ctx = capi.Context(prob)
+    xd = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    rc, rep = ctx.solve_dev(xd.data_ptr(), kspace=50, max_its=1000, precond=1, precision=prec,
+                            max_newton=30)
+    assert rep.converged, rep.summary()
+    assert rep.resid_L2[rep.newton_its - 1] < 1e-10
+    x = xd.cpu().numpy()
+    assert np.abs(x - x_ref).max() <= 1e-9 * np.abs(x_ref).max()
+    ctx.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_config3_multigrid_matches_oracle(prec, capi, ref4):
+    prob, o, x_ref = ref4
+    print(f"[case] config-3 physics n=4, multigrid (benchmark path), prec={prec}", flush=True)
+    ctx = capi.Context(prob)
+    assert ctx.attach_multigrid(2) == 1
+    xd = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    rc, rep = ctx.solve_dev(xd.data_ptr(), kspace=50, max_its=200, precond=2, precision=prec,
+                            max_newton=40, mg_nu=1, mg_omega=0.7, line_search=1, mg_smoother=1,
+                            mg_galerkin=1, lin_tol_adapt=1)
+    assert rep.converged, rep.summary()
+    x = xd.cpu().numpy()
+    assert np.abs(x - x_ref).max() <= 1e-8 * np.abs(x_ref).max()
+    ctx.close()
+
+
+def test_config3_fp32_jacobian_matches_oracle(capi, ref4, oracle_mod):
+    prob, o, x_ref = ref4
+    print("[case] config-3 physics n=4, FP32 Jacobian entries", flush=True)
+    ctx = capi.Context(prob)
+    for x in (x_ref, x_ref + 0.1 * np.random.default_rng(3).standard_normal(o.N)):
+        rc, r_ref, v_ref = o.fill(x)
+        xd = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+        rd = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+        vd = torch.zeros(ctx.nval, dtype=torch.float32, device="cuda")
+        assert ctx.fill32_dev(xd.data_ptr(), rd.data_ptr(), vd.data_ptr()) == 0
+        v32 = vd.cpu().numpy().astype(np.float64)
+        big = np.abs(v_ref).max()
+        err = np.abs(v32 - v_ref)
+        assert err.max() <= 2.0 ** -20 * big, err.max() / big
+        nz = np.abs(v_ref) > 1e-6 * big
+        rel = err[nz] / np.abs(v_ref[nz])
+        assert np.quantile(rel, 0.99) <= 2.0 ** -22
+        # the residual stays FP64
+        r = rd.cpu().numpy()
+        assert np.abs(r - r_ref).max() <= 1e-12 * max(np.abs(r_ref).max(), 1.0)
+    ctx.close()
+
+
+def test_config3_full_size_properties(capi):
+    """48^3 HEX27: converged FP64 and mixed solves checked through a fresh
+    FP64 assembly, and mixed against FP64."""
+    from goma_amd.mesh import CENTROID_LOCAL
+
+    prob = _prob(48)
+    print("[case] config 3 at 48^3: FP64 and mixed solves", flush=True)
+    ctx = capi.Context(prob)
+    ctx.attach_multigrid(2)
+    opts = dict(kspace=50, max_its=200, precond=2, max_newton=40, mg_nu=1, mg_omega=0.7,
+                line_search=1, mg_smoother=1, mg_galerkin=1, lin_tol_adapt=1)
+    sols = {}
+    for prec in (0, 1):
+        xd = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+        rc, rep = ctx.solve_dev(xd.data_ptr(), precision=prec, **opts)
+        print(f"[case]   prec={prec}: {rep.newton_its} passes, {rep.lin_its_total} GMRES", flush=True)
+        assert rep.converged
+        sols[prec] = xd
+    rd = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    sd = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    for prec, xd in sols.items():
+        vd = torch.zeros(ctx.nval, dtype=torch.float64, device="cuda")
+        rd.zero_()
+        assert ctx.fill_dev(xd.data_ptr(), rd.data_ptr(), vd.data_ptr()) == 0
+        ctx.row_sum_scale_dev(vd.data_ptr(), rd.data_ptr(), sd.data_ptr())
+        del vd
+        l2 = float(torch.linalg.vector_norm(rd))
+        loo = float(rd.abs().max())
+        assert l2 < 1e-10 and loo < 1e-10, (prec, l2, loo)
+    x64, x32 = sols[0].cpu().numpy(), sols[1].cpu().numpy()
+    fu = ctx.first_unknown()[: prob.mesh.num_nodes].astype(np.int64)
+    cen = fu[prob.mesh.connect[:, CENTROID_LOCAL[3]]] + 4
+    groups = {"velocity": np.concatenate([fu + k for k in range(3)]), "temperature": fu + 3,
+              "pressure": np.concatenate([cen + k for k in range(4)])}
+    for name, idx in groups.items():
+        err = np.abs(x32[idx] - x64[idx]).max() / np.abs(x64[idx]).max()
+        assert err <= 1e-8, (name, err)
+    ctx.close()

@@ -2,19 +2,28 @@
 """Benchmark: seconds per converged steady Newton solve on one MI355X.
 
 A step is one converged Newton solve (||D^-1 R||_2 < 1e-10 and
-||D^-1 R||_oo < 1e-10, Goma's test) from the zero initial guess, with the
-mesh, matrix graph and all work arrays already resident in HBM.  The default
-workload is 2D Boussinesq natural convection on a 256x256 QUAD9 mesh
-(Q2/P1 velocity-pressure plus Q2 temperature, Pr = 0.71, Ra = 1e5), FP64.
+||D^-1 R||_oo < 1e-10, Goma's test) from the zero initial guess, in plain
+FP64, with the mesh, matrix graph and all work arrays already resident in
+HBM.  The default workload is the 3D lid-driven cavity with energy on a
+48x48x48 HEX27 mesh (Q2/P1 velocity-pressure plus Q2 temperature, Re = 400,
+Ra = 1e4, Pr = 0.71; 4,093,060 unknowns).
 
 Prints one JSON line.  Besides the headline value (wall time of the timed
-solves, no instrumentation) it reports, for the kernel that takes the most
-time, its algorithmic bytes per launch over its average launch duration as a
-fraction of the MI355X HBM peak.  Durations come from a repeat of the timed
-solve in which every launch carries HIP events stamped by its own dispatch on
-the library's stream.  It also reports the mixed-precision solve of the same
-workload with its nodal error against the FP64 solution, and the oracle's
-FP64 CPU path timed on a bounded sample of the same workload.
+solves, no instrumentation) it reports:
+  * for the kernel that takes the most time, its algorithmic bytes (or flops)
+    per launch over its average launch duration as a fraction of the MI355X
+    peak, and the same for the element assembly; durations come from a
+    repeat of the timed solve in which every launch carries HIP events
+    stamped by its own dispatch on the library's stream;
+  * variants on the same GPU at the same tolerance, each with its nodal error
+    against the FP64 solution: mixed precision from the zero guess; and, at
+    Ra = 1e4 held out of a continuation sweep, FP64 and mixed precision from
+    the previous-step guess and from the learned guess (the network's guess
+    evaluation inside the timed region; the sweep and the training reported
+    separately with the number of solves after which training pays off);
+  * the oracle's FP64 CPU path timed on the GPU box's host on a bounded sample
+    of the same workload, single-threaded and on all host cores, scaled by the
+    counts of one GPU solve.
 """
 from __future__ import annotations
 
@@ -22,6 +31,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -31,7 +41,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6   # MI355X FP64 (vector and matrix)
-FP32_PEAK_TFS = 157.3  # MI355X FP32 vector
+FP32_PEAK_TFS = 157.3  # MI355X FP32 vector / matrix
 METRIC = ("s per converged Newton solve (‖R‖<1e-10) vs plain FP64 GPU; "
           "assembly HBM GB/s")
 
@@ -64,41 +74,40 @@ def algorithmic_bytes(ctx, prob, prec: int):
     b_asm = nnz * sv + 16 * N + 8 * d * m.num_nodes + 4 * nve * m.num_elems
     # SpMV y = A x: values + column ids, row pointers, x and y
     b_spmv = nnz * (sv + 4) + 4 * (N + 1) + N * (sv + sv)
-    # one ILU sweep (L or U half): its values and column ids, row pointers,
-    # diagonal pointers, permutation, input and output vectors
-    half = (nnz - N) / 2
-    b_sweep_L = half * (sv + 4) + 4 * (N + 1) + 4 * N + 4 * N + N * (sv + sv)
-    b_sweep_U = (half + N) * (sv + 4) + 4 * (N + 1) + 4 * N + 4 * N + N * (sv + sv)
-    return dict(asm=b_asm, spmv=b_spmv, ilu_lower=b_sweep_L, ilu_upper=b_sweep_U)
+    return dict(asm=b_asm, spmv=b_spmv)
 
 
-def cpu_baseline(prob, gpu_counts: dict, budget_s: float):
-    """The oracle -- a single-threaded FP64 restatement of Goma's CPU path
-    (element loop with in_list scatter, GMRES(50) + natural-order ILU(0)) --
-    timed on a bounded sample of the same workload.
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    Times one assembly over a prefix of the elements (scaled to the mesh), one
-    ILU(0) factorisation and ten GMRES iterations on the full system at the
-    zero guess, then scales each by the counts of the GPU solve.  The GPU's
-    multigrid needs far fewer GMRES iterations than ILU(0) does, so the
-    figure is a lower bound on the CPU path's time.
-    """
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+def host_threads() -> int:
+    """Host cores this process may use (the box's CPU share caps it at 16)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def _cpu_sample(prob, out: dict, key, gmres_its: int = 10):
+    """The oracle on one sample problem: one full assembly, one natural-order
+    ILU(0) factorisation and gmres_its GMRES(50) iterations (FP64, 1 thread)."""
     import ctypes as C
 
     import oracle as O
 
-    O.build()
     o = O.Oracle(prob)
-    ne = prob.mesh.num_elems
-    x = np.zeros(o.N)
-    e1 = min(ne, 4096)
-    t0 = time.perf_counter()
-    rc, r, v = o.fill(x, 0, e1)
-    t_asm_elem = (time.perf_counter() - t0) / e1
+    x = np.random.default_rng(1).standard_normal(o.N) * 0.1
     t0 = time.perf_counter()
     rc, r, v = o.fill(x)
-    t_full = time.perf_counter() - t0
+    t_fill = time.perf_counter() - t0
     s = np.zeros(o.N)
     O.lib().om_row_sum_scale(o.N, v, o.ija, r, s)
     rr = C.c_double()
@@ -106,26 +115,74 @@ def cpu_baseline(prob, gpu_counts: dict, budget_s: float):
     its1 = O.lib().om_gmres_msr(o.N, v, o.ija, r, np.zeros(o.N), o.h, 1, 50, 1, 1e-30, C.byref(rr))
     t_one = time.perf_counter() - t0
     t0 = time.perf_counter()
-    its = O.lib().om_gmres_msr(o.N, v, o.ija, r, np.zeros(o.N), o.h, 1, 50, 11, 1e-30, C.byref(rr))
+    its = O.lib().om_gmres_msr(o.N, v, o.ija, r, np.zeros(o.N), o.h, 1, 50, 1 + gmres_its, 1e-30,
+                               C.byref(rr))
     t_gm = time.perf_counter() - t0
     t_it = max((t_gm - t_one) / max(its - its1, 1), 1e-9)
-    t_fact = max(t_one - t_it, 0.0)
-    nasm = gpu_counts["assemblies"]
-    nres = gpu_counts["residual_fills"]
-    est = ((nasm + nres) * t_asm_elem * ne + gpu_counts["factorisations"] * t_fact
-           + gpu_counts["gmres_its"] * t_it)
-    return dict(value=round(est, 2), unit="s per converged solve (extrapolated lower bound)",
-                cores=1, kind="port",
-                sample=(f"oracle FP64 C, 1 thread: assembly of {e1} of {ne} elements "
-                        f"({t_asm_elem * 1e6:.1f} us/element; full assembly {t_full:.2f} s), one "
-                        f"natural-order ILU(0) factorisation ({t_fact:.2f} s) and {its - its1} "
-                        f"GMRES(50) iterations ({t_it * 1e3:.1f} ms each) on the full "
-                        f"{o.N}-unknown system at the zero guess; scaled by the GPU solve's "
-                        f"{nasm} Jacobian and {nres} residual assemblies, "
-                        f"{gpu_counts['factorisations']} factorisations and "
-                        f"{gpu_counts['gmres_its']} GMRES iterations (ILU(0) needs more)"),
-                assembly_elems_per_s=round(1.0 / t_asm_elem, 1), gmres_it_s=t_it,
-                ilu_factor_s=t_fact)
+    out[key] = dict(fill=t_fill, fact=max(t_one - t_it, 0.0), it=t_it, its=its - its1,
+                    elems=prob.mesh.num_elems, nnz=o.nval - 1)
+
+
+def cpu_baseline(workload: str, ctx, prob, counts: dict, threads: int):
+    """The oracle -- a single-threaded FP64 restatement of Goma's CPU path
+    (element loop with in_list scatter, GMRES(50) with natural-order ILU(0)) --
+    timed on a bounded sample of the same workload, then scaled by the counts
+    of one GPU solve.
+
+    The sample is the same physics and element type on a small mesh (8^3 in
+    3D, 32^2 in 2D): one assembly, one ILU(0) factorisation and ten GMRES(50)
+    iterations.  Assembly scales with the element count, the factorisation
+    and the iterations with the nonzeros.  It runs once on one thread, then
+    as ``threads`` concurrent copies (one per core, the multi-rank pattern
+    without its communication), whose aggregate rates give the all-core
+    figure.  The GPU's multigrid needs far fewer GMRES iterations than
+    ILU(0) would, so both figures are lower bounds on the CPU path's time.
+    """
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    O.build()
+    small, _ = make_problem(workload, 8 if prob.mesh.dim == 3 else 32)
+    one = {}
+    _cpu_sample(small, one, 0)
+    s1 = one[0]
+    par = {}
+    ths = [threading.Thread(target=_cpu_sample, args=(small, par, k)) for k in range(threads)]
+    t0 = time.perf_counter()
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    wall = time.perf_counter() - t0
+    # aggregate rates of the concurrent copies: each phase's wall time inside
+    # a thread includes the others' contention, so T copies do T samples of
+    # work in the mean phase time
+    mean = {k: float(np.mean([p[k] for p in par.values()])) for k in ("fill", "fact", "it")}
+    ne, nnz = prob.mesh.num_elems, ctx.nval - 1
+    nasm = counts["assemblies"] + counts["residual_fills"]
+
+    def estimate(fill_elem, fact_nnz, it_nnz):
+        return (nasm * fill_elem * ne + counts["factorisations"] * fact_nnz * nnz
+                + counts["gmres_its"] * it_nnz * nnz)
+
+    e1 = estimate(s1["fill"] / s1["elems"], s1["fact"] / s1["nnz"], s1["it"] / s1["nnz"])
+    eT = estimate(mean["fill"] / s1["elems"] / threads, mean["fact"] / s1["nnz"] / threads,
+                  mean["it"] / s1["nnz"] / threads)
+    return dict(
+        value=round(eT, 1), unit="s per converged solve (extrapolated lower bound)",
+        cores=threads, kind="port", cpu=cpu_model(),
+        sample=(f"oracle FP64 C on {small.name} ({s1['elems']} elements, {s1['nnz']} nonzeros): "
+                f"one assembly, one natural-order ILU(0) factorisation and {s1['its']} GMRES(50) "
+                f"iterations, run by {threads} concurrent threads (one copy each, "
+                f"{wall:.1f} s wall); scaled to {ne} elements and {nnz} nonzeros by the GPU "
+                f"solve's {counts['assemblies']} Jacobian and {counts['residual_fills']} "
+                f"residual assemblies, {counts['factorisations']} preconditioner set-ups and "
+                f"{counts['gmres_its']} GMRES iterations (ILU(0) needs more)"),
+        single_thread=dict(value=round(e1, 1), cores=1,
+                           assembly_elems_per_s=round(s1["elems"] / s1["fill"], 1),
+                           gmres_it_s_per_1e9_nnz=round(s1["it"] / s1["nnz"] * 1e9, 3)),
+        all_cores_assembly_elems_per_s=round(s1["elems"] * threads / mean["fill"], 1),
+        counts=counts)
 
 
 def recorded_traffic(workload: str, kernel: str):
@@ -157,32 +214,263 @@ def field_groups(ctx, prob):
     return groups
 
 
-def mixed_variant(ctx, prob, x_fp64, opts, torch):
-    """The mixed-precision solve of the same workload at the same tolerance,
-    and its nodal error per field against the FP64 solution."""
-    ref = x_fp64.cpu().numpy()
-    xm = torch.zeros_like(x_fp64)
-    o = dict(opts)
-    o["precision"] = 1
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    rc, rep = ctx.solve_dev(xm.data_ptr(), **o)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    xh = xm.cpu().numpy()
-    err = {k: float(np.max(np.abs(xh[i] - ref[i])) / max(np.max(np.abs(ref[i])), 1e-300))
-           for k, i in field_groups(ctx, prob).items()}
-    print(f"[bench] mixed precision: newton {rep.newton_its} gmres {rep.lin_its_total} {dt:.2f}s "
-          f"nodal error {err}", file=sys.stderr, flush=True)
-    return {"fp64_s": None, "mixed_s": round(dt, 4), "mixed_converged": bool(rep.converged),
-            "mixed_newton_its": rep.newton_its, "mixed_gmres_its": rep.lin_its_total,
-            "mixed_nodal_error_vs_fp64": err}
+def nodal_error(groups, x, ref):
+    xh, rh = x.cpu().numpy(), ref.cpu().numpy()
+    return {k: float(np.max(np.abs(xh[i] - rh[i])) / max(np.max(np.abs(rh[i])), 1e-300))
+            for k, i in groups.items()}
 
 
 def mlp_flops_per_point(n_in, n_out, hidden, width):
     """Algorithmic (unpadded) forward flops of the network per point."""
     dims = [n_in] + [width] * hidden + [n_out]
     return 2 * sum(a * b for a, b in zip(dims[:-1], dims[1:]))
+
+
+def kernel_roofline(name, k, prob, prec, workload):
+    """Roofline entry of one kernel family from its live timing."""
+    if name == "k_vanka_invert":
+        # factorisation of n x n element blocks: 2 n^3 flops per element (LU
+        # plus the n right-hand sides of the explicit inverse); the library's
+        # byte count is elements x n^2 x (2 s + 4), so flops follow from it
+        d, en = prob.mesh.dim, int(prob.phys.energy)
+        nloc = 9 * (d + en) + 3 if d == 2 else 27 * (d + en) + 4
+        s = 8 if prec == 0 else 4
+        flops = k["bytes_per_launch"] * 2.0 * nloc / (2 * s + 4)
+        tf = flops / (k["avg_ms"] * 1e-3) / 1e12
+        peak = FP64_PEAK_TFS if prec == 0 else FP32_PEAK_TFS
+        return dict(kernel=name, bound="mfma", achieved=round(tf, 2), peak=peak, unit="TFLOP/s",
+                    frac=round(tf / peak, 4), traffic=None, flops_per_launch=round(flops),
+                    avg_launch_ms=round(k["avg_ms"], 5),
+                    precision="fp64" if prec == 0 else "fp32")
+    a = k["GBps"] or 0.0
+    tr = recorded_traffic(workload, name) if prec == 0 else None
+    return dict(kernel=name, bound="hbm", achieved=round(a, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(a / HBM_PEAK_GBS, 4), traffic=round(tr) if tr else None,
+                bytes_per_launch=round(k["bytes_per_launch"] or 0),
+                avg_launch_ms=round(k["avg_ms"], 5))
+
+
+def learned_variants(args, torch, ctx, prob, target, x_ref, t_plain, fp64, mixed, log):
+    """Previous-step and learned guesses at a Rayleigh number held out of a
+    continuation sweep, in FP64 and mixed precision.
+
+    Sweep: Ra values a quarter decade apart, offset so that the target is the
+    geometric midpoint of two of them (FP64, zeroth-order continuation).  The
+    network is trained on the GPU on the sweep's solutions.  At the target:
+    previous-step guess = the sweep solution just below; learned guess = that
+    solution moved by the network's predicted change (delta form, projected
+    onto the sweep's subspace), with the previous-step guess as fallback."""
+    from goma_amd import continuation, learned, problems
+
+    groups = field_groups(ctx, prob)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    lt = np.log10(target)
+    ks = np.arange(args.sweep_count) - args.sweep_count // 2
+    ras = 10 ** (lt + (ks + 0.5) * args.sweep_step)
+    (steps, sols), t_sweep = timed(lambda: continuation.sweep(ctx, prob, ras, fp64))
+    if not all(s.converged for s in steps):
+        raise SystemExit("[bench] the continuation sweep did not converge")
+    log(f"[bench] sweep of {len(ras)} values [{ras[0]:.4g}, {ras[-1]:.4g}] in {t_sweep:.1f}s")
+    lg = learned.LearnedGuess(prob, ctx, ras, sols, hidden=4, width=256, seed=20261202)
+    hist, t_train = timed(lambda: lg.train(epochs=args.epochs, batch=args.batch, lr=2e-3,
+                                           lr_final=2e-5, seed=1))
+    fpp = mlp_flops_per_point(prob.mesh.dim + 1, lg.n_out, 4, 256)
+    tflops = 3.0 * fpp * lg.num_points * args.epochs / t_train / 1e12
+    nb = lg.set_basis(sols)
+    log(f"[bench] trained on {lg.num_points} points x {args.epochs} epochs in {t_train:.1f}s "
+        f"({tflops:.1f} TFLOP/s), loss {hist[0]:.2e} -> {hist[-1]:.2e}, subspace {nb}")
+    kp = int(np.searchsorted(ras, target)) - 1
+    ra_prev = float(ras[kp])
+    xprev = torch.tensor(sols[kp], device="cuda")
+    ctx.set_physics(problems.physics_at(prob, target))
+    # warm-up of the guess path (buffers allocated on first use), untimed
+    xw = xprev.clone()
+    lg.guess(xw.data_ptr(), target, ra_prev, xprev.data_ptr())
+    out = {}
+    for name, opts, use_net in (("fp64+previous", fp64, False), ("mixed+previous", mixed, False),
+                                ("fp64+learned", fp64, True), ("mixed+learned", mixed, True)):
+        x = torch.empty_like(xprev)
+
+        def run():
+            if use_net:
+                lg.guess(x.data_ptr(), target, ra_prev, xprev.data_ptr())
+                return ctx.solve_guess_dev(x.data_ptr(), xprev.data_ptr(), 3, 10.0, **opts)
+            x.copy_(xprev)
+            return ctx.solve_dev(x.data_ptr(), **opts)
+
+        (_, rep), dt = timed(run)
+        out[name] = dict(seconds=round(dt, 4), newton_its=rep.newton_its + rep.discarded_its,
+                         gmres_its=rep.lin_its_total, converged=bool(rep.converged),
+                         fell_back=bool(rep.fell_back), r0=rep.resid_L2[0],
+                         nodal_error_vs_fp64=nodal_error(groups, x, x_ref))
+        log(f"[bench] {name}: {dt:.3f}s, {out[name]['newton_its']} passes, "
+            f"{rep.lin_its_total} GMRES, r0 {rep.resid_L2[0]:.2e}, fallback {bool(rep.fell_back)}")
+    t_prev = out["fp64+previous"]["seconds"]
+    t_ml = out["mixed+learned"]["seconds"]
+    gain = out["mixed+previous"]["seconds"] - t_ml
+    res = dict(
+        target_Ra=target, sweep_Ra=[round(float(r), 1) for r in ras], held_out_between=[
+            round(ra_prev, 1), round(float(ras[kp + 1]), 1)],
+        variants=out,
+        speedup_mixed_learned_vs_fp64_previous=round(t_prev / t_ml, 3),
+        speedup_mixed_learned_vs_fp64_zero=round(t_plain / t_ml, 3),
+        sweep_seconds=round(t_sweep, 2),
+        training=dict(points=lg.num_points, epochs=args.epochs, seconds=round(t_train, 2),
+                      algorithmic_TFLOPs=round(tflops, 2), bf16_peak_TFLOPs=2500.0,
+                      loss_last=hist[-1],
+                      break_even_solves=(round(t_train / gain, 1) if gain > 0 else None)),
+        network=f"{prob.mesh.dim + 1}-256-256-256-256-{lg.n_out} tanh, BF16 MFMA",
+        subspace_dim=nb)
+    lg.close()
+    return res
+
+
+def headline(args, torch, capi):
+    """The timed FP64 solves, the profiled repeat, the variants and the CPU
+    baseline of one workload."""
+    from goma_amd import problems
+
+    log = lambda s: print(s, file=sys.stderr, flush=True)
+    prob, params = make_problem(args.workload, args.n)
+    t0 = time.perf_counter()
+    ctx = capi.Context(prob)
+    pc = {"ilu": 1, "mg": 2, "band": 3}[args.precond]
+    nlev = ctx.attach_multigrid(args.mg_coarse) if pc == 2 else 0
+    t_setup = time.perf_counter() - t0
+    log(f"[bench] {prob.name}: N={ctx.N} nnz={ctx.nval - 1} levels {nlev} setup {t_setup:.2f}s")
+    base = dict(kspace=args.kspace, max_its=args.max_its, patch_bins=args.patch_bins,
+                max_newton=40, verbose=1 if args.verbose else 0, precond=pc, mg_nu=args.mg_nu,
+                mg_omega=args.mg_omega, line_search=args.line_search, pc_reuse=args.pc_reuse,
+                mg_smoother=args.mg_smoother, mg_galerkin=args.mg_galerkin,
+                lin_tol_adapt=args.lin_adapt)
+    fp64, mixed = dict(base, precision=0), dict(base, precision=1)
+    x = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    for w in range(args.warmup):
+        x.zero_()
+        torch.cuda.synchronize()
+        rc, rep = ctx.solve_dev(x.data_ptr(), **fp64)
+        log(f"[bench] warmup {w}: newton {rep.newton_its} gmres {rep.lin_its_total} "
+            f"{rep.t_total:.2f}s")
+    reps = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        x.zero_()
+        rc, rep = ctx.solve_dev(x.data_ptr(), **fp64)
+        reps.append(rep.summary())
+        log(f"[bench] step {k}: newton {rep.newton_its} gmres {rep.lin_its_total} "
+            f"converged {bool(rep.converged)} {rep.t_total:.2f}s")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    if not all(r["converged"] for r in reps):
+        log("[bench] WARNING: a step did not converge")
+    t_plain = elapsed / args.steps
+    # Kernel timing on a repeat of the timed solve (same inputs, same launches):
+    # every launch carries events stamped by its own dispatch.  Kept out of the
+    # timed steps because per-launch events slow the launch-bound parts.
+    ctx.prof_enable(True, every=args.prof_every)
+    x.zero_()
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    rc, prep = ctx.solve_dev(x.data_ptr(), **fp64)
+    torch.cuda.synchronize()
+    t_prof = time.perf_counter() - tp
+    kern = ctx.prof_all()
+    ctx.prof_enable(False)
+    log(f"[bench] profiled repeat: newton {prep.newton_its} gmres {prep.lin_its_total} "
+        f"{t_prof:.2f}s")
+    x_ref = x.clone()
+    groups = field_groups(ctx, prob)
+
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    roof = kernel_roofline(dom, kern[dom], prob, 0, prob.name)
+    # assembly: a Jacobian + residual assembly is one launch of k_fill per
+    # Newton pass (per element colour if the kernel is coloured)
+    nasm = prep.newton_its
+    k_fill = kern.get("k_fill")
+    asm_s = k_fill["total_ms"] * 1e-3 / max(nasm, 1) if k_fill else None
+    byt = algorithmic_bytes(ctx, prob, 0)
+    asm_roof = kernel_roofline("k_fill", k_fill, prob, 0, prob.name) if k_fill else None
+    if asm_roof:
+        asm_roof["per_assembly_s"] = round(asm_s, 6)
+    out = {
+        "metric": METRIC,
+        "value": round(t_plain, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_plain * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{prob.name}: " + ", ".join(f"{k}={v}" for k, v in params.items())
+                   + ", zero initial guess, plain FP64",
+                   "unknowns": ctx.N, "nnz": ctx.nval - 1, "elements": prob.mesh.num_elems,
+                   "solver": ("Eisenstat-Walker forcing terms (>= 1e-6), " if args.lin_adapt else "")
+                   + f"GMRES({args.kspace}) + " + (
+                       "multigrid V(%d,%d) %s element-block smoothing, %s coarse operators, %d levels,"
+                       " exact coarsest solve" % (args.mg_nu, args.mg_nu,
+                                                  ("multiplicative", "additive")[args.mg_smoother],
+                                                  ("re-assembled", "Galerkin")[args.mg_galerkin], nlev)
+                       if pc == 2 else "ILU(0) patch order" if pc == 1 else "banded LU")
+                   + ", backtracking line search" * bool(args.line_search)},
+        "newton_its": reps[0]["newton_its"],
+        "gmres_its": reps[0]["lin_its_total"],
+        "final_resid_L2": reps[0]["resid_L2"][-1] if reps[0]["resid_L2"] else None,
+        "roofline": roof,
+        "assembly": dict(asm_roof or {}, element_assemblies_per_s=(
+            round(prob.mesh.num_elems / asm_s, 1) if asm_s else None),
+            GBps=round(byt["asm"] / asm_s / 1e9, 1) if asm_s else None),
+        "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in kern.items()},
+        "kernel_timing": ("HIP events stamped by each launch's own dispatch, on the library "
+                          f"stream, over a repeat of the timed solve ({t_prof:.2f} s with events)"),
+    }
+    if args.variants:
+        xm = torch.zeros_like(x)
+        torch.cuda.synchronize()
+        tm = time.perf_counter()
+        rc, rm = ctx.solve_dev(xm.data_ptr(), **mixed)
+        torch.cuda.synchronize()
+        t_mixed = time.perf_counter() - tm
+        # the first mixed solve allocates its FP32 buffers: time a second one
+        xm.zero_()
+        torch.cuda.synchronize()
+        tm = time.perf_counter()
+        rc, rm = ctx.solve_dev(xm.data_ptr(), **mixed)
+        torch.cuda.synchronize()
+        t_mixed = time.perf_counter() - tm
+        var = {"fp64+zero": dict(seconds=round(t_plain, 4), newton_its=reps[0]["newton_its"],
+                                 gmres_its=reps[0]["lin_its_total"]),
+               "mixed+zero": dict(seconds=round(t_mixed, 4), newton_its=rm.newton_its,
+                                  gmres_its=rm.lin_its_total, converged=bool(rm.converged),
+                                  nodal_error_vs_fp64=nodal_error(groups, xm, x_ref),
+                                  speedup_vs_fp64_zero=round(t_plain / t_mixed, 3))}
+        log(f"[bench] mixed precision: newton {rm.newton_its} gmres {rm.lin_its_total} "
+            f"{t_mixed:.2f}s nodal error {var['mixed+zero']['nodal_error_vs_fp64']}")
+        out["variants"] = var
+        if args.learned and "Ra" in params:
+            out["learned"] = learned_variants(args, torch, ctx, prob, float(params["Ra"]), x_ref,
+                                              t_plain, fp64, mixed, log)
+            ctx.set_physics(prob.phys)
+    if not args.no_cpu_baseline:
+        counts = dict(assemblies=prep.newton_its, residual_fills=prep.resid_fills,
+                      factorisations=prep.pc_builds, gmres_its=prep.lin_its_total)
+        out["cpu_baseline"] = cpu_baseline(args.workload, ctx, prob, counts, args.cpu_threads
+                                           or host_threads())
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
 
 
 def sweep_bench(args, torch, capi):
@@ -321,126 +609,6 @@ def sweep_bench(args, torch, capi):
     return 0
 
 
-def learned_bench(args, torch, capi):
-    """Mixed precision plus the learned initial guess against the plain FP64
-    path, on one target Rayleigh number.
-
-    Plain path: FP64 Newton from the zero guess at the target Ra.  Learned
-    path: a continuation sweep over Ra values a quarter decade apart, offset
-    so that the target is the geometric midpoint of two of them (the target is
-    held out), trains the network on the GPU; at the target, Newton in mixed
-    precision starts from the learned guess (delta form from the sweep
-    solution just below, projected onto the sweep's subspace) with the
-    previous-step guess as fallback.  The sweep and the training are reported
-    separately: they are done once per parameter study, not per solve.
-    """
-    from goma_amd import continuation, learned, problems
-
-    log = lambda s: print(s, file=sys.stderr, flush=True)
-    prob, params = make_problem(args.learned_base, args.n)
-    target = float(params["Ra"])
-    ctx = capi.Context(prob)
-    nlev = ctx.attach_multigrid(args.mg_coarse)
-    base = dict(kspace=args.kspace, max_its=args.max_its, max_newton=40, precond=2,
-                mg_nu=args.mg_nu, mg_omega=args.mg_omega, line_search=args.line_search,
-                pc_reuse=args.pc_reuse, mg_smoother=args.mg_smoother, mg_galerkin=args.mg_galerkin,
-                lin_tol_adapt=args.lin_adapt)
-    fp64, mixed = dict(base, precision=0), dict(base, precision=1)
-    groups = field_groups(ctx, prob)
-    log(f"[learned] {prob.name}: N={ctx.N} levels {nlev}, target Ra={target:g}")
-
-    def timed(fn):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        r = fn()
-        torch.cuda.synchronize()
-        return r, time.perf_counter() - t0
-
-    def nodal_error(x, ref):
-        xh, rh = x.cpu().numpy(), ref.cpu().numpy()
-        return {k: float(np.max(np.abs(xh[i] - rh[i])) / max(np.max(np.abs(rh[i])), 1e-300))
-                for k, i in groups.items()}
-
-    # plain FP64 path at the target, from zero (warm-up solve first)
-    ctx.set_physics(problems.physics_at(prob, target))
-    xw = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
-    ctx.solve_dev(xw.data_ptr(), **fp64)
-    x_ref = torch.zeros_like(xw)
-    (_, rp), t_plain = timed(lambda: ctx.solve_dev(x_ref.data_ptr(), **fp64))
-    log(f"[learned] plain FP64 from zero: {rp.newton_its} passes, {rp.lin_its_total} GMRES, "
-        f"{t_plain:.2f}s")
-    # sweep with the target held out
-    lt = np.log10(target)
-    ks = np.arange(args.sweep_count) - args.sweep_count // 2
-    ras = 10 ** (lt + (ks + 0.5) * args.sweep_step)
-    (steps, sols), t_sweep = timed(lambda: continuation.sweep(ctx, prob, ras, fp64, log=log if args.verbose else None))
-    if not all(s.converged for s in steps):
-        raise SystemExit("[learned] the sweep did not converge")
-    log(f"[learned] sweep of {len(ras)} values [{ras[0]:.4g}, {ras[-1]:.4g}] in {t_sweep:.1f}s")
-    lg = learned.LearnedGuess(prob, ctx, ras, sols, hidden=4, width=256, seed=20261202)
-    hist, t_train = timed(lambda: lg.train(epochs=args.epochs, batch=args.batch, lr=2e-3, lr_final=2e-5,
-                                           seed=1))
-    fpp = mlp_flops_per_point(prob.mesh.dim + 1, lg.n_out, 4, 256)
-    tflops = 3.0 * fpp * lg.num_points * args.epochs / t_train / 1e12
-    nb = lg.set_basis(sols)
-    log(f"[learned] trained on {lg.num_points} points x {args.epochs} epochs in {t_train:.1f}s "
-        f"({tflops:.1f} TFLOP/s), loss {hist[0]:.2e} -> {hist[-1]:.2e}, subspace {nb}")
-    kp = int(np.searchsorted(ras, target)) - 1
-    ra_prev = float(ras[kp])
-    xprev = torch.tensor(sols[kp], device="cuda")
-    ctx.set_physics(problems.physics_at(prob, target))
-    # warm-up of the mixed-precision buffers and of the guess path (allocated on
-    # first use, then kept for every later solve), untimed
-    xw = xprev.clone()
-    lg.guess(xw.data_ptr(), target, ra_prev, xprev.data_ptr())
-    ctx.solve_dev(xw.data_ptr(), **mixed)
-    out_v = {}
-    for name, opts, use_net in (("mixed+learned", mixed, True), ("fp64+learned", fp64, True),
-                                ("mixed+previous", mixed, False), ("fp64+previous", fp64, False)):
-        x = torch.empty_like(xprev)
-
-        def run():
-            if use_net:
-                lg.guess(x.data_ptr(), target, ra_prev, xprev.data_ptr())
-                return ctx.solve_guess_dev(x.data_ptr(), xprev.data_ptr(), 3, 10.0, **opts)
-            x.copy_(xprev)
-            return ctx.solve_dev(x.data_ptr(), **opts)
-
-        (_, rep), dt = timed(run)
-        out_v[name] = dict(seconds=round(dt, 4), newton_its=rep.newton_its + rep.discarded_its,
-                           gmres_its=rep.lin_its_total, converged=bool(rep.converged),
-                           fell_back=bool(rep.fell_back), r0=rep.resid_L2[0],
-                           phases_s={k: round(getattr(rep, k), 4) for k in
-                                     ("t_assemble", "t_scale", "t_precond", "t_solve", "t_update")},
-                           nodal_error_vs_fp64=nodal_error(x, x_ref))
-        log(f"[learned] {name}: {dt:.2f}s, {out_v[name]['newton_its']} passes, r0 "
-            f"{rep.resid_L2[0]:.2e}, fallback {bool(rep.fell_back)}, phases {out_v[name]['phases_s']}")
-    t_ml = out_v["mixed+learned"]["seconds"]
-    out = {
-        "metric": "s per converged Newton solve (‖R‖<1e-10): mixed precision + learned guess vs "
-                  "plain FP64 GPU",
-        "value": t_ml, "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f32/f64",
-        "data": "synthetic",
-        "config": {"workload": f"{prob.name} at Ra={target:g} " +
-                   ", ".join(f"{k}={v}" for k, v in params.items()),
-                   "unknowns": ctx.N, "nnz": ctx.nval - 1, "elements": prob.mesh.num_elems,
-                   "sweep": [round(float(r), 1) for r in ras], "network":
-                   f"{prob.mesh.dim + 1}-256-256-256-256-{lg.n_out} tanh, BF16 MFMA"},
-        "plain_fp64": {"seconds": round(t_plain, 4), "newton_its": rp.newton_its,
-                       "gmres_its": rp.lin_its_total, "initial_guess": "zero"},
-        "speedup_vs_plain_fp64": round(t_plain / t_ml, 3),
-        "variants": out_v,
-        "sweep_seconds": round(t_sweep, 2),
-        "training": {"points": lg.num_points, "epochs": args.epochs, "seconds": round(t_train, 2),
-                     "algorithmic_TFLOPs": round(tflops, 2), "loss_last": hist[-1]},
-        "subspace_dim": nb,
-    }
-    print(json.dumps(out), flush=True)
-    lg.close()
-    ctx.close()
-    return 0
-
-
 def copy_calibration(torch):
     """A plain device copy of known size (1 GiB read, 1 GiB written, three
     times): the reference for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE
@@ -462,15 +630,19 @@ def copy_calibration(torch):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="natconv2d")
+    ap.add_argument("--workload", default="cavity3d",
+                    help="cavity3d (48^3 HEX27 lid cavity + energy, the default), natconv2d "
+                         "(256^2 Boussinesq), carreau2d, cavity2d, sweep (config-4 Ra sweep and "
+                         "learned guess), copycal (PMC counter calibration copy)")
     ap.add_argument("--n", type=int, default=None, help="elements per side (default: workload's)")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--kspace", type=int, default=50)
     ap.add_argument("--max-its", type=int, default=200)
     ap.add_argument("--patch-bins", type=int, default=0)
     ap.add_argument("--precond", choices=["mg", "ilu", "band"], default="mg")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="sweep workload: Newton precision")
     ap.add_argument("--mg-nu", type=int, default=None,
                     help="smoothing sweeps per level (default 1)")
     ap.add_argument("--mg-omega", type=float, default=None,
@@ -483,6 +655,23 @@ def main(argv=None):
                     help="1: Galerkin R A P coarse operators; 0: re-assembled on each coarse mesh")
     ap.add_argument("--lin-adapt", type=int, default=1,
                     help="1: Eisenstat-Walker linear tolerances; 0: every solve to 1e-6 (Goma's fixed)")
+    ap.add_argument("--pc-reuse", type=int, default=0,
+                    help="preconditioner reuse across Newton passes: 0 recalc, 1 reuse, 2 adaptive")
+    ap.add_argument("--prof-every", type=int, default=1,
+                    help="time one launch in this many (pseudo-random), per kernel")
+    ap.add_argument("--variants", type=int, default=1,
+                    help="1: also time mixed precision and the previous-step / learned guesses")
+    ap.add_argument("--learned", type=int, default=1,
+                    help="1: continuation sweep, network training and the guess variants")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the all-core CPU sample (default: the host cores, at most 16)")
+    ap.add_argument("--sweep-count", type=int, default=8,
+                    help="Ra values of the continuation sweep (64 for the sweep workload)")
+    ap.add_argument("--sweep-step", type=float, default=0.25,
+                    help="log10 Ra spacing of the continuation sweep around the target")
+    ap.add_argument("--epochs", type=int, default=60, help="training epochs")
+    ap.add_argument("--batch", type=int, default=8192, help="points per training step")
     ap.add_argument("--sweep-precond", choices=["band", "mg"], default="band",
                     help="sweep workload: banded LU direct solve or multigrid")
     ap.add_argument("--held-out", type=int, default=32, help="sweep workload: held-out Ra values")
@@ -491,26 +680,12 @@ def main(argv=None):
                          "1: first-order, with the sensitivity dx/dRa)")
     ap.add_argument("--basis-half", type=int, default=3,
                     help="sweep workload: sweep solutions on each side of a held-out value in its subspace")
-    ap.add_argument("--pc-reuse", type=int, default=0,
-                    help="preconditioner reuse across Newton passes: 0 recalc, 1 reuse, 2 adaptive")
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--prof-every", type=int, default=1,
-                    help="time one launch in this many (pseudo-random), per kernel")
-    ap.add_argument("--variants", type=int, default=1,
-                    help="1: also time the mixed-precision solve and report its nodal error")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--sweep-count", type=int, default=64, help="sweep workload: Ra values")
-    ap.add_argument("--epochs", type=int, default=60, help="sweep / learned: training epochs")
-    ap.add_argument("--batch", type=int, default=8192, help="sweep / learned: points per training step")
-    ap.add_argument("--learned-base", default="cavity3d", help="learned workload: the problem")
-    ap.add_argument("--sweep-step", type=float, default=0.25,
-                    help="learned workload: log10 Ra spacing of the sweep")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args(argv)
     if args.mg_nu is None:  # measured with Eisenstat-Walker tolerances: V(1,1) additive everywhere
         args.mg_nu = 1
     if args.mg_omega is None:  # measured: 0.65 best of 0.5-0.9 at 256^2; 0.7 at 48^3
-        args.mg_omega = 0.7 if args.workload in ("cavity3d", "learned") else 0.65
+        args.mg_omega = 0.7 if args.workload == "cavity3d" else 0.65
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
@@ -525,136 +700,10 @@ def main(argv=None):
     if args.workload == "copycal":
         return copy_calibration(torch)
     if args.workload == "sweep":
+        if args.sweep_count == 8:
+            args.sweep_count = 64
         return sweep_bench(args, torch, capi)
-    if args.workload == "learned":
-        if args.sweep_count == 64:
-            args.sweep_count = 8
-        return learned_bench(args, torch, capi)
-    prec = 1 if args.precision == "mixed" else 0
-    prob, params = make_problem(args.workload, args.n)
-    t0 = time.perf_counter()
-    ctx = capi.Context(prob)
-    t_setup = time.perf_counter() - t0
-    bins = args.patch_bins or 0
-    pc = {"ilu": 1, "mg": 2, "band": 3}[args.precond]
-    nlev = ctx.attach_multigrid(args.mg_coarse) if pc == 2 else 0
-    t_setup = time.perf_counter() - t0
-    print(f"[bench] {prob.name}: N={ctx.N} nnz={ctx.nval - 1} levels {nlev} setup {t_setup:.2f}s",
-          file=sys.stderr, flush=True)
-    x = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
-    opts = dict(kspace=args.kspace, max_its=args.max_its, patch_bins=bins, precision=prec,
-                max_newton=40, verbose=1 if args.verbose else 0, precond=pc, mg_nu=args.mg_nu,
-                mg_omega=args.mg_omega, line_search=args.line_search, pc_reuse=args.pc_reuse, mg_smoother=args.mg_smoother,
-                mg_galerkin=args.mg_galerkin, lin_tol_adapt=args.lin_adapt)
-    for w in range(args.warmup):
-        x.zero_()
-        torch.cuda.synchronize()
-        rc, rep = ctx.solve_dev(x.data_ptr(), **opts)
-        print(f"[bench] warmup {w}: newton {rep.newton_its} gmres {rep.lin_its_total} "
-              f"{rep.t_total:.2f}s", file=sys.stderr, flush=True)
-    reps = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        x.zero_()
-        rc, rep = ctx.solve_dev(x.data_ptr(), **opts)
-        reps.append(rep.summary())
-        print(f"[bench] step {k}: newton {rep.newton_its} gmres {rep.lin_its_total} "
-              f"converged {bool(rep.converged)} {rep.t_total:.2f}s", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    if not all(r["converged"] for r in reps):
-        print("[bench] WARNING: a step did not converge", file=sys.stderr, flush=True)
-    # Kernel timing on a repeat of the timed solve (same inputs, same launches):
-    # every launch carries events stamped by its own dispatch.  Kept out of the
-    # timed steps because per-launch events slow this launch-bound solve.
-    ctx.prof_enable(True, every=args.prof_every)
-    x.zero_()
-    torch.cuda.synchronize()
-    tp = time.perf_counter()
-    rc, prep = ctx.solve_dev(x.data_ptr(), **opts)
-    torch.cuda.synchronize()
-    t_prof = time.perf_counter() - tp
-    kern = ctx.prof_all()
-    ctx.prof_enable(False)
-    print(f"[bench] profiled repeat: newton {prep.newton_its} gmres {prep.lin_its_total} "
-          f"{t_prof:.2f}s", file=sys.stderr, flush=True)
-    variants = None
-    if args.variants and prec == 0:
-        variants = mixed_variant(ctx, prob, x, opts, torch)
-
-    ms_step = elapsed / args.steps * 1e3
-    n_ecol = capi.lib().gm_num_elem_colours(ctx.h)
-    assemblies = kern["k_fill"]["launches"] // max(n_ecol, 1) if "k_fill" in kern else 0
-    # dominant kernel (largest total time) and its roofline
-    dom = max(kern, key=lambda k: kern[k]["total_ms"])
-    roof = None
-    if dom == "k_vanka_invert":
-        # Gauss-Jordan on n x n element blocks: 2 n^3 flops per element; the
-        # library's byte count is elements x n^2 x (2 s + 4), so flops follow
-        # from it.  Bound: FP64 (or FP32) vector arithmetic through LDS.
-        nloc = (9 * (prob.mesh.dim + int(prob.phys.energy)) + 3 if prob.mesh.dim == 2 else
-                27 * (prob.mesh.dim + int(prob.phys.energy)) + 4)
-        s = 8 if prec == 0 else 4
-        flops = kern[dom]["bytes_per_launch"] * 2.0 * nloc / (2 * s + 4)
-        tf = flops / (kern[dom]["avg_ms"] * 1e-3) / 1e12
-        peak = FP64_PEAK_TFS if prec == 0 else FP32_PEAK_TFS
-        roof = dict(kernel=dom, bound="fp64" if prec == 0 else "fp32", achieved=round(tf, 2),
-                    peak=peak, unit="TFLOP/s", frac=round(tf / peak, 4), traffic=None,
-                    flops_per_launch=round(flops), avg_launch_ms=round(kern[dom]["avg_ms"], 5))
-    elif kern[dom]["GBps"]:
-        a = kern[dom]["GBps"]
-        tr = recorded_traffic(prob.name, dom) if prec == 0 else None
-        roof = dict(kernel=dom, bound="hbm", achieved=round(a, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(a / HBM_PEAK_GBS, 4), traffic=round(tr) if tr else None,
-                    bytes_per_launch=round(kern[dom]["bytes_per_launch"]),
-                    avg_launch_ms=round(kern[dom]["avg_ms"], 5))
-    gm_its = sum(r["lin_its_total"] for r in reps) // len(reps)
-    newton = reps[0]["newton_its"]
-    asm_s = kern["k_fill"]["total_ms"] * 1e-3 / max(assemblies, 1)
-    byt = algorithmic_bytes(ctx, prob, prec)
-    out = {
-        "metric": METRIC,
-        "value": round(elapsed / args.steps, 4),
-        "unit": "s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 2),
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if prec == 0 else "f32/f64",
-        "data": "synthetic",
-        "config": {"workload": f"{prob.name} ({args.workload}) " + ", ".join(f"{k}={v}" for k, v in params.items()),
-                   "unknowns": ctx.N, "nnz": ctx.nval - 1, "elements": prob.mesh.num_elems,
-                   "precision": args.precision, "solver": ("Eisenstat-Walker forcing terms (>= 1e-6), " if args.lin_adapt else "") + f"GMRES({args.kspace}) + " + ("multigrid V(%d,%d) %s element-block smoothing, %s coarse operators, %d levels, exact coarsest solve" % (args.mg_nu, args.mg_nu, ("multiplicative", "additive")[args.mg_smoother], ("re-assembled", "Galerkin")[args.mg_galerkin], nlev) if pc == 2 else "ILU(0) patch order" if pc == 1 else "banded LU") + ", preconditioner reuse " + ("recalc", "reuse", "adaptive")[args.pc_reuse],
-                   "initial_guess": "zero"},
-        "newton_its": newton,
-        "gmres_its": gm_its,
-        "final_resid_L2": reps[0]["resid_L2"][-1] if reps[0]["resid_L2"] else None,
-        "assembly_s": round(asm_s, 6),
-        "assembly_GBps": round(byt["asm"] / asm_s / 1e9, 1) if asm_s > 0 else None,
-        "element_assemblies_per_s": round(prob.mesh.num_elems / asm_s, 1) if asm_s > 0 else None,
-        "spmv_GBps": round(kern["k_spmv_msr"]["GBps"], 1) if "k_spmv_msr" in kern else None,
-        "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
-                    for k, v in kern.items()},
-        "roofline": roof,
-        "kernel_timing": ("HIP events stamped by each launch's own dispatch, on the library "
-                          f"stream, over a repeat of the timed solve ({t_prof:.2f} s with events)"),
-    }
-    if variants:
-        variants["fp64_s"] = out["value"]
-        out["variants"] = variants
-    if not args.no_cpu_baseline:
-        nres = kern.get("k_fill(residual only)", {}).get("launches", 0) // max(n_ecol, 1)
-        counts = dict(assemblies=assemblies // max(args.steps, 1),
-                      residual_fills=nres // max(args.steps, 1), gmres_its=gm_its,
-                      factorisations=max(newton - 1, 1))
-        out["cpu_baseline"] = cpu_baseline(prob, counts, args.cpu_budget)
-    print(json.dumps(out), flush=True)
-    ctx.close()
-    return 0
+    return headline(args, torch, capi)
 
 
 if __name__ == "__main__":

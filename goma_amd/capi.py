@@ -73,7 +73,7 @@ class GmNewtonReport(C.Structure):
                 ("t_assemble", C.c_double),
                 ("t_scale", C.c_double), ("t_precond", C.c_double), ("t_solve", C.c_double),
                 ("t_update", C.c_double), ("t_total", C.c_double), ("fell_back", C.c_int),
-                ("discarded_its", C.c_int), ("pc_builds", C.c_int)]
+                ("discarded_its", C.c_int), ("pc_builds", C.c_int), ("resid_fills", C.c_int)]
 
     def summary(self) -> dict:
         n = min(self.newton_its, MAX_REPORT)
@@ -84,7 +84,8 @@ class GmNewtonReport(C.Structure):
                     t_assemble=self.t_assemble,
                     t_scale=self.t_scale, t_precond=self.t_precond, t_solve=self.t_solve,
                     t_update=self.t_update, t_total=self.t_total, fell_back=bool(self.fell_back),
-                    discarded_its=self.discarded_its, pc_builds=self.pc_builds)
+                    discarded_its=self.discarded_its, pc_builds=self.pc_builds,
+                    resid_fills=self.resid_fills)
 
 
 MLP_MAX_IN = 8

@@ -364,12 +364,14 @@ static int ensure_newton_buffers(gm_ctx *c, int prec) {
 // 0.5|R|^2 <= 0.5|R0|^2 + 0.5 slope damp, slope = -|(J dx).R(x - dx)|, down to
 // the minimum damping; keep the step with the smallest residual norm.
 // Residual-only assemblies, scaled with the current row sums.
-static int backtrack(gm_ctx *c, double *x, const gm_solver_opts *o, double L2, double *best_out) {
+static int backtrack(gm_ctx *c, double *x, const gm_solver_opts *o, double L2, double *best_out,
+                     int *trials) {
   int rc;
   const double eps0 = o->eps_resid;
   GM_HIP(hipMemcpyAsync(c->d_xsave, x, c->N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   auto trial = [&](double damp, double *g) -> int {
     int r;
+    (*trials)++;
     if ((r = gm_lincomb(c, x, c->d_xsave, damp, c->d_dx))) return r;
     if ((r = gm_zero(c, c->d_rtrial, c->N))) return r;
     if ((r = gm_fill_launch_ex(c, x, c->d_rtrial, nullptr, nullptr, 1))) return r;
@@ -524,7 +526,7 @@ static int solve_impl(gm_ctx *c, double *x, const gm_solver_opts *o, gm_newton_r
       if ((rc = gm_corr_norms(c, c->d_dx, x, &cL2r, &cLoor))) return rc;
       double taken = o->damp;
       if (o->line_search) {
-        if ((rc = backtrack(c, x, o, L2, &taken))) return rc;
+        if ((rc = backtrack(c, x, o, L2, &taken, &rep->resid_fills))) return rc;
       } else if ((rc = gm_axpy_update(c, x, c->d_dx, o->damp))) {
         return rc;
       }
@@ -571,6 +573,7 @@ extern "C" int gm_solve_nonlinear_guess_dev(gm_ctx *c, double *x, const double *
   rep->discarded_its = first.newton_its;
   rep->lin_its_total += first.lin_its_total;
   rep->pc_builds += first.pc_builds;
+  rep->resid_fills += first.resid_fills;
   rep->t_assemble += first.t_assemble;
   rep->t_scale += first.t_scale;
   rep->t_precond += first.t_precond;

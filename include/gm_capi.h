@@ -137,6 +137,7 @@ typedef struct gm_newton_report {
   int fell_back;       /* gm_solve_nonlinear_guess_dev: 1 if the guess was abandoned */
   int discarded_its;   /* Newton passes spent on the abandoned guess */
   int pc_builds;       /* preconditioner set-ups (see pc_reuse) */
+  int resid_fills;     /* residual-only assemblies (line-search trials) */
 } gm_newton_report;
 
 typedef struct gm_ctx gm_ctx;

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgm.so")
 OBJ = os.path.join(HERE, "_build")
 
-SOURCES = ["gm_setup.cpp", "gm_capi.cpp", "gm_fill.hip", "gm_linalg.hip", "gm_mg.hip", "gm_mlp.hip",
+SOURCES = ["gm_setup.cpp", "gm_capi.cpp", "gm_hier.cpp", "gm_fill.hip", "gm_linalg.hip", "gm_mg.hip", "gm_mlp.hip",
            "gm_band.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -23,6 +23,7 @@ GM_ERR_ARG = -2
 GM_ERR_HIP = -3
 GM_ERR_PIVOT = -4
 GM_ERR_NAN = -5
+GM_ERR_CONV = -6
 GM_PREC_FP64 = 0
 GM_PREC_MIXED = 1
 GM_PC_NONE = 0
@@ -88,6 +89,17 @@ class GmNewtonReport(C.Structure):
                     resid_fills=self.resid_fills)
 
 
+class GmTransientOpts(C.Structure):
+    _fields_ = [("delta_t", C.c_double), ("theta", C.c_double), ("num_steps", C.c_int),
+                ("startup_steps", C.c_int)]
+
+
+class GmTransientReport(C.Structure):
+    _fields_ = [("steps", C.c_int), ("time", C.c_double), ("newton_its_total", C.c_int),
+                ("lin_its_total", C.c_int), ("newton_its", C.c_int * MAX_REPORT),
+                ("t_total", C.c_double)]
+
+
 MLP_MAX_IN = 8
 MLP_MAX_OUT = 8
 
@@ -122,7 +134,9 @@ EXPORTS = [
     "gm_mlp_create", "gm_mlp_destroy", "gm_mlp_num_params", "gm_mlp_get_params",
     "gm_mlp_set_params", "gm_mlp_default_train_opts", "gm_mlp_set_data", "gm_mlp_train",
     "gm_mlp_loss_grad", "gm_mlp_predict_dev", "gm_mlp_predict", "gm_mlp_guess_dev",
-    "gm_mlp_set_basis", "gm_mlp_clear_basis",
+    "gm_mlp_set_basis", "gm_mlp_clear_basis", "gm_matrix_fill_full_transient_dev",
+    "gm_add_qside", "gm_clear_qside", "gm_solve_nonlinear_transient_dev", "gm_solve_transient_dev",
+    "gm_num_nodes", "gm_num_elems", "gm_mg_build", "gm_mg_level", "gm_mg_copy_prolongation",
 ]
 
 _lib = None
@@ -142,7 +156,8 @@ def lib():
     L.gm_ctx_create.argtypes = [C.POINTER(GmMesh), C.POINTER(GmPhysics), C.POINTER(GmBC),
                                 C.POINTER(vp)]
     L.gm_ctx_destroy.argtypes = [vp]
-    for f in ("gm_num_unknowns", "gm_msr_size", "gm_num_elem_colours"):
+    for f in ("gm_num_unknowns", "gm_msr_size", "gm_num_elem_colours", "gm_num_nodes",
+              "gm_num_elems"):
         getattr(L, f).argtypes = [vp]
     for f in ("gm_copy_msr_graph", "gm_copy_first_unknown", "gm_copy_elem_colours"):
         getattr(L, f).argtypes = [vp, vp]
@@ -164,12 +179,24 @@ def lib():
     L.gm_default_solver_opts.restype = None
     L.gm_mg_add_level.argtypes = [vp, vp, vp, vp, vp, vp]
     L.gm_mg_num_levels.argtypes = [vp]
+    L.gm_mg_build.argtypes = [vp, ip]
+    L.gm_mg_level.argtypes = [vp, ip]
+    L.gm_mg_level.restype = vp
+    L.gm_mg_copy_prolongation.argtypes = [vp, ip, vp, vp, vp]
     L.gm_mg_setup_dev.argtypes = [vp, vp, vp, vp]
     L.gm_mg_copy_level_matrix.argtypes = [vp, ip, vp]
     L.gm_prof_enable.argtypes = [vp, ip]
     L.gm_prof_get.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.gm_solve_nonlinear_guess_dev.argtypes = [vp, vp, vp, C.POINTER(GmSolverOpts), ip,
                                                C.c_double, C.POINTER(GmNewtonReport)]
+    L.gm_matrix_fill_full_transient_dev.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double, vp, vp]
+    L.gm_add_qside.argtypes = [vp, ip, vp, vp, C.c_double]
+    L.gm_clear_qside.argtypes = [vp]
+    L.gm_solve_nonlinear_transient_dev.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double,
+                                                   C.POINTER(GmSolverOpts),
+                                                   C.POINTER(GmNewtonReport)]
+    L.gm_solve_transient_dev.argtypes = [vp, vp, vp, C.POINTER(GmTransientOpts),
+                                         C.POINTER(GmSolverOpts), C.POINTER(GmTransientReport)]
     L.gm_mlp_create.argtypes = [C.POINTER(GmMlpShape), C.POINTER(vp)]
     L.gm_mlp_destroy.argtypes = [vp]
     L.gm_mlp_destroy.restype = None
@@ -242,12 +269,13 @@ class Context:
         self.nval = L.gm_msr_size(h)
 
     def close(self):
+        # the level contexts belong to this one and go with it
+        for cc in getattr(self, "_coarse", []):
+            cc.h = None
+        self._coarse = []
         if getattr(self, "h", None):
             lib().gm_ctx_destroy(self.h)
             self.h = None
-        for cc in getattr(self, "_coarse", []):
-            cc.close()
-        self._coarse = []
 
     def __del__(self):
         try:
@@ -262,7 +290,7 @@ class Context:
         return out
 
     def first_unknown(self) -> np.ndarray:
-        out = np.zeros(self.prob.mesh.num_nodes + 1, np.int32)
+        out = np.zeros(lib().gm_num_nodes(self.h) + 1, np.int32)
         _check(lib().gm_copy_first_unknown(self.h, out.ctypes.data), "gm_copy_first_unknown")
         return out
 
@@ -273,7 +301,7 @@ class Context:
         return mask, val
 
     def elem_colours(self) -> np.ndarray:
-        out = np.zeros(self.prob.mesh.num_elems, np.int32)
+        out = np.zeros(lib().gm_num_elems(self.h), np.int32)
         _check(lib().gm_copy_elem_colours(self.h, out.ctypes.data), "gm_copy_elem_colours")
         return out
 
@@ -300,6 +328,44 @@ class Context:
         for a in (x, resid, val):
             assert a.dtype == np.float64 and a.flags.c_contiguous
         return lib().gm_matrix_fill_full(self.h, x.ctypes.data, resid.ctypes.data, val.ctypes.data)
+
+    def fill_transient_dev(self, x_ptr, x_old_ptr, xdot_old_ptr, delta_t, theta, resid_ptr,
+                           val_ptr) -> int:
+        """The assembly of one theta-method time step (transient mass terms)."""
+        return lib().gm_matrix_fill_full_transient_dev(self.h, x_ptr, x_old_ptr, xdot_old_ptr,
+                                                       delta_t, theta, resid_ptr, val_ptr)
+
+    def add_qside(self, elems, sides, q: float):
+        """QSIDE weak heat flux q on the given (element, 1-based side) pairs."""
+        e = np.ascontiguousarray(elems, dtype=np.int32)
+        sd = np.ascontiguousarray(sides, dtype=np.int32)
+        assert e.shape == sd.shape
+        _check(lib().gm_add_qside(self.h, len(e), e.ctypes.data, sd.ctypes.data, float(q)),
+               "gm_add_qside")
+
+    def clear_qside(self):
+        _check(lib().gm_clear_qside(self.h), "gm_clear_qside")
+
+    def solve_transient_step_dev(self, x_ptr, x_old_ptr, xdot_old_ptr, delta_t, theta, **opts):
+        """Newton for one theta-method time step; x holds the prediction."""
+        o = solver_opts(**opts)
+        rep = GmNewtonReport()
+        rc = lib().gm_solve_nonlinear_transient_dev(self.h, x_ptr, x_old_ptr, xdot_old_ptr,
+                                                    delta_t, theta, C.byref(o), C.byref(rep))
+        _check(rc, "gm_solve_nonlinear_transient_dev")
+        return rc, rep
+
+    def solve_transient_dev(self, x_ptr, xdot_ptr, delta_t, theta, num_steps, startup_steps=3,
+                            **opts):
+        """The fixed-step time loop: x (initial condition -> final solution) and
+        xdot (initial derivative -> final derivative) are device arrays."""
+        t = GmTransientOpts(delta_t, theta, num_steps, startup_steps)
+        o = solver_opts(**opts)
+        rep = GmTransientReport()
+        rc = lib().gm_solve_transient_dev(self.h, x_ptr, xdot_ptr, C.byref(t), C.byref(o),
+                                          C.byref(rep))
+        _check(rc, "gm_solve_transient_dev")
+        return rc, rep
 
     def row_sum_scale_dev(self, val_ptr, b_ptr, scale_ptr):
         _check(lib().gm_row_sum_scale_dev(self.h, val_ptr, b_ptr, scale_ptr), "gm_row_sum_scale_dev")
@@ -351,10 +417,9 @@ class Context:
 
     # ---- multigrid hierarchy
     def attach_multigrid(self, min_elems_per_side: int = 2):
-        """Builds coarse contexts down to ``min_elems_per_side`` elements per
-        side and attaches them with their Q2/P1 prolongations."""
-        from . import multigrid
-
+        """Builds the multigrid hierarchy in the library (gm_mg_build): coarse
+        contexts down to ``min_elems_per_side`` elements per side with their
+        Q2/P1 prolongations.  Returns the number of levels."""
         if self.prob.mesh.n < 1:
             raise ValueError("geometric multigrid needs the structured mesh hierarchy; this mesh "
                              "has none (read from a file or renumbered, mesh.n == 0): use "
@@ -363,18 +428,23 @@ class Context:
             # the library keeps pointers to the attached levels: dropping them here
             # would leave it with freed contexts
             raise RuntimeError("a multigrid hierarchy is already attached to this context")
-        self._coarse = []
-        above = self
-        for cp in multigrid.hierarchy(self.prob, min_elems_per_side):
-            cc = Context(cp)
-            rp, ci, cv, inj = multigrid.prolongation(above.first_unknown(), cc.first_unknown(),
-                                                     above.prob.mesh, cp.mesh, cp.phys.energy)
-            self._mg_arrays = getattr(self, "_mg_arrays", []) + [(rp, ci, cv, inj)]
-            _check(lib().gm_mg_add_level(self.h, cc.h, rp.ctypes.data, ci.ctypes.data,
-                                         cv.ctypes.data, inj.ctypes.data), "gm_mg_add_level")
-            self._coarse.append(cc)
-            above = cc
-        return lib().gm_mg_num_levels(self.h)
+        nl = _check(lib().gm_mg_build(self.h, min_elems_per_side), "gm_mg_build")
+        self._coarse = [LevelContext(lib().gm_mg_level(self.h, k)) for k in range(1, nl)]
+        return nl
+
+    def mg_prolongation(self, level: int):
+        """CSR (rowptr, col, val) of the prolongation into level ``level`` - 1
+        from level ``level`` (>= 1) of the attached hierarchy."""
+        L = lib()
+        above = self if level == 1 else self._coarse[level - 2]
+        nnz = _check(L.gm_mg_copy_prolongation(self.h, level, None, None, None),
+                     "gm_mg_copy_prolongation")
+        rp = np.zeros(above.N + 1, np.int32)
+        ci = np.zeros(nnz, np.int32)
+        cv = np.zeros(nnz)
+        _check(L.gm_mg_copy_prolongation(self.h, level, rp.ctypes.data, ci.ctypes.data,
+                                         cv.ctypes.data), "gm_mg_copy_prolongation")
+        return rp, ci, cv
 
     def mg_level_matrix(self, level: int) -> np.ndarray:
         """FP64 host copy of the Jacobian of multigrid level ``level`` (1 = first
@@ -409,6 +479,20 @@ class Context:
                                bytes_per_launch=by / cnt if by else None,
                                GBps=(by / (ms * 1e-3) / 1e9) if (by and ms) else None)
         return out
+
+
+class LevelContext(Context):
+    """A multigrid level's context, owned by the finest level's Context:
+    the query methods work on it; it is released with its owner."""
+
+    def __init__(self, handle):  # noqa: D107 - borrowed handle, no set-up
+        self.h = C.c_void_p(handle)
+        self.prob = None
+        self.N = lib().gm_num_unknowns(self.h)
+        self.nval = lib().gm_msr_size(self.h)
+
+    def close(self):
+        self.h = None
 
 
 def mlp_query(param, out_scale, out_shift) -> GmMlpQuery:

@@ -108,7 +108,8 @@ static void free_ctx(gm_ctx *c) {
                   c->d_bindx, c->d_ecol_elems, c->d_e2nz, c->d_flag, c->d_tables, c->d_val,
                   c->d_resid, c->d_dx,    c->d_scale, c->d_val32,    c->d_red,   c->d_xwork,
                   c->d_xsave, c->d_rtrial, c->d_w, c->d_mg_rows, c->d_mg_wmul, c->d_kel,
-                  c->d_mg_gptr, c->d_mg_gent, c->d_mg_corr};
+                  c->d_mg_gptr, c->d_mg_gent, c->d_mg_corr, c->d_stab, c->d_qs_ptr,
+                  c->d_qs_side, c->d_qs_q};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->h_red) hipHostFree(c->h_red);
@@ -117,6 +118,8 @@ static void free_ctx(gm_ctx *c) {
     for (hipEvent_t e : c->ev_e[i]) hipEventDestroy(e);
   }
   if (c->stream) hipStreamDestroy(c->stream);
+  // the level contexts gm_mg_build made, after this context's levels were released
+  for (gm_ctx *l : c->owned_levels) free_ctx(l);
   delete c;
 }
 
@@ -181,7 +184,9 @@ static int create_impl(const gm_mesh *m, const gm_physics *ph, const gm_bc *bc, 
     c->dbc_val[u] = bc->datum_value;
   }
 
-  GM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  // a blocking stream: ordered after work the caller queued on the default
+  // stream (e.g. zeroing the arrays it passes in) and before what it queues later
+  GM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamDefault));
   if ((rc = upload(&c->d_coor, c->coor.data(), c->coor.size()))) return rc;
   if ((rc = upload(&c->d_conn, c->conn.data(), c->conn.size()))) return rc;
   if ((rc = upload(&c->d_fu, c->first_unknown.data(), c->first_unknown.size()))) return rc;
@@ -216,6 +221,8 @@ extern "C" int gm_ctx_create(const gm_mesh *m, const gm_physics *ph, const gm_bc
 extern "C" void gm_ctx_destroy(gm_ctx *c) { free_ctx(c); }
 
 extern "C" int gm_num_unknowns(const gm_ctx *c) { return c ? c->N : GM_ERR_ARG; }
+extern "C" int gm_num_nodes(const gm_ctx *c) { return c ? c->num_nodes : GM_ERR_ARG; }
+extern "C" int gm_num_elems(const gm_ctx *c) { return c ? c->num_elems : GM_ERR_ARG; }
 extern "C" int gm_msr_size(const gm_ctx *c) { return c ? (int)c->nval : GM_ERR_ARG; }
 
 extern "C" int gm_copy_msr_graph(const gm_ctx *c, int *bindx) {
@@ -303,6 +310,97 @@ extern "C" int gm_matrix_fill_full(gm_ctx *c, const double *x, double *resid, do
   }
   cleanup();
   return rc;
+}
+
+// Sets the theta-method state that the next assemblies read (x_old null: steady).
+static int set_transient(gm_ctx *c, const double *x_old, const double *xdot_old, double dt,
+                         double theta) {
+  if (!x_old) {
+    c->tr_x_old = c->tr_xdot_old = nullptr;
+    c->tr_cmass = c->tr_cold = 0.0;
+    return GM_OK;
+  }
+  if (!xdot_old || !(dt > 0.0) || !std::isfinite(dt) || !(theta >= 0.0 && theta <= 1.0)) {
+    gm_set_error("transient step needs x_old, xdot_old, delta_t > 0 and theta in [0, 1]");
+    return GM_ERR_ARG;
+  }
+  c->tr_x_old = x_old;
+  c->tr_xdot_old = xdot_old;
+  c->tr_cmass = (1.0 + 2.0 * theta) / dt;
+  c->tr_cold = 2.0 * theta;
+  return GM_OK;
+}
+
+extern "C" int gm_matrix_fill_full_transient_dev(gm_ctx *c, const double *x, const double *x_old,
+                                                 const double *xdot_old, double dt, double theta,
+                                                 double *resid, double *val) {
+  if (!c || !x || !resid || !val) return GM_ERR_ARG;
+  int rc = set_transient(c, x_old, xdot_old, dt, theta);
+  if (rc == GM_OK && !x_old) rc = GM_ERR_ARG;
+  if (rc) return rc;
+  rc = gm_fill_launch(c, x, resid, val, nullptr);
+  set_transient(c, nullptr, nullptr, 0.0, 0.0);
+  return rc;
+}
+
+// Groups the QSIDE entries per element (card order kept within an element)
+// and uploads them.
+static int upload_qside(gm_ctx *c) {
+  if (c->d_qs_ptr) hipFree(c->d_qs_ptr);
+  if (c->d_qs_side) hipFree(c->d_qs_side);
+  if (c->d_qs_q) hipFree(c->d_qs_q);
+  c->d_qs_ptr = c->d_qs_side = nullptr;
+  c->d_qs_q = nullptr;
+  const size_t n = c->qs_elem.size();
+  if (n == 0) return GM_OK;
+  std::vector<int> ptr(c->num_elems + 1, 0), side(n);
+  std::vector<double> q(n);
+  for (int e : c->qs_elem) ptr[e + 1]++;
+  for (int e = 0; e < c->num_elems; e++) ptr[e + 1] += ptr[e];
+  std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+  for (size_t k = 0; k < n; k++) {
+    const int at = pos[c->qs_elem[k]]++;
+    side[at] = c->qs_side[k];
+    q[at] = c->qs_q[k];
+  }
+  int rc;
+  if ((rc = upload(&c->d_qs_ptr, ptr.data(), ptr.size()))) return rc;
+  if ((rc = upload(&c->d_qs_side, side.data(), n))) return rc;
+  return upload(&c->d_qs_q, q.data(), n);
+}
+
+extern "C" int gm_add_qside(gm_ctx *c, int n, const int *elems, const int *sides, double q) {
+  if (!c || n < 0 || (n > 0 && (!elems || !sides)) || !std::isfinite(q)) return GM_ERR_ARG;
+  if (!c->energy) {
+    gm_set_error("QSIDE needs the energy equation");
+    return GM_ERR_ARG;
+  }
+  std::vector<int> count(c->num_elems, 0);
+  for (int e : c->qs_elem) count[e]++;
+  for (int k = 0; k < n; k++) {
+    if (elems[k] < 0 || elems[k] >= c->num_elems || sides[k] < 1 || sides[k] > 2 * c->dim) {
+      gm_set_error("QSIDE entry %d: element %d side %d outside the mesh", k, elems[k], sides[k]);
+      return GM_ERR_ARG;
+    }
+    if (++count[elems[k]] > GM_QS_MAX) {
+      gm_set_error("QSIDE: more than %d side entries on element %d", GM_QS_MAX, elems[k]);
+      return GM_ERR_ARG;
+    }
+  }
+  for (int k = 0; k < n; k++) {
+    c->qs_elem.push_back(elems[k]);
+    c->qs_side.push_back(sides[k]);
+    c->qs_q.push_back(q);
+  }
+  return upload_qside(c);
+}
+
+extern "C" int gm_clear_qside(gm_ctx *c) {
+  if (!c) return GM_ERR_ARG;
+  c->qs_elem.clear();
+  c->qs_side.clear();
+  c->qs_q.clear();
+  return upload_qside(c);
 }
 
 extern "C" int gm_row_sum_scale_dev(gm_ctx *c, double *val, double *b, double *scale) {
@@ -555,6 +653,92 @@ extern "C" int gm_solve_nonlinear_dev(gm_ctx *c, double *x, const gm_solver_opts
   if (!c || !x || !o || !rep) return GM_ERR_ARG;
   bool stalled = false;
   return solve_impl(c, x, o, rep, 0, 0.0, &stalled);
+}
+
+extern "C" int gm_solve_nonlinear_transient_dev(gm_ctx *c, double *x, const double *x_old,
+                                                const double *xdot_old, double dt, double theta,
+                                                const gm_solver_opts *o, gm_newton_report *rep) {
+  if (!c || !x || !x_old || !o || !rep) return GM_ERR_ARG;
+  int rc = set_transient(c, x_old, xdot_old, dt, theta);
+  if (rc) return rc;
+  bool stalled = false;
+  rc = solve_impl(c, x, o, rep, 0, 0.0, &stalled);
+  set_transient(c, nullptr, nullptr, 0.0, 0.0);
+  return rc;
+}
+
+extern "C" int gm_solve_transient_dev(gm_ctx *c, double *x, double *xdot,
+                                      const gm_transient_opts *t, const gm_solver_opts *o,
+                                      gm_transient_report *rep) {
+  if (!c || !x || !xdot || !t || !o || !rep) return GM_ERR_ARG;
+  std::memset(rep, 0, sizeof(*rep));
+  const double dt = t->delta_t;
+  if (!(dt > 0.0) || !std::isfinite(dt) || !(t->theta >= 0.0 && t->theta <= 1.0) ||
+      t->num_steps < 0 || t->startup_steps < 0) {
+    gm_set_error("transient options: delta_t > 0, theta in [0, 1], step counts >= 0");
+    return GM_ERR_ARG;
+  }
+  const double t0 = now_s();
+  const size_t nb = (size_t)c->N * sizeof(double);
+  // history: x_old, x_older, x_oldest, xdot_old, xdot_older
+  double *h[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  auto release = [&]() {
+    for (double *p : h)
+      if (p) hipFree(p);
+  };
+  for (double *&p : h)
+    if (hipMalloc(&p, nb) != hipSuccess) {
+      release();
+      gm_set_error("device allocation failed");
+      return GM_ERR_HIP;
+    }
+  double *x_old = h[0], *x_older = h[1], *x_oldest = h[2], *xd_old = h[3], *xd_older = h[4];
+  int rc = GM_OK;
+  auto run = [&]() -> int {
+    // the initial condition fills the history (src/rf_solve.c:1222-1229)
+    GM_HIP(hipMemcpyAsync(x_old, x, nb, hipMemcpyDeviceToDevice, c->stream));
+    GM_HIP(hipMemcpyAsync(x_older, x, nb, hipMemcpyDeviceToDevice, c->stream));
+    GM_HIP(hipMemcpyAsync(x_oldest, x, nb, hipMemcpyDeviceToDevice, c->stream));
+    GM_HIP(hipMemcpyAsync(xd_old, xdot, nb, hipMemcpyDeviceToDevice, c->stream));
+    int r;
+    if ((r = gm_zero(c, xd_older, c->N))) return r;
+    for (int n = 0; n < t->num_steps; n++) {
+      // backward Euler for the start-up steps, then the requested scheme
+      // (src/rf_solve.c:1378-1386)
+      const double th = n < t->startup_steps ? 0.0 : t->theta;
+      // predict_solution (src/rf_solve.c:2767-2830) with equal steps
+      if (th == 0.5) r = gm_lin3(c, x, 3.0, x_old, -3.0, x_older, 1.0, x_oldest);
+      else r = gm_lin3(c, x, 1.0, x_old, dt * (1.0 + th), xd_old, -th * dt, xd_older);
+      if (r) return r;
+      gm_newton_report nr;
+      if ((r = gm_solve_nonlinear_transient_dev(c, x, x_old, xd_old, dt, th, o, &nr)) < 0) return r;
+      rep->newton_its_total += nr.newton_its;
+      rep->lin_its_total += nr.lin_its_total;
+      if (n < GM_MAX_REPORT) rep->newton_its[n] = nr.newton_its;
+      if (!nr.converged) {
+        gm_set_error("time step %d (t = %g): Newton did not converge", n, (n + 1) * dt);
+        return GM_ERR_CONV;
+      }
+      // xdot consistent with the converged x, then the history shift
+      // (src/rf_solve.c:2359-2365)
+      const double cm = (1.0 + 2.0 * th) / dt;
+      if ((r = gm_lin3(c, xdot, cm, x, -cm, x_old, -2.0 * th, xd_old))) return r;
+      std::swap(xd_older, xd_old);
+      GM_HIP(hipMemcpyAsync(xd_old, xdot, nb, hipMemcpyDeviceToDevice, c->stream));
+      std::swap(x_oldest, x_older);
+      std::swap(x_older, x_old);
+      GM_HIP(hipMemcpyAsync(x_old, x, nb, hipMemcpyDeviceToDevice, c->stream));
+      rep->steps = n + 1;
+      rep->time = (n + 1) * dt;
+    }
+    GM_HIP(hipStreamSynchronize(c->stream));
+    return GM_OK;
+  };
+  rc = run();
+  hipStreamSynchronize(c->stream);
+  release();
+  rep->t_total = now_s() - t0;
+  return rc < 0 ? rc : rep->steps;
 }
 
 extern "C" int gm_solve_nonlinear_guess_dev(gm_ctx *c, double *x, const double *x_prev,

@@ -53,6 +53,15 @@ struct FillArgs {
   double *kel;  // optional [num_elems][E]: element matrices before the Dirichlet rows
                 // (for Galerkin multigrid coarse operators), in the e2nz layout
   int *flag;
+  // theta-method time step (nullptr x_old: steady): at each node
+  // xdot = cmass (x - x_old) - cold xdot_old, cmass = (1 + 2 theta) / dt, cold = 2 theta
+  const double *x_old, *xdot_old;
+  double cmass, cold;
+  // QSIDE weak heat flux: the side entries of element e are [qs_ptr[e], qs_ptr[e+1]),
+  // each a side number (1-based, Exodus order) and a flux; nullptr qs_ptr: none
+  const int *qs_ptr, *qs_side;
+  const double *qs_q;
+  const double *stab;  // side tables: [side][point] phi[NVE], dphidxi[NVE][DIM], weight
   int num_nodes;
   int resid_only;  // 1: residual only (Assemble_Jacobian = FALSE)
   long long E;
@@ -69,6 +78,18 @@ struct Tab {
   static constexpr int PSI = DPHI + NQ * NVE * DIM;
   static constexpr int WT = PSI + NQ * NP;
   static constexpr int SIZE = WT + NQ;
+};
+
+// Side tables for surface integrals (QSIDE): for each of the 2*DIM sides in
+// Exodus order and each surface Gauss point (3-point rule per free axis),
+// phi[NVE], dphidxi[NVE][DIM] and the weight.
+template <int DIM>
+struct STab {
+  static constexpr int NVE = El<DIM>::NVE;
+  static constexpr int NSIDE = 2 * DIM;
+  static constexpr int NSP = DIM == 2 ? 3 : 9;
+  static constexpr int REC = NVE + NVE * DIM + 1;
+  static constexpr int SIZE = NSIDE * NSP * REC;
 };
 
 template <int DIM, int EN, int VISC, typename JT, int NT>
@@ -97,6 +118,14 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
   __shared__ double s_dmu[NQ];           // (dmu/dgammadot) / gammadot, 0 if gammadot == 0
   __shared__ double s_f[NQ][DIM];
   __shared__ double s_div[NQ];
+  __shared__ double s_ud[NV][NVE];       // nodal time derivatives (transient)
+  __shared__ double s_vd[NQ][DIM];       // v_dot at the quadrature points
+  __shared__ double s_Td[NQ];            // T_dot
+  constexpr int NSP = STab<DIM>::NSP, SREC = STab<DIM>::REC;
+  __shared__ double s_qw[GM_QS_MAX][NSP];  // QSIDE: weight * surface det * flux per point
+  __shared__ int s_qsd[GM_QS_MAX];         // QSIDE: side index (0-based) of each entry
+  const int qs0 = A.qs_ptr ? A.qs_ptr[e] : 0;
+  const int nqs = A.qs_ptr ? A.qs_ptr[e + 1] - qs0 : 0;
 
   // 1. gather
   if (tid < NVE) {
@@ -108,6 +137,9 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
     for (int a = 0; a < NV; a++) {
       s_gun[a][tid] = f + a;
       s_u[a][tid] = A.x[f + a];
+      // xdot as solve_nonlinear_problem forms it after each update (src/rf_solve.c:2848)
+      s_ud[a][tid] = A.x_old ? A.cmass * (A.x[f + a] - A.x_old[f + a]) - A.cold * A.xdot_old[f + a]
+                             : 0.0;
     }
   } else if (tid >= 64 && tid < 64 + NP) {
     const int k = tid - 64;
@@ -198,6 +230,20 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
         for (int p = 0; p < DIM; p++) gT[p] += tk * s_g[q][k][p];
       }
     }
+    double vd[DIM], Td = 0.0;
+#pragma unroll
+    for (int a = 0; a < DIM; a++) vd[a] = 0.0;
+    if (A.x_old) {
+      for (int k = 0; k < NVE; k++) {
+        const double ph = tb[T::PHI + q * NVE + k];
+#pragma unroll
+        for (int a = 0; a < DIM; a++) vd[a] += s_ud[a][k] * ph;
+        if constexpr (EN) Td += s_ud[DIM][k] * ph;
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < DIM; a++) s_vd[q][a] = vd[a];
+    s_Td[q] = Td;
     double P = 0.0;
 #pragma unroll
     for (int k = 0; k < NP; k++) P += s_p[k] * tb[T::PSI + q * NP + k];
@@ -238,10 +284,44 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
     s_mu[q] = mu;
     s_dmu[q] = dmu;
     s_div[q] = div;
+  } else if (EN && tid >= NT / 2 && tid < NT / 2 + nqs * NSP) {
+    // QSIDE (src/bc_integ.c:1518-1519, 2156-2157): w * |surface Jacobian| * q
+    // at one surface Gauss point of one side entry; the side's free reference
+    // axes give the tangents, as get_side_info orders them
+    // (src/mm_fill_fill.c:3049-3072, 3182-3250)
+    const int k = (tid - NT / 2) / NSP, g = tid - NT / 2 - k * NSP;
+    const int sd = A.qs_side[qs0 + k] - 1;
+    const double *st = A.stab + (size_t)(sd * NSP + g) * SREC;
+    const int ax = (sd == 0 || sd == 2) ? 1 : (sd == 1 || sd == 3) ? 0 : 2;
+    int fr[2], nf = 0;
+#pragma unroll
+    for (int a = 0; a < DIM; a++)
+      if (a != ax) fr[nf++] = a;
+    double t1[DIM], t2[DIM];
+#pragma unroll
+    for (int j = 0; j < DIM; j++) {
+      t1[j] = 0.0;
+      t2[j] = 0.0;
+      for (int i = 0; i < NVE; i++) {
+        t1[j] += s_x[j][i] * st[NVE + i * DIM + fr[0]];
+        if constexpr (DIM == 3) t2[j] += s_x[j][i] * st[NVE + i * DIM + fr[1]];
+      }
+    }
+    double sdet;
+    if constexpr (DIM == 2) {
+      sdet = sqrt(t1[0] * t1[0] + t1[1] * t1[1]);
+    } else {
+      const double c0 = t1[1] * t2[2] - t1[2] * t2[1], c1 = t1[2] * t2[0] - t1[0] * t2[2],
+                   c2 = t1[0] * t2[1] - t1[1] * t2[0];
+      sdet = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    }
+    s_qw[k][g] = st[NVE + NVE * DIM] * sdet * A.qs_q[qs0 + k];
+    if (g == 0) s_qsd[k] = sd;
   }
   __syncthreads();
 
   const double rho = A.rho, rCp = A.rho * A.Cp, kc = A.kcond;
+  const double cm = A.x_old ? A.cmass : 0.0;  // d xdot / d x: the mass-matrix coefficient
   JT *__restrict__ out = (JT *)A.val;
   const int *__restrict__ map = A.e2nz + (size_t)e * A.E;
 
@@ -262,7 +342,7 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
             const double Pi_ap = (a == p ? -s_P[q] : 0.0) + s_mu[q] * (s_gv[q][a][p] + s_gv[q][p][a]);
             dif += s_g[q][i][p] * Pi_ap;
           }
-          R += s_dA[q] * (-rho * ph * adv - dif + ph * s_f[q][a]);
+          R += s_dA[q] * (-rho * ph * adv - dif + ph * s_f[q][a] - rho * ph * s_vd[q][a]);
         }
       } else {
         for (int q = 0; q < NQ; q++) {
@@ -273,8 +353,10 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
             adv += s_v[q][p] * s_gT[q][p];
             dif += s_g[q][i][p] * s_gT[q][p];
           }
-          R += s_dA[q] * (-rCp * ph * adv - kc * dif + ph * A.hsrc);
+          R += s_dA[q] * (-rCp * ph * adv - kc * dif + ph * A.hsrc - rCp * ph * s_Td[q]);
         }
+        for (int k = 0; k < nqs; k++)
+          for (int g = 0; g < NSP; g++) R += s_qw[k][g] * A.stab[(size_t)(s_qsd[k] * NSP + g) * SREC + i];
       }
     } else {
       const int k = t - NV * NVE;
@@ -329,7 +411,7 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
           dmb[a] = c * s2;
         }
       }
-      const double diag = rho * phi_i * advj + mu * GG;
+      const double diag = rho * phi_i * advj + mu * GG + rho * cm * pij;
 #pragma unroll
       for (int a = 0; a < DIM; a++) {
 #pragma unroll
@@ -346,7 +428,7 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
           acc[a][DIM] += dA * (-rho * A.g[a] * A.beta * pij);
           acc[DIM][a] += dA * (-rCp * pij * s_gT[q][a]);
         }
-        acc[DIM][DIM] += dA * (-rCp * phi_i * advj - kc * GG);
+        acc[DIM][DIM] += dA * (-rCp * phi_i * advj - kc * GG - rCp * cm * pij);
       }
     }
     if (A.kel) {
@@ -558,6 +640,54 @@ static std::vector<double> make_tables(int dim) {
   return t;
 }
 
+// Side tables (STab layout): the Q2 basis and its reference derivatives at
+// the 3 (2D) or 3 x 3 (3D) Gauss points of each side, sides in Exodus order
+// (get_side_info, src/mm_fill_fill.c:3049-3072, 3182-3250): the fixed
+// reference axis and value of sides 1..2*dim are (t,-1) (s,+1) (t,+1) (s,-1)
+// (u,-1) (u,+1); the free axes run in ascending order, the first slowest.
+static std::vector<double> make_side_tables(int dim) {
+  const int nve = dim == 2 ? 9 : 27, nside = 2 * dim, nsp = dim == 2 ? 3 : 9;
+  const int rec = nve + nve * dim + 1;
+  static const int axis[6] = {1, 0, 1, 0, 2, 2};
+  static const double fixv[6] = {-1.0, 1.0, 1.0, -1.0, -1.0, 1.0};
+  const double F2 = 0.77459666924148340428, w1 = 0.55555555555555555556, w2 = 0.88888888888888888888;
+  const double gp[3] = {-F2, 0.0, F2}, gw[3] = {w1, w2, w1};
+  std::vector<double> t((size_t)nside * nsp * rec);
+  for (int sd = 0; sd < nside; sd++) {
+    int fr[2], nf = 0;
+    for (int a = 0; a < dim; a++)
+      if (a != axis[sd]) fr[nf++] = a;
+    for (int g = 0; g < nsp; g++) {
+      const int g1 = dim == 2 ? g : g / 3, g2 = dim == 2 ? 0 : g % 3;
+      double ref[3] = {0.0, 0.0, 0.0};
+      ref[axis[sd]] = fixv[sd];
+      ref[fr[0]] = gp[g1];
+      double w = gw[g1];
+      if (dim == 3) {
+        ref[fr[1]] = gp[g2];
+        w *= gw[g2];
+      }
+      double *r = t.data() + (size_t)(sd * nsp + g) * rec;
+      for (int k = 0; k < nve; k++) {
+        const signed char *pos = dim == 2 ? kQuad9[k] : kHex27[k];
+        double v[3], dv[3];
+        for (int d = 0; d < dim; d++) lagrange1(pos[d], ref[d], v[d], dv[d]);
+        double ph = 1.0;
+        for (int d = 0; d < dim; d++) ph *= v[d];
+        r[k] = ph;
+        for (int d = 0; d < dim; d++) {
+          double dd = dv[d];
+          for (int e = 0; e < dim; e++)
+            if (e != d) dd *= v[e];
+          r[nve + k * dim + d] = dd;
+        }
+      }
+      r[nve + nve * dim] = w;
+    }
+  }
+  return t;
+}
+
 int gm_fill_setup_device(gm_ctx *c) {
   ElemLayout &L = c->lay;
   L.dim = c->dim;
@@ -574,6 +704,9 @@ int gm_fill_setup_device(gm_ctx *c) {
   std::vector<double> tab = make_tables(c->dim);
   GM_HIP(hipMalloc(&c->d_tables, tab.size() * sizeof(double)));
   GM_HIP(hipMemcpy(c->d_tables, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<double> stab = make_side_tables(c->dim);
+  GM_HIP(hipMalloc(&c->d_stab, stab.size() * sizeof(double)));
+  GM_HIP(hipMemcpy(c->d_stab, stab.data(), stab.size() * sizeof(double), hipMemcpyHostToDevice));
   const size_t ne = (size_t)c->num_elems;
   GM_HIP(hipMalloc(&c->d_e2nz, ne * L.total * sizeof(int)));
   GM_HIP(hipMemsetAsync(c->d_flag, 0, 4 * sizeof(int), c->stream));
@@ -643,6 +776,14 @@ int gm_fill_launch_ex(gm_ctx *c, const double *x, double *resid, double *val, fl
   a.val = val32 ? (void *)val32 : (void *)val;
   a.kel = c->kel_on ? c->d_kel : nullptr;
   a.flag = c->d_flag;
+  a.x_old = c->tr_x_old;
+  a.xdot_old = c->tr_xdot_old;
+  a.cmass = c->tr_cmass;
+  a.cold = c->tr_cold;
+  a.qs_ptr = c->qs_side.empty() ? nullptr : c->d_qs_ptr;
+  a.qs_side = c->d_qs_side;
+  a.qs_q = c->d_qs_q;
+  a.stab = c->d_stab;
   a.num_nodes = c->num_nodes;
   a.E = c->lay.total;
   a.off_mp = c->lay.off_mp;

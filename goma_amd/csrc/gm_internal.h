@@ -9,6 +9,7 @@
 
 #include "../../include/gm_capi.h"
 
+#define GM_QS_MAX 8 /* QSIDE side entries per element */
 #define GM_MAXROW 1024 /* longest Jacobian row (HEX27 vertex row: 125 nodes x 4 + 8 x 4 P = 532) */
 
 void gm_set_error(const char *fmt, ...);
@@ -109,6 +110,9 @@ struct MgLevel {
   double *d_pcol = nullptr, *d_prow = nullptr;  // pivot column / row of one elimination step
   int *d_perm = nullptr;      // row interchange of each step
   int dense_n = 0;
+  // host copy of P, kept by gm_mg_build (gm_mg_copy_prolongation)
+  std::vector<int> P_host_rowptr, P_host_col;
+  std::vector<double> P_host_val;
 };
 
 #define GM_MG_DIRECT_MAX 4096 /* largest coarsest level solved exactly */
@@ -141,6 +145,18 @@ struct gm_ctx {
   int *d_e2nz = nullptr;        // [num_elems * lay.total]
   int *d_flag = nullptr;        // device error flags
   double *d_tables = nullptr;   // reference basis tables
+  double *d_stab = nullptr;     // side basis tables (surface integrals)
+
+  // theta-method state of the next assemblies (tr_x_old == nullptr: steady):
+  // xdot = tr_cmass (x - x_old) - tr_cold xdot_old
+  const double *tr_x_old = nullptr, *tr_xdot_old = nullptr;
+  double tr_cmass = 0.0, tr_cold = 0.0;
+  // QSIDE weak heat-flux entries (element, 1-based side, flux), host copy in
+  // card order; on the device grouped per element: d_qs_ptr [num_elems+1]
+  std::vector<int> qs_elem, qs_side;
+  std::vector<double> qs_q;
+  int *d_qs_ptr = nullptr, *d_qs_side = nullptr;
+  double *d_qs_q = nullptr;
 
   // Newton work vectors
   double *d_val = nullptr, *d_resid = nullptr, *d_dx = nullptr, *d_scale = nullptr;
@@ -156,6 +172,7 @@ struct gm_ctx {
   KrylovWork kry;
   BandLU band;
   std::vector<MgLevel> mg;  // coarse levels, finest first
+  std::vector<gm_ctx *> owned_levels;  // level contexts made by gm_mg_build (freed with this one)
   MgLevel mg0;              // level-0 vectors and element inverses
   const double *mg_scale = nullptr;  // row-sum scale of the level-0 matrix
   int *d_mg_rows = nullptr;          // unknowns of each element colour (smoother residuals)
@@ -201,6 +218,8 @@ int gm_spmv_launch(gm_ctx *c, const double *val, const double *x, double *y);
 int gm_spmv32_launch(gm_ctx *c, const float *val, const double *x, double *y);
 int gm_lincomb(gm_ctx *c, double *out, const double *a, double s, const double *b);  // out = a - s b
 int gm_div_scale(gm_ctx *c, double *v, const double *scale);                        // v /= scale
+int gm_lin3(gm_ctx *c, double *out, double w0, const double *v0, double w1, const double *v1,
+            double w2, const double *v2);  // out = w0 v0 + w1 v1 + w2 v2 (v1, v2 may be null)
 int gm_dot(gm_ctx *c, const double *a, const double *b, double *out);
 int gm_ilu_plan(gm_ctx *c, int bins);
 int gm_ilu_factor(gm_ctx *c, const double *val, const float *val32, int prec);

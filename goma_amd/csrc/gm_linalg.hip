@@ -262,6 +262,25 @@ int gm_lincomb(gm_ctx *c, double *out, const double *a, double s, const double *
   return GM_OK;
 }
 
+// out = w0 v0 + w1 v1 + w2 v2 (null vectors and their weights skipped)
+__global__ void k_lin3(int n, double *__restrict__ out, double w0, const double *__restrict__ v0,
+                       double w1, const double *__restrict__ v1, double w2,
+                       const double *__restrict__ v2) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double s = w0 * v0[i];
+    if (v1) s += w1 * v1[i];
+    if (v2) s += w2 * v2[i];
+    out[i] = s;
+  }
+}
+
+int gm_lin3(gm_ctx *c, double *out, double w0, const double *v0, double w1, const double *v1,
+            double w2, const double *v2) {
+  k_lin3<<<2048, 256, 0, c->stream>>>(c->N, out, w0, v0, w1, v1, w2, v2);
+  GM_HIP(hipGetLastError());
+  return GM_OK;
+}
+
 int gm_div_scale(gm_ctx *c, double *v, const double *scale) {
   k_div_scale<<<2048, 256, 0, c->stream>>>(c->N, v, scale);
   GM_HIP(hipGetLastError());

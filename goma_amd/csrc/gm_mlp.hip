@@ -617,7 +617,9 @@ static int create_impl(const gm_mlp_shape *s, gm_mlp *m) {
     m->Nr.push_back(l == m->L - 1 ? s->n_out : s->width);
     m->N.push_back(l == m->L - 1 ? OUT_PAD : s->width);
   }
-  GM_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+  // a blocking stream: ordered after work the caller queued on the default
+  // stream (e.g. zeroing the arrays it passes in) and before what it queues later
+  GM_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamDefault));
   int rc;
   uint64_t st = s->seed;
   for (int l = 0; l < m->L; l++) {

@@ -20,7 +20,11 @@
  *   - No entry point aborts: every call returns GM_OK (0) or a negative
  *     status; gm_last_error() gives a message.
  *   - A context is not re-entrant: one host thread per context.  Every call
- *     is synchronous: device work is complete when it returns.
+ *     is synchronous: device work is complete when it returns.  The library
+ *     works on its own blocking stream, so its kernels start after the work
+ *     a caller queued on the default (NULL) stream, such as zeroing the
+ *     arrays it passes in; work on other non-blocking streams must be
+ *     complete before the call.
  *
  * Pointer arguments ending in _dev are device (HBM) pointers; the others are
  * host pointers.
@@ -38,6 +42,7 @@ extern "C" {
 #define GM_ERR_ELEM -1     /* zero or negative element Jacobian (matrix_fill_full's -1) */
 #define GM_ERR_PIVOT -4    /* zero pivot in ILU(0) */
 #define GM_ERR_NAN -5      /* non-finite residual norm (solve_nonlinear_problem's -1) */
+#define GM_ERR_CONV -6     /* a time step's Newton solve did not converge */
 
 #define GM_VAR_U1 0
 #define GM_VAR_U2 1
@@ -152,6 +157,8 @@ void gm_ctx_destroy(gm_ctx *ctx);
 const char *gm_last_error(void);
 
 int gm_num_unknowns(const gm_ctx *ctx);
+int gm_num_nodes(const gm_ctx *ctx);
+int gm_num_elems(const gm_ctx *ctx);
 int gm_msr_size(const gm_ctx *ctx);                /* bindx[N] = length of bindx and val */
 int gm_copy_msr_graph(const gm_ctx *ctx, int *bindx);        /* host copy, length bindx[N] */
 int gm_copy_first_unknown(const gm_ctx *ctx, int *fu);       /* [num_nodes+1] */
@@ -177,6 +184,28 @@ int gm_matrix_fill_full(gm_ctx *ctx, const double *x, double *resid, double *val
  * (+=); the residual stays FP64.  Replaces matrix_fill_full
  * (src/mm_fill.c:158-312) where the Jacobian is stored in single precision. */
 int gm_matrix_fill_full32_dev(gm_ctx *ctx, const double *x_dev, double *resid_dev, float *val32_dev);
+
+/* matrix_fill_full of one theta-method time step: the steady terms plus the
+ * transient mass terms of momentum and energy (src/mm_fill_momentum.c:534-539,
+ * J 1586-1590; src/mm_fill_energy.c:322-328, J 435-441), with the time
+ * derivative that solve_nonlinear_problem keeps in step with x,
+ *   xdot = (1 + 2 theta) / delta_t (x - x_old) - 2 theta xdot_old
+ * (src/rf_solve.c:2848), and its derivative (1 + 2 theta) / delta_t in J.
+ * theta = 0 is backward Euler, 0.5 Crank-Nicolson.  delta_t > 0. */
+int gm_matrix_fill_full_transient_dev(gm_ctx *ctx, const double *x_dev, const double *x_old_dev,
+                                      const double *xdot_old_dev, double delta_t, double theta,
+                                      double *resid_dev, double *val_dev);
+
+/* QSIDE weak heat flux on element sides (BC = QSIDE SS <id> <q>,
+ * src/bc_integ.c:1518-1519, 2156-2157): every later assembly adds
+ * q * phi_i * |surface Jacobian| integrated over side sides[k] (1-based,
+ * Exodus numbering as get_side_info, src/mm_fill_fill.c:3049-3072, 3182-3250)
+ * of element elems[k] to the energy rows of that element's nodes (no Jacobian
+ * term; Dirichlet rows keep x - V).  Entries accumulate over calls, at most
+ * 8 per element.  Needs the energy equation. */
+int gm_add_qside(gm_ctx *ctx, int n, const int *elems, const int *sides, double q);
+/* Drops every QSIDE entry. */
+int gm_clear_qside(gm_ctx *ctx);
 
 /* row_sum_scale_MSR (src/sl_matrix_util.c:506): scales val and b in place. */
 int gm_row_sum_scale_dev(gm_ctx *ctx, double *val_dev, double *b_dev, double *scale_dev);
@@ -212,6 +241,45 @@ int gm_solve_nonlinear_guess_dev(gm_ctx *ctx, double *x_dev, const double *x_pre
                                  const gm_solver_opts *opt, int watch_its, double drop,
                                  gm_newton_report *rep);
 
+/* solve_nonlinear_problem for one theta-method time step
+ * (src/mm_sol_nonlinear.c:596, called from src/rf_solve.c:1905): Newton on
+ * the transient residual of gm_matrix_fill_full_transient_dev.  x_dev holds
+ * the predicted solution on entry and the solution at t + delta_t on exit.
+ * With multigrid, Galerkin coarse operators (mg_galerkin = 1) carry the mass
+ * terms to the coarse levels; re-assembled coarse operators omit them. */
+int gm_solve_nonlinear_transient_dev(gm_ctx *ctx, double *x_dev, const double *x_old_dev,
+                                     const double *xdot_old_dev, double delta_t, double theta,
+                                     const gm_solver_opts *opt, gm_newton_report *rep);
+
+typedef struct gm_transient_opts {
+  double delta_t;      /* fixed time step */
+  double theta;        /* Time step parameter: 0 backward Euler, 0.5 Crank-Nicolson */
+  int num_steps;       /* time steps to take */
+  int startup_steps;   /* first steps taken with theta = 0 (Goma: 3, src/rf_solve.c:1378-1386) */
+} gm_transient_opts;
+
+typedef struct gm_transient_report {
+  int steps;           /* converged time steps taken */
+  double time;         /* time reached */
+  int newton_its_total, lin_its_total;
+  int newton_its[GM_MAX_REPORT];  /* per step (first GM_MAX_REPORT steps) */
+  double t_total;      /* seconds */
+} gm_transient_report;
+
+/* The fixed-step time loop of solve_problem (src/rf_solve.c:1307-2372):
+ * per step the predictor of predict_solution (src/rf_solve.c:2767-2830;
+ * second-order extrapolation for theta = 0.5, first-order otherwise), a
+ * transient Newton solve, then xdot = (1 + 2 theta)/dt (x - x_old) - 2 theta
+ * xdot_old and the history shift x_oldest <- x_older <- x_old <- x.
+ * x_dev is the initial condition on entry and the solution at
+ * t0 + steps * delta_t on exit; xdot_dev the initial time derivative
+ * (Goma starts from zero) on entry and the last one on exit.  Returns the
+ * number of steps taken, or a negative status; a step whose Newton solve does
+ * not converge ends the loop with GM_ERR_CONV (Goma would cut the step). */
+int gm_solve_transient_dev(gm_ctx *ctx, double *x_dev, double *xdot_dev,
+                           const gm_transient_opts *topt, const gm_solver_opts *opt,
+                           gm_transient_report *rep);
+
 void gm_default_solver_opts(gm_solver_opts *opt);
 
 /* Multigrid hierarchy for GM_PC_MG (the reference has no counterpart: Goma's
@@ -224,6 +292,26 @@ void gm_default_solver_opts(gm_solver_opts *opt);
 int gm_mg_add_level(gm_ctx *fine, gm_ctx *coarse, const int *P_rowptr, const int *P_col,
                     const double *P_val, const int *inject);
 int gm_mg_num_levels(const gm_ctx *ctx);
+/* Builds and attaches the whole hierarchy of a structured lattice mesh (nodes
+ * on a (2n+1)^dim lattice and elements numbered x fastest, as the generated
+ * workloads are): coarse meshes keeping every other element vertex, down to
+ * min_elems_per_side elements per side while n is even; each level a context
+ * owned by ctx (freed with it) with the physics of ctx and the Dirichlet masks
+ * of the finer level at the same nodes; Q2 / P1 prolongations exact for
+ * Q2 x P1 fields on affine meshes; injection at shared nodes.  Returns the
+ * number of levels including ctx, or GM_ERR_ARG for a mesh with another
+ * numbering (ILU(0) or the banded LU serve those) or when a hierarchy is
+ * already attached.  No counterpart in the reference (Goma's AztecOO path has
+ * one-level preconditioners only); it is what selects GM_PC_MG at Goma's
+ * solve_nonlinear_problem call sites. */
+int gm_mg_build(gm_ctx *ctx, int min_elems_per_side);
+/* The context of multigrid level `level` (0 = ctx itself), borrowed: valid
+ * while ctx lives; NULL if there is no such level. */
+gm_ctx *gm_mg_level(gm_ctx *ctx, int level);
+/* Host copy of the prolongation of level `level` >= 1 (finer unknowns x this
+ * level's unknowns) built by gm_mg_build, as CSR: rowptr [N_finer + 1],
+ * col, val [nnz] (any may be NULL); returns nnz. */
+int gm_mg_copy_prolongation(const gm_ctx *ctx, int level, int *rowptr, int *col, double *val);
 /* Coarse Jacobians at the injected state x_dev and element-block inverses for
  * the level-0 matrix val_dev (row-sum scaled by scale_dev, or NULL). */
 int gm_mg_setup_dev(gm_ctx *ctx, const double *x_dev, const double *val_dev, const double *scale_dev);

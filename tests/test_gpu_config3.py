@@ -68,8 +68,10 @@ def test_config3_ilu_gmres50_matches_oracle(prec, capi, ref4):
     print(f"[case] config-3 physics n=4, ILU(0)-GMRES(50), prec={prec}", flush=True)
     ctx = capi.Context(prob)
     xd = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    # Goma's backtracking line search (NLS_BACKTRACK): from the zero guess at
+    # Re 400 the full mixed-precision step can overshoot
     rc, rep = ctx.solve_dev(xd.data_ptr(), kspace=50, max_its=1000, precond=1, precision=prec,
-                            max_newton=30)
+                            max_newton=30, line_search=1, verbose=1)
     assert rep.converged, rep.summary()
     assert rep.resid_L2[rep.newton_its - 1] < 1e-10
     x = xd.cpu().numpy()
@@ -82,7 +84,7 @@ def test_config3_multigrid_matches_oracle(prec, capi, ref4):
     prob, o, x_ref = ref4
     print(f"[case] config-3 physics n=4, multigrid (benchmark path), prec={prec}", flush=True)
     ctx = capi.Context(prob)
-    assert ctx.attach_multigrid(2) == 1
+    assert ctx.attach_multigrid(2) == 2  # 4 -> 2 elements per side
     xd = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
     rc, rep = ctx.solve_dev(xd.data_ptr(), kspace=50, max_its=200, precond=2, precision=prec,
                             max_newton=40, mg_nu=1, mg_omega=0.7, line_search=1, mg_smoother=1,

@@ -60,7 +60,7 @@ def test_galerkin_coarse_matrices_equal_rap(name, capi, oracle_mod):
     A = oracle_mod.msr_to_scipy(orc.ija, v, orc.N)
     n_prev = ctx.N
     for lev in range(1, nl):
-        rp, ci, cv, _ = ctx._mg_arrays[lev - 1]
+        rp, ci, cv = ctx.mg_prolongation(lev)
         cc = ctx._coarse[lev - 1]
         P = sp.csr_matrix((cv, ci, rp), shape=(n_prev, cc.N))
         A = (P.T @ A @ P).tocsr()  # natural Galerkin operator of this level

@@ -1,0 +1,244 @@
+"""Theta-method time steps and QSIDE weak heat flux on the GPU, against the
+FP64 CPU oracle (whose mass terms and QSIDE integral are pinned in
+test_oracle_transient.py and test_oracle_qside.py).
+
+* Transient assembly: the GPU residual and Jacobian of one time step
+  (src/mm_fill_momentum.c:534-539, 1586-1590; src/mm_fill_energy.c:322-328,
+  435-441) with xdot = (1 + 2 theta)/dt (x - x_old) - 2 theta xdot_old
+  (src/rf_solve.c:2848) equal the oracle's within 1e-12 of the largest entry,
+  2D jittered Carreau and 3D, backward Euler and Crank-Nicolson.
+* Time loop: gm_solve_transient_dev (predictor, transient Newton, xdot update,
+  history shift; src/rf_solve.c:1307-2372) reaches the state of the same
+  loop written here over the oracle's assembly with exact sparse solves,
+  within 1e-9, through backward-Euler start-up steps and Crank-Nicolson steps.
+* Backward-Euler conduction of sin(pi x) decays as exp(-pi^2 t) with
+  first-order error in dt.
+* QSIDE: the GPU assembly with flux sides on two walls equals the oracle's
+  within 1e-12; conduction with T = 0 at x = 0 and flux q at x = 1 solves to
+  T = q x / k at the nodes; bad entries are refused.
+"""
+from dataclasses import replace
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from goma_amd import exodus, problems
+from goma_amd.problems import TEMP
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+BAND = dict(precond=3, kspace=20, max_its=40, lin_tol=1e-10, max_newton=12)
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from goma_amd import capi as M
+
+    M.lib()
+    assert torch.cuda.is_available()
+    return M
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def _relerr(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _transient_cases():
+    return {
+        "carreau2d_jitter": lambda: problems.natural_convection(n=4, Ra=1e4, carreau=True,
+                                                                jitter=0.15),
+        "cavity3d_energy": lambda: problems.lid_cavity(n=2, Re=400.0, dim=3, energy=True, Ra=1e4),
+    }
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.5])
+@pytest.mark.parametrize("name", list(_transient_cases()))
+def test_transient_fill_matches_oracle(name, theta, capi, oracle_mod):
+    print(f"[case] transient fill {name} theta {theta}", flush=True)
+    prob = _transient_cases()[name]()
+    prob = replace(prob, phys=replace(prob.phys, rho=1.3, Cp=2.1))
+    ctx = capi.Context(prob)
+    orc = oracle_mod.Oracle(prob)
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(orc.N) * 0.4
+    x_old = x + rng.standard_normal(orc.N) * 0.05
+    xdot_old = rng.standard_normal(orc.N)
+    dt = 0.02
+    c = (1.0 + 2.0 * theta) / dt
+    rc, R_ref, V_ref = orc.fill_transient(x, c * (x - x_old) - 2.0 * theta * xdot_old, c)
+    assert rc == 0
+    r = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    v = torch.zeros(ctx.nval, dtype=torch.float64, device="cuda")
+    xd, xo, xdo = _dev(x), _dev(x_old), _dev(xdot_old)
+    assert ctx.fill_transient_dev(xd.data_ptr(), xo.data_ptr(), xdo.data_ptr(), dt, theta,
+                                  r.data_ptr(), v.data_ptr()) == 0
+    assert _relerr(r.cpu().numpy(), R_ref) < 1e-12
+    assert _relerr(v.cpu().numpy(), V_ref) < 1e-12
+    # the mass terms are really there: the steady fill differs
+    _, R_st, V_st = orc.fill(x)
+    assert _relerr(R_st, R_ref) > 1e-3 and _relerr(V_st, V_ref) > 1e-3
+    # a steady assembly after the transient one carries no mass terms
+    r.zero_()
+    v.zero_()
+    assert ctx.fill_dev(xd.data_ptr(), r.data_ptr(), v.data_ptr()) == 0
+    assert _relerr(r.cpu().numpy(), R_st) < 1e-12 and _relerr(v.cpu().numpy(), V_st) < 1e-12
+    ctx.close()
+
+
+def _oracle_time_loop(oracle_mod, orc, x0, dt, theta, nsteps, startup):
+    """The fixed-step theta-method loop over the oracle's transient assembly,
+    each step's Newton with exact sparse solves to ||R||oo < 1e-12."""
+    x_old, xd_old = x0.copy(), np.zeros_like(x0)
+    for n in range(nsteps):
+        th = 0.0 if n < startup else theta
+        c = (1.0 + 2.0 * th) / dt
+        x = x_old.copy()
+        for _ in range(30):
+            rc, R, V = orc.fill_transient(x, c * (x - x_old) - 2.0 * th * xd_old, c)
+            assert rc == 0
+            if np.abs(R).max() < 1e-12:
+                break
+            x -= spla.spsolve(oracle_mod.msr_to_scipy(orc.ija, V, orc.N).tocsc(), R)
+        else:
+            raise AssertionError("oracle Newton did not converge")
+        xd_old = c * (x - x_old) - 2.0 * th * xd_old
+        x_old = x
+    return x_old, xd_old
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_time_loop_matches_oracle(dim, capi, oracle_mod):
+    print(f"[case] transient loop dim {dim}", flush=True)
+    prob = (problems.natural_convection(n=4, Ra=1e4, jitter=0.1) if dim == 2 else
+            problems.lid_cavity(n=2, Re=100.0, dim=3, energy=True, Ra=1e3))
+    ctx = capi.Context(prob)
+    orc = oracle_mod.Oracle(prob)
+    x0 = np.zeros(orc.N)
+    x0[orc.dbc_mask != 0] = orc.dbc_val[orc.dbc_mask != 0]  # find_and_set_Dirichlet
+    dt, theta, nsteps, startup = 0.01, 0.5, 6, 3
+    x_ref, xd_ref = _oracle_time_loop(oracle_mod, orc, x0, dt, theta, nsteps, startup)
+    x = _dev(x0)
+    xd = torch.zeros_like(x)
+    rc, rep = ctx.solve_transient_dev(x.data_ptr(), xd.data_ptr(), dt, theta, nsteps, startup,
+                                      eps_resid=1e-11, **BAND)
+    assert rc == nsteps and rep.steps == nsteps
+    assert rep.time == pytest.approx(nsteps * dt)
+    assert _relerr(x.cpu().numpy(), x_ref) < 1e-9
+    assert np.abs(xd.cpu().numpy() - xd_ref).max() < 1e-6 * np.abs(xd_ref).max()
+    # the flow moved: not the initial state
+    assert _relerr(x.cpu().numpy(), x0) > 1e-3
+    ctx.close()
+
+
+def _conduction(dim, n, k, flux_wall):
+    """No flow (g = 0), T = 0 on x = 0 and, unless flux_wall, on x = 1."""
+    prob = (problems.natural_convection(n=n, Ra=0.0, jitter=0.1) if dim == 2 else
+            problems.lid_cavity(n=n, dim=3, energy=True, Ra=0.0, Re=1.0, jitter=0.1))
+    keep = prob.dbc_var != TEMP
+    lat = prob.mesh.lattice()
+    walls = lat[:, 0] == 0 if flux_wall else (lat[:, 0] == 0) | (lat[:, 0] == 2 * n)
+    tn = np.flatnonzero(walls)
+    return replace(prob, dbc_node=np.concatenate([prob.dbc_node[keep], tn]).astype(np.int32),
+                   dbc_var=np.concatenate([prob.dbc_var[keep], np.full(tn.size, TEMP)]).astype(np.int32),
+                   dbc_value=np.zeros(int(keep.sum()) + tn.size),
+                   phys=replace(prob.phys, kcond=k, g=(0.0, 0.0, 0.0), rho=1.0, Cp=1.0))
+
+
+def test_backward_euler_conduction_decays_first_order(capi):
+    print("[case] conduction decay", flush=True)
+    prob = _conduction(2, 8, 1.0, flux_wall=False)
+    ctx = capi.Context(prob)
+    fu = ctx.first_unknown().astype(np.int64)[:-1]
+    tdof = fu + 2
+    xs = prob.mesh.coor[0]
+    tend = 0.05
+    errs = []
+    for steps in (5, 10):
+        x0 = np.zeros(ctx.N)
+        x0[tdof] = np.sin(np.pi * xs)
+        x = _dev(x0)
+        xd = torch.zeros_like(x)
+        rc, rep = ctx.solve_transient_dev(x.data_ptr(), xd.data_ptr(), tend / steps, 0.0, steps, 0,
+                                          **BAND)
+        assert rc == steps
+        T = x.cpu().numpy()[tdof]
+        errs.append(np.abs(T - np.exp(-np.pi ** 2 * tend) * np.sin(np.pi * xs)).max())
+    order = np.log2(errs[0] / errs[1])
+    assert 0.85 < order < 1.15, (errs, order)
+    ctx.close()
+
+
+@pytest.mark.parametrize("dim,n", [(2, 3), (3, 2)])
+def test_qside_fill_matches_oracle(dim, n, capi, oracle_mod):
+    print(f"[case] qside fill dim {dim}", flush=True)
+    prob = (problems.natural_convection(n=n, Ra=1e4, jitter=0.15) if dim == 2 else
+            problems.lid_cavity(n=n, Re=100.0, dim=3, energy=True, Ra=1e3, jitter=0.15))
+    sides = exodus.wall_side_sets(prob.mesh)
+    ctx = capi.Context(prob)
+    orc = oracle_mod.Oracle(prob)
+    for face, q in ((2, 0.7), (3, -0.3)):  # x = 1 and y = 0: they share an edge
+        e, sd = sides[face]
+        ctx.add_qside(e, sd, q)
+        assert orc.add_qside(e, sd, q) == 0
+    x = np.random.default_rng(5).standard_normal(orc.N) * 0.3
+    _, R_ref, V_ref = orc.fill(x)
+    r = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    v = torch.zeros(ctx.nval, dtype=torch.float64, device="cuda")
+    xd = _dev(x)
+    assert ctx.fill_dev(xd.data_ptr(), r.data_ptr(), v.data_ptr()) == 0
+    assert _relerr(r.cpu().numpy(), R_ref) < 1e-12 and _relerr(v.cpu().numpy(), V_ref) < 1e-12
+    # cleared: back to the plain residual
+    ctx.clear_qside()
+    _, R0, _ = oracle_mod.Oracle(prob).fill(x)
+    r.zero_()
+    v.zero_()
+    assert ctx.fill_dev(xd.data_ptr(), r.data_ptr(), v.data_ptr()) == 0
+    assert _relerr(r.cpu().numpy(), R0) < 1e-12
+    ctx.close()
+
+
+@pytest.mark.parametrize("dim,n,k", [(2, 4, 2.5), (3, 2, 0.4)])
+def test_qside_conduction_is_linear(dim, n, k, capi):
+    print(f"[case] qside conduction dim {dim}", flush=True)
+    prob = _conduction(dim, n, k, flux_wall=True)
+    e, sd = exodus.wall_side_sets(prob.mesh)[2]
+    ctx = capi.Context(prob)
+    q = 0.7
+    ctx.add_qside(e, sd, q)
+    x = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    rc, rep = ctx.solve_dev(x.data_ptr(), **BAND)
+    assert rep.converged
+    tdof = ctx.first_unknown().astype(np.int64)[:-1] + dim
+    T = x.cpu().numpy()[tdof]
+    assert np.abs(T - q * prob.mesh.coor[0] / k).max() <= 1e-10 * q / k
+    ctx.close()
+
+
+def test_qside_refuses_bad_input(capi):
+    prob = _conduction(2, 2, 1.0, flux_wall=True)
+    ctx = capi.Context(prob)
+    with pytest.raises(capi.GomaError):
+        ctx.add_qside([0], [5], 1.0)  # QUAD sides are 1..4
+    with pytest.raises(capi.GomaError):
+        ctx.add_qside([prob.mesh.num_elems], [1], 1.0)
+    with pytest.raises(capi.GomaError):
+        ctx.add_qside([0] * 9, [1] * 9, 1.0)  # more than 8 entries on one element
+    ctx.close()
+    iso = capi.Context(problems.lid_cavity(n=2, Re=100.0))  # no energy equation
+    with pytest.raises(capi.GomaError):
+        iso.add_qside([0], [1], 1.0)
+    iso.close()
+    # a transient step needs delta_t > 0
+    prob = problems.lid_cavity(n=2, Re=100.0)
+    ctx = capi.Context(prob)
+    x = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    with pytest.raises(capi.GomaError):
+        ctx.solve_transient_step_dev(x.data_ptr(), x.data_ptr(), x.data_ptr(), 0.0, 0.5)
+    ctx.close()

@@ -137,6 +137,7 @@ EXPORTS = [
     "gm_mlp_set_basis", "gm_mlp_clear_basis", "gm_matrix_fill_full_transient_dev",
     "gm_add_qside", "gm_clear_qside", "gm_solve_nonlinear_transient_dev", "gm_solve_transient_dev",
     "gm_num_nodes", "gm_num_elems", "gm_mg_build", "gm_mg_level", "gm_mg_copy_prolongation",
+    "gm_matrix_fill_full_assign_dev",
 ]
 
 _lib = None
@@ -167,6 +168,7 @@ def lib():
     L.gm_matrix_fill_full_dev.argtypes = [vp, vp, vp, vp]
     L.gm_matrix_fill_full.argtypes = [vp, vp, vp, vp]
     L.gm_matrix_fill_full32_dev.argtypes = [vp, vp, vp, vp]
+    L.gm_matrix_fill_full_assign_dev.argtypes = [vp, vp, vp, vp]
     L.gm_row_sum_scale_dev.argtypes = [vp, vp, vp, vp]
     L.gm_msr_matvec_dev.argtypes = [vp, vp, vp, vp]
     L.gm_gmres_dev.argtypes = [vp, vp, vp, vp, C.POINTER(GmSolverOpts), C.POINTER(ip),
@@ -319,6 +321,10 @@ class Context:
     # ---- compute (device pointers)
     def fill_dev(self, x_ptr: int, resid_ptr: int, val_ptr: int) -> int:
         return lib().gm_matrix_fill_full_dev(self.h, x_ptr, resid_ptr, val_ptr)
+
+    def fill_assign_dev(self, x_ptr: int, resid_ptr: int, val_ptr: int) -> int:
+        """The assembly with the Jacobian stored, not accumulated (val needs no zeroing)."""
+        return lib().gm_matrix_fill_full_assign_dev(self.h, x_ptr, resid_ptr, val_ptr)
 
     def fill32_dev(self, x_ptr: int, resid_ptr: int, val32_ptr: int) -> int:
         """The assembly with FP32 Jacobian values (mixed precision)."""

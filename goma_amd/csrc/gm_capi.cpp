@@ -109,7 +109,7 @@ static void free_ctx(gm_ctx *c) {
                   c->d_resid, c->d_dx,    c->d_scale, c->d_val32,    c->d_red,   c->d_xwork,
                   c->d_xsave, c->d_rtrial, c->d_w, c->d_mg_rows, c->d_mg_wmul, c->d_kel,
                   c->d_mg_gptr, c->d_mg_gent, c->d_mg_corr, c->d_stab, c->d_qs_ptr,
-                  c->d_qs_side, c->d_qs_q};
+                  c->d_qs_side, c->d_qs_q, c->d_rank};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->h_red) hipHostFree(c->h_red);
@@ -331,6 +331,17 @@ static int set_transient(gm_ctx *c, const double *x_old, const double *xdot_old,
   return GM_OK;
 }
 
+extern "C" int gm_matrix_fill_full_assign_dev(gm_ctx *c, const double *x, double *resid,
+                                              double *val) {
+  if (!c || !x || !resid || !val) return GM_ERR_ARG;
+  int rc;
+  if (!c->d_rank && (rc = gm_zero(c, val, c->nval))) return rc;
+  c->fill_overwrite = 1;
+  rc = gm_fill_launch(c, x, resid, val, nullptr);
+  c->fill_overwrite = 0;
+  return rc;
+}
+
 extern "C" int gm_matrix_fill_full_transient_dev(gm_ctx *c, const double *x, const double *x_old,
                                                  const double *xdot_old, double dt, double theta,
                                                  double *resid, double *val) {
@@ -528,12 +539,19 @@ static int solve_impl(gm_ctx *c, double *x, const gm_solver_opts *o, gm_newton_r
     double ta = now_s();
     if ((rc = gm_zero(c, c->d_resid, c->N))) return rc;
     if ((rc = gm_zero(c, c->d_dx, c->N))) return rc;
-    if (prec == GM_PREC_MIXED) rc = gm_zero32(c, c->d_val32, c->nval);
-    else rc = gm_zero(c, c->d_val, c->nval);
-    if (rc) return rc;
+    // the HEX27 assembly stores each node pair's first contribution, so the
+    // Jacobian needs no zeroing pass
+    const bool assign = c->d_rank != nullptr;
+    if (!assign) {
+      if (prec == GM_PREC_MIXED) rc = gm_zero32(c, c->d_val32, c->nval);
+      else rc = gm_zero(c, c->d_val, c->nval);
+      if (rc) return rc;
+    }
     c->kel_on = galerkin;
+    c->fill_overwrite = assign;
     rc = gm_fill_launch(c, x, c->d_resid, c->d_val, prec == GM_PREC_MIXED ? c->d_val32 : nullptr);
     c->kel_on = 0;
+    c->fill_overwrite = 0;
     if (rc) return rc;
     double un, unoo;
     if ((rc = gm_norms(c, c->d_resid, c->N, &un, &unoo))) return rc;

@@ -62,6 +62,9 @@ struct FillArgs {
   const int *qs_ptr, *qs_side;
   const double *qs_q;
   const double *stab;  // side tables: [side][point] phi[NVE], dphidxi[NVE][DIM], weight
+  const int *bindx;                 // MSR graph
+  const unsigned short *rank;       // rank map [num_elems][2][nve * nve] (k_build_ranks)
+  int overwrite;                    // 1: first touch of a node pair stores (no zeroing needed)
   int num_nodes;
   int resid_only;  // 1: residual only (Assemble_Jacobian = FALSE)
   long long E;
@@ -509,6 +512,608 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
   }
 }
 
+// ---------------------------------------------------------------------------
+// HEX27 assembly on the matrix cores.
+//
+// One workgroup (four waves) per element.  The basis tables, the physical
+// gradients and the quadrature-point fields are staged in LDS; every
+// node-pair block of the element Jacobian is a sum of products
+//   E(X, w, Y)[i][j] = sum_q X[q][i] w[q] Y[q][j]
+// of 27 x 27 operand tables (X, Y in {phi, d phi / d x_p}, weights from the
+// fields), computed with v_mfma_f64_16x16x4f64 on the 27 x 27 node-pair
+// matrix padded to 32 x 32 (two 16-wide tiles per side, one tile per wave)
+// and 27 quadrature points padded to 28 (seven k-steps).  Per block (a, b),
+// with c = d xdot / d x (transient), mu constant or Carreau mu(q):
+//   J_ab = E(phi, -rho dA d_b v_a, phi) - E(G_b, mu dA, G_a)
+//          - delta_ab [rho E(phi, dA, v.G) + sum_p E(G_p, mu dA, G_p) + rho c E(phi, dA, phi)]
+//          - E(H_a, c_mu dA, H_b)           (Carreau: H_a = sum_p G_p gamma_ap,
+//                                            c_mu = (d mu / d gammadot) / gammadot)
+//   J_aT = -rho g_a beta E(phi, dA, phi),   J_Ta = E(phi, -rho Cp dA d_a T, phi)
+//   J_TT = -rho Cp E(phi, dA, v.G) - k sum_p E(G_p, dA, G_p) - rho Cp c E(phi, dA, phi)
+// (the terms of src/mm_fill_momentum.c:1564-1735, 746-915 and
+// src/mm_fill_energy.c:425-487, 620-690 regrouped as products over q).
+// MFMA lane layout (checked on the device): lane l supplies A[l%16][l/16]
+// and B[l/16][l%16] and holds D[l/16 + 4r][l%16], r = 0..3 (rows interleaved).
+//
+// Scatter: each entry's MSR position comes from the element's rank map
+// (rank of node j's first column in the rows of node i, see
+// k_build_ranks), so a node pair's NV x NV block lands at
+// bindx[row] + rank + b - (col > row).  With `overwrite` the first element
+// (in colour order) to touch a node pair stores its values, later ones add:
+// no zeroing pass and no read of the old value on first touch.
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+template <int EN, int VISC, typename JT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VISC ? 3 : 4, VISC ? 3 : 4))) void k_fill3m(
+    FillArgs A, const int *__restrict__ elems) {
+  constexpr int DIM = 3, NVE = 27, NQ = 27, NP = 4, NV = DIM + EN, NVE2 = NVE * NVE;
+  constexpr int KP = 28, LD = 32, NT = 256, CL = 20;
+  using T = Tab<3>;
+  constexpr int NSP = STab<3>::NSP, SREC = STab<3>::REC;
+  const int e = elems[xcd_block()];
+  const int tid = threadIdx.x;
+  const double *__restrict__ tb = A.tables;
+
+  __shared__ double s_x[DIM][NVE];
+  __shared__ double s_u[NV][NVE];
+  __shared__ double s_ud[NV][NVE];
+  __shared__ double s_p[NP];
+  __shared__ int s_gun[NV][NVE];   // global row of each nodal unknown
+  __shared__ int s_brow[NV][NVE];  // bindx[row]: start of its off-diagonal list
+  __shared__ signed char s_dir[NV][NVE];
+  __shared__ int s_gp[NP], s_bp[NP];
+  __shared__ signed char s_dirp[NP];
+  __shared__ double s_B[NQ][DIM][DIM];
+  __shared__ double s_dA[KP];
+  __shared__ double s_Phi[KP][LD];
+  __shared__ double s_G[DIM][KP][LD];
+  __shared__ double s_v[KP][DIM];
+  __shared__ double s_gv[KP][DIM][DIM];  // gv[p][a] = d v_a / d x_p
+  __shared__ double s_gT[KP][DIM];
+  __shared__ double s_P[KP], s_mu[KP], s_dmu[KP], s_div[KP], s_Td[KP];
+  __shared__ double s_f[KP][DIM], s_vd[KP][DIM];
+  __shared__ double s_qw[GM_QS_MAX][NSP];
+  __shared__ int s_qsd[GM_QS_MAX];
+  const int qs0 = A.qs_ptr ? A.qs_ptr[e] : 0;
+  const int nqs = A.qs_ptr ? A.qs_ptr[e + 1] - qs0 : 0;
+
+  // 1. gather; basis values into the padded operand table
+  if (tid < NVE) {
+    const int n = A.conn[(size_t)e * NVE + tid];
+    const int f = A.fu[n];
+#pragma unroll
+    for (int k = 0; k < DIM; k++) s_x[k][tid] = A.coor[(size_t)k * A.num_nodes + n];
+#pragma unroll
+    for (int a = 0; a < NV; a++) {
+      const int g = f + a;
+      s_gun[a][tid] = g;
+      s_brow[a][tid] = A.bindx[g];
+      s_dir[a][tid] = A.dbc[g];
+      s_u[a][tid] = A.x[g];
+      s_ud[a][tid] = A.x_old ? A.cmass * (A.x[g] - A.x_old[g]) - A.cold * A.xdot_old[g] : 0.0;
+    }
+  } else if (tid >= 64 && tid < 64 + NP) {
+    const int k = tid - 64;
+    const int g = A.fu[A.centroid[e]] + DIM + EN + k;
+    s_gp[k] = g;
+    s_bp[k] = A.bindx[g];
+    s_dirp[k] = A.dbc[g];
+    s_p[k] = A.x[g];
+  }
+  for (int idx = tid; idx < KP * LD; idx += NT) {
+    const int q = idx / LD, i = idx - q * LD;
+    s_Phi[q][i] = (q < NQ && i < NVE) ? tb[T::PHI + q * NVE + i] : 0.0;
+  }
+  __syncthreads();
+
+  // 2. map per quadrature point (beer_belly)
+  if (tid < NQ) {
+    const int q = tid;
+    double J[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; i++)
+#pragma unroll
+      for (int j = 0; j < DIM; j++) {
+        double s = 0.0;
+#pragma unroll 3
+        for (int k = 0; k < NVE; k++) s += s_x[j][k] * tb[T::DPHI + (q * NVE + k) * DIM + i];
+        J[i][j] = s;
+      }
+    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                       J[0][1] * (J[1][0] * J[2][2] - J[2][0] * J[1][2]) +
+                       J[0][2] * (J[1][0] * J[2][1] - J[2][0] * J[1][1]);
+    s_B[q][0][0] = (J[1][1] * J[2][2] - J[2][1] * J[1][2]) / det;
+    s_B[q][0][1] = -(J[0][1] * J[2][2] - J[2][1] * J[0][2]) / det;
+    s_B[q][0][2] = (J[0][1] * J[1][2] - J[1][1] * J[0][2]) / det;
+    s_B[q][1][0] = -(J[1][0] * J[2][2] - J[2][0] * J[1][2]) / det;
+    s_B[q][1][1] = (J[0][0] * J[2][2] - J[2][0] * J[0][2]) / det;
+    s_B[q][1][2] = -(J[0][0] * J[1][2] - J[1][0] * J[0][2]) / det;
+    s_B[q][2][0] = (J[1][0] * J[2][1] - J[2][0] * J[1][1]) / det;
+    s_B[q][2][1] = -(J[0][0] * J[2][1] - J[2][0] * J[0][1]) / det;
+    s_B[q][2][2] = (J[0][0] * J[1][1] - J[1][0] * J[0][1]) / det;
+    if (!(det > 0.0)) atomicOr(&A.flag[0], 1);
+    s_dA[q] = det * tb[T::WT + q];
+  } else if (tid == NQ) {
+    s_dA[NQ] = 0.0;  // the padding point carries no weight
+  } else if (EN && tid >= 64 && tid < 64 + nqs * NSP) {
+    // QSIDE (src/bc_integ.c:1518-1519, 2156-2157): w |surface Jacobian| q at
+    // one surface Gauss point of one side entry
+    const int t2 = tid - 64;
+    const int k = t2 / NSP, g = t2 - k * NSP;
+    const int sd = A.qs_side[qs0 + k] - 1;
+    const double *st = A.stab + (size_t)(sd * NSP + g) * SREC;
+    const int ax = (sd == 0 || sd == 2) ? 1 : (sd == 1 || sd == 3) ? 0 : 2;
+    int fr[2], nf = 0;
+#pragma unroll
+    for (int a = 0; a < DIM; a++)
+      if (a != ax) fr[nf++] = a;
+    double t1[DIM], tt2[DIM];
+#pragma unroll
+    for (int j = 0; j < DIM; j++) {
+      t1[j] = 0.0;
+      tt2[j] = 0.0;
+      for (int i = 0; i < NVE; i++) {
+        t1[j] += s_x[j][i] * st[NVE + i * DIM + fr[0]];
+        tt2[j] += s_x[j][i] * st[NVE + i * DIM + fr[1]];
+      }
+    }
+    const double c0 = t1[1] * tt2[2] - t1[2] * tt2[1], c1 = t1[2] * tt2[0] - t1[0] * tt2[2],
+                 c2 = t1[0] * tt2[1] - t1[1] * tt2[0];
+    s_qw[k][g] = st[NVE + NVE * DIM] * sqrt(c0 * c0 + c1 * c1 + c2 * c2) * A.qs_q[qs0 + k];
+    if (g == 0) s_qsd[k] = sd;
+  }
+  __syncthreads();
+
+  // 3. physical gradients into the padded operand tables (load_bf_grad)
+  for (int idx = tid; idx < DIM * KP * LD; idx += NT) {
+    const int p = idx / (KP * LD), r0 = idx - p * KP * LD, q = r0 / LD, i = r0 - q * LD;
+    double s = 0.0;
+    if (q < NQ && i < NVE) {
+#pragma unroll
+      for (int r = 0; r < DIM; r++) s += s_B[q][p][r] * tb[T::DPHI + (q * NVE + i) * DIM + r];
+    }
+    s_G[p][q][i] = s;
+  }
+  __syncthreads();
+
+  // 4. fields at the quadrature points (load_fv, load_fv_grads): eight
+  // threads per point, each summing a quarter of the nodes, then a
+  // butterfly over the eight
+  if (tid < NQ * 8) {
+    const int q = tid >> 3, sub = tid & 7;
+    double v[DIM] = {0.0, 0.0, 0.0}, gv[DIM][DIM], Tq = 0.0, gT[DIM] = {0.0, 0.0, 0.0};
+    double vd[DIM] = {0.0, 0.0, 0.0}, Td = 0.0;
+#pragma unroll
+    for (int a = 0; a < DIM; a++)
+#pragma unroll
+      for (int p = 0; p < DIM; p++) gv[p][a] = 0.0;
+#pragma unroll 1
+    for (int k = sub; k < NVE; k += 8) {
+      const double ph = s_Phi[q][k];
+      double g[DIM];
+#pragma unroll
+      for (int p = 0; p < DIM; p++) g[p] = s_G[p][q][k];
+#pragma unroll
+      for (int a = 0; a < DIM; a++) {
+        const double ua = s_u[a][k];
+        v[a] += ua * ph;
+        vd[a] += s_ud[a][k] * ph;
+#pragma unroll
+        for (int p = 0; p < DIM; p++) gv[p][a] += ua * g[p];
+      }
+      if constexpr (EN) {
+        const double tk = s_u[DIM][k];
+        Tq += tk * ph;
+        Td += s_ud[DIM][k] * ph;
+#pragma unroll
+        for (int p = 0; p < DIM; p++) gT[p] += tk * g[p];
+      }
+    }
+#pragma unroll
+    for (int o = 4; o > 0; o >>= 1) {
+#pragma unroll
+      for (int a = 0; a < DIM; a++) {
+        v[a] += __shfl_xor(v[a], o, 8);
+        vd[a] += __shfl_xor(vd[a], o, 8);
+        gT[a] += __shfl_xor(gT[a], o, 8);
+#pragma unroll
+        for (int p = 0; p < DIM; p++) gv[p][a] += __shfl_xor(gv[p][a], o, 8);
+      }
+      Tq += __shfl_xor(Tq, o, 8);
+      Td += __shfl_xor(Td, o, 8);
+    }
+    if (sub == 0) {
+      double P = 0.0;
+#pragma unroll
+      for (int k = 0; k < NP; k++) P += s_p[k] * tb[T::PSI + q * NP + k];
+      double mu = A.mu0, dmu = 0.0;
+      if constexpr (VISC == 1) {
+        // calc_shearrate + carreau_viscosity (src/mm_fill_util.c:4564,
+        // src/mm_viscosity.c:1065-1116)
+        double gd = 0.0;
+#pragma unroll
+        for (int a = 0; a < DIM; a++)
+#pragma unroll
+          for (int b = 0; b < DIM; b++) {
+            const double gab = gv[a][b] + gv[b][a];
+            gd += gab * gab;
+          }
+        gd = sqrt(0.5 * fabs(gd));
+        const bool nz = fabs(gd) > 1.0e-298;
+        const double val2 = nz ? pow(A.lam * gd, A.aexp) : 0.0;
+        mu = A.muinf + (A.mu0 - A.muinf) * pow(1.0 + val2, (A.nexp - 1.0) / A.aexp);
+        if (nz) {
+          const double val = pow(A.lam * gd, A.aexp - 1.0);
+          const double val1 = pow(1.0 + val2, (A.nexp - 1.0 - A.aexp) / A.aexp);
+          dmu = (A.mu0 - A.muinf) * (A.nexp - 1.0) * A.lam * val * val1 / gd;
+        }
+      }
+      double div = 0.0;
+#pragma unroll
+      for (int a = 0; a < DIM; a++) {
+        s_v[q][a] = v[a];
+        s_vd[q][a] = vd[a];
+        s_gT[q][a] = gT[a];
+        div += gv[a][a];
+#pragma unroll
+        for (int p = 0; p < DIM; p++) s_gv[q][p][a] = gv[p][a];
+        s_f[q][a] = EN ? A.rho * A.g[a] * (-A.beta * (Tq - A.Tref)) : 0.0;
+      }
+      s_P[q] = P;
+      s_mu[q] = mu;
+      s_dmu[q] = dmu;
+      s_div[q] = div;
+      s_Td[q] = Td;
+    }
+  } else if (tid == NT - 1) {
+    // the padding point: finite zeros for the weights read by the products
+#pragma unroll
+    for (int a = 0; a < DIM; a++) {
+      s_v[NQ][a] = 0.0;
+      s_vd[NQ][a] = 0.0;
+      s_gT[NQ][a] = 0.0;
+      s_f[NQ][a] = 0.0;
+#pragma unroll
+      for (int p = 0; p < DIM; p++) s_gv[NQ][p][a] = 0.0;
+    }
+    s_P[NQ] = s_mu[NQ] = s_dmu[NQ] = s_div[NQ] = s_Td[NQ] = 0.0;
+  }
+  __syncthreads();
+
+  const double rho = A.rho, rCp = A.rho * A.Cp, kc = A.kcond;
+  const double cm = A.x_old ? A.cmass : 0.0;
+  JT *__restrict__ out = (JT *)A.val;
+  const bool ow = A.overwrite != 0;
+  const unsigned short *__restrict__ rk = A.rank + (size_t)e * 2 * NVE2;
+
+  // 5a. residual rows
+  for (int t = tid; t < NV * NVE + NP; t += NT) {
+    int row;
+    double R = 0.0;
+    if (t < NV * NVE) {
+      const int a = t / NVE, i = t - a * NVE;
+      row = s_gun[a][i];
+      if (a < DIM) {
+#pragma unroll 1
+        for (int q = 0; q < NQ; q++) {
+          const double ph = s_Phi[q][i];
+          double adv = 0.0, dif = 0.0;
+#pragma unroll
+          for (int p = 0; p < DIM; p++) {
+            adv += s_v[q][p] * s_gv[q][p][a];
+            const double Pi_ap = (a == p ? -s_P[q] : 0.0) + s_mu[q] * (s_gv[q][a][p] + s_gv[q][p][a]);
+            dif += s_G[p][q][i] * Pi_ap;
+          }
+          R += s_dA[q] * (-rho * ph * adv - dif + ph * s_f[q][a] - rho * ph * s_vd[q][a]);
+        }
+      } else {
+#pragma unroll 1
+        for (int q = 0; q < NQ; q++) {
+          const double ph = s_Phi[q][i];
+          double adv = 0.0, dif = 0.0;
+#pragma unroll
+          for (int p = 0; p < DIM; p++) {
+            adv += s_v[q][p] * s_gT[q][p];
+            dif += s_G[p][q][i] * s_gT[q][p];
+          }
+          R += s_dA[q] * (-rCp * ph * adv - kc * dif + ph * A.hsrc - rCp * ph * s_Td[q]);
+        }
+        for (int k = 0; k < nqs; k++)
+          for (int g = 0; g < NSP; g++) R += s_qw[k][g] * A.stab[(size_t)(s_qsd[k] * NSP + g) * SREC + i];
+      }
+      if (s_dir[a][i]) R = s_u[a][i] - A.dbc_val[row];
+    } else {
+      const int k = t - NV * NVE;
+      row = s_gp[k];
+#pragma unroll 1
+      for (int q = 0; q < NQ; q++) R += s_dA[q] * tb[T::PSI + q * NP + k] * s_div[q];
+      if (s_dirp[k]) R = s_p[k] - A.dbc_val[row];
+    }
+    A.resid[row] += R;
+  }
+
+  if (A.resid_only) return;
+
+  // 5b. nodal blocks on the matrix cores: wave w computes rows 16 (w >> 1) ..
+  // +15 and columns 16 (w & 1) .. +15 of every block
+  {
+    const int lane = tid & 63, wave = tid >> 6;
+    const int r16 = lane & 15, kk = lane >> 4;
+    const int ii = 16 * (wave >> 1) + r16;  // row operand index (A)
+    const int jj = 16 * (wave & 1) + r16;   // column operand index (B)
+#define GM_MFMA64(a, b, c) __builtin_amdgcn_mfma_f64_16x16x4f64((a), (b), (c), 0, 0, 0)
+    const d4v z = {0.0, 0.0, 0.0, 0.0};
+    // shared products, four independent chains per k-step: E(phi, dA, phi),
+    // E(phi, dA, v.G), sum_p E(G_p, dA, G_p) and (Carreau) its mu-weighted
+    // form, folded into the diagonal-block terms
+    d4v A1 = z, Dadv = z, L1 = z, Lmu = z;
+#pragma unroll 1
+    for (int q0 = 0; q0 < KP; q0 += 4) {
+      const int q = q0 + kk;
+      const double dA = s_dA[q];
+      const double phA = s_Phi[q][ii] * dA, phB = s_Phi[q][jj];
+      double vB = 0.0;
+#pragma unroll
+      for (int p = 0; p < DIM; p++) vB += s_v[q][p] * s_G[p][q][jj];
+      A1 = GM_MFMA64(phA, phB, A1);
+      Dadv = GM_MFMA64(phA, vB, Dadv);
+#pragma unroll
+      for (int p = 0; p < DIM; p++) {
+        const double gA = s_G[p][q][ii] * dA, gB = s_G[p][q][jj];
+        L1 = GM_MFMA64(gA, gB, L1);
+        if constexpr (VISC == 1) Lmu = GM_MFMA64(gA * s_mu[q], gB, Lmu);
+      }
+    }
+    if constexpr (VISC == 0) Lmu = A.mu0 * L1;
+    const d4v Dm = -(rho * Dadv + Lmu + (rho * cm) * A1);
+    const d4v Tt = -(rCp * Dadv + kc * L1 + (rCp * cm) * A1);
+
+    // this lane's rows i = 16 (wave >> 1) + kk + 4 r, column j = jj: rank
+    // words of the four node pairs
+    const int j = jj;
+    unsigned rmw[4], rew[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int i = 16 * (wave >> 1) + kk + 4 * r;
+      const bool in = i < NVE && j < NVE;
+      rmw[r] = in ? rk[i * NVE + j] : 0u;
+      rew[r] = (in && EN) ? rk[NVE2 + i * NVE + j] : 0u;
+    }
+    // scatter of one block row a (put_dirichlet_in_matrix, load_lec)
+    auto scatter_row = [&](int a, const d4v *Jr) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = 16 * (wave >> 1) + kk + 4 * r;
+        if (i >= NVE || j >= NVE) continue;
+        const int t = i * NVE + j;
+        const bool first = ow && (rmw[r] & 0x8000u);
+        const int row = s_gun[a][i];
+        const bool dir = s_dir[a][i] != 0;
+        const int base = s_brow[a][i] + (a < DIM ? (int)(rmw[r] & 0x7fffu) : (int)rew[r]);
+#pragma unroll
+        for (int b = 0; b < NV; b++) {
+          double v = Jr[b][r];
+          if (A.kel) A.kel[(size_t)e * A.E + (size_t)(a * NV + b) * NVE2 + t] = v;
+          if (dir) {
+            if (a == b && i == j) v = 1.0;
+            else if (first) v = 0.0;
+            else continue;
+          }
+          const int col = s_gun[b][j];
+          const long long pos = row == col ? row : base + b - (col > row ? 1 : 0);
+          if (first) out[pos] = (JT)v;
+          else out[pos] += (JT)v;
+        }
+      }
+    };
+    // block row a of the momentum equation: three independent chains (b)
+#pragma unroll 1
+    for (int a = 0; a < DIM; a++) {
+      d4v Jr[NV];
+#pragma unroll
+      for (int b = 0; b < DIM; b++) Jr[b] = a == b ? Dm : z;
+#pragma unroll 1
+      for (int q0 = 0; q0 < KP; q0 += 4) {
+        const int q = q0 + kk;
+        const double dA = s_dA[q];
+        const double phA = s_Phi[q][ii] * (-rho * dA), phB = s_Phi[q][jj];
+        const double gaB = s_G[a][q][jj];
+        const double wmu = -dA * (VISC ? s_mu[q] : A.mu0);
+        double hA = 0.0;
+        if constexpr (VISC == 1) {
+          // Carreau: (d mu / d gammadot) / gammadot (G_i . gamma_a) (g_j . gamma_b)
+#pragma unroll
+          for (int p = 0; p < DIM; p++) hA += s_G[p][q][ii] * (s_gv[q][a][p] + s_gv[q][p][a]);
+          hA *= -dA * s_dmu[q];
+        }
+#pragma unroll
+        for (int b = 0; b < DIM; b++) {
+          // advection through d_b v_a, then mu g_j,a G_i,b
+          Jr[b] = GM_MFMA64(phA * s_gv[q][b][a], phB, Jr[b]);
+          Jr[b] = GM_MFMA64(s_G[b][q][ii] * wmu, gaB, Jr[b]);
+          if constexpr (VISC == 1) {
+            double hB = 0.0;
+#pragma unroll
+            for (int p = 0; p < DIM; p++) hB += s_G[p][q][jj] * (s_gv[q][b][p] + s_gv[q][p][b]);
+            Jr[b] = GM_MFMA64(hA, hB, Jr[b]);
+          }
+        }
+      }
+      if constexpr (EN) Jr[DIM] = (-rho * A.g[a] * A.beta) * A1;
+      scatter_row(a, Jr);
+    }
+    if constexpr (EN) {
+      d4v Jr[NV];
+#pragma unroll
+      for (int b = 0; b < DIM; b++) Jr[b] = z;
+#pragma unroll 1
+      for (int q0 = 0; q0 < KP; q0 += 4) {
+        const int q = q0 + kk;
+        const double phA = s_Phi[q][ii] * (-rCp * s_dA[q]), phB = s_Phi[q][jj];
+#pragma unroll
+        for (int b = 0; b < DIM; b++) Jr[b] = GM_MFMA64(phA * s_gT[q][b], phB, Jr[b]);
+      }
+      Jr[DIM] = Tt;
+      scatter_row(DIM, Jr);
+    }
+#undef GM_MFMA64
+  }
+
+  // 5c. momentum x pressure (J_m_P): int grad_a phi_i psi_k dA
+  for (int t = tid; t < NVE * NP; t += NT) {
+    const int i = t / NP, k = t - i * NP;
+    double acc[DIM] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int q = 0; q < NQ; q++) {
+      const double w = s_dA[q] * tb[T::PSI + q * NP + k];
+#pragma unroll
+      for (int a = 0; a < DIM; a++) acc[a] += w * s_G[a][q][i];
+    }
+    const unsigned rm = rk[i * NVE + CL];
+    const bool first = ow && (rm & 0x8000u);
+    const int col = s_gp[k];
+#pragma unroll
+    for (int a = 0; a < DIM; a++) {
+      if (A.kel) A.kel[(size_t)e * A.E + A.off_mp + a * NVE * NP + t] = acc[a];
+      double v = acc[a];
+      if (s_dir[a][i]) {
+        if (!first) continue;
+        v = 0.0;
+      }
+      const int row = s_gun[a][i];
+      const long long pos = s_brow[a][i] + (int)(rm & 0x7fffu) + NV + k - (col > row ? 1 : 0);
+      if (first) out[pos] = (JT)v;
+      else out[pos] += (JT)v;
+    }
+  }
+
+  // 5d. continuity x velocity (J_c_v): int psi_k grad_b phi_j dA
+  for (int t = tid; t < NP * NVE; t += NT) {
+    const int k = t / NVE, j = t - k * NVE;
+    const bool dir = s_dirp[k] != 0;
+    double acc[DIM] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int q = 0; q < NQ; q++) {
+      const double w = s_dA[q] * tb[T::PSI + q * NP + k];
+#pragma unroll
+      for (int b = 0; b < DIM; b++) acc[b] += w * s_G[b][q][j];
+    }
+    const unsigned rm = rk[CL * NVE + j];
+    const bool first = ow && (rm & 0x8000u);
+    const int row = s_gp[k];
+#pragma unroll
+    for (int b = 0; b < DIM; b++) {
+      if (A.kel) A.kel[(size_t)e * A.E + A.off_cv + b * NP * NVE + t] = acc[b];
+      double v = acc[b];
+      if (dir) {
+        if (!first) continue;
+        v = 0.0;
+      }
+      const int col = s_gun[b][j];
+      const long long pos = s_bp[k] + (int)(rm & 0x7fffu) + b - (col > row ? 1 : 0);
+      if (first) out[pos] = (JT)v;
+      else out[pos] += (JT)v;
+    }
+    // continuity x temperature: in the graph (the continuity rows couple to
+    // every unknown of the element's nodes) but zero in the weak form
+    if (EN && first) {
+      const int col = s_gun[DIM][j];
+      out[s_bp[k] + (int)(rm & 0x7fffu) + DIM - (col > row ? 1 : 0)] = (JT)0.0;
+    }
+  }
+
+  // 5e. continuity x pressure: zero for Galerkin Q2-P1, unit diagonal on a
+  // Dirichlet (datum) row
+  if (tid < NP * NP) {
+    const int k = tid / NP, k2 = tid - k * NP;
+    if (A.kel) A.kel[(size_t)e * A.E + A.off_cp + tid] = 0.0;
+    const unsigned rm = rk[CL * NVE + CL];
+    const bool first = ow && (rm & 0x8000u);
+    const double v = (k == k2 && s_dirp[k]) ? 1.0 : 0.0;
+    const long long pos = k == k2 ? (long long)s_gp[k] : s_bp[k] + (int)(rm & 0x7fffu) + NV + k2 - (k2 > k ? 1 : 0);
+    if (first) out[pos] = (JT)v;
+    else if (v != 0.0) out[pos] += (JT)v;
+  }
+}
+
+// Rank map of every element: for node pair (i, j) (local order, i row node),
+//   rank[e][0][i*nve + j] = number of columns below fu[n_j] in the full column
+//     set (off-diagonals and the diagonal) of the momentum rows of node n_i,
+//     with bit 15 set when e is the first element, in colour order, holding
+//     both nodes (filled in by k_first_touch);
+//   rank[e][1][i*nve + j] = the same for the energy row (no pressure columns).
+// Every momentum row of a node has the same full column set, so the off-
+// diagonal position of column fu[n_j] + b in row fu[n_i] + a is
+// bindx[row] + rank + b - (col > row).  Continuity rows live on the centroid
+// node, whose neighbours are the element's own nodes, so they share the
+// column set of the centroid's momentum rows.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_build_ranks(const int *__restrict__ conn,
+                                                     const int *__restrict__ fu,
+                                                     const int *__restrict__ bindx, int en,
+                                                     unsigned short *__restrict__ rank, int *flag) {
+  constexpr int NVE = El<DIM>::NVE, NVE2 = NVE * NVE;
+  const int e = blockIdx.x;
+  __shared__ int s_f[NVE];
+  if (threadIdx.x < NVE) s_f[threadIdx.x] = fu[conn[(size_t)e * NVE + threadIdx.x]];
+  __syncthreads();
+  for (int t = threadIdx.x; t < NVE2; t += blockDim.x) {
+    const int i = t / NVE, j = t - i * NVE;
+    const int col = s_f[j];
+    for (int w = 0; w < 1 + en; w++) {
+      const int row = s_f[i] + (w ? DIM : 0);
+      // lower bound of col in the sorted off-diagonal list of row
+      int lo = bindx[row], hi = bindx[row + 1];
+      const int start = lo;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (bindx[mid] < col) lo = mid + 1;
+        else hi = mid;
+      }
+      const int r = lo - start + (row < col ? 1 : 0);
+      if (r > 0x7fff) atomicOr(&flag[1], 1);
+      rank[(size_t)e * 2 * NVE2 + w * NVE2 + t] = (unsigned short)r;
+    }
+    if (!en) rank[(size_t)e * 2 * NVE2 + NVE2 + t] = 0;
+  }
+}
+
+// First touch, one element colour per launch (colours in order): the element
+// of this colour holding both nodes of a pair marks it when no earlier colour
+// has; `touched` is indexed by the MSR position of the pair's (U1, U1) entry.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_first_touch(const int *__restrict__ elems,
+                                                     const int *__restrict__ conn,
+                                                     const int *__restrict__ fu,
+                                                     const int *__restrict__ bindx,
+                                                     unsigned short *__restrict__ rank,
+                                                     unsigned char *__restrict__ touched,
+                                                     int *flag) {
+  constexpr int NVE = El<DIM>::NVE, NVE2 = NVE * NVE;
+  const int e = elems[blockIdx.x];
+  __shared__ int s_f[NVE];
+  if (threadIdx.x < NVE) s_f[threadIdx.x] = fu[conn[(size_t)e * NVE + threadIdx.x]];
+  __syncthreads();
+  for (int t = threadIdx.x; t < NVE2; t += blockDim.x) {
+    const int i = t / NVE, j = t - i * NVE;
+    const int row = s_f[i], col = s_f[j];
+    unsigned short *rp = rank + (size_t)e * 2 * NVE2 + t;
+    const int r = *rp & 0x7fff;
+    long long pos = row;
+    if (row != col) {
+      pos = (long long)bindx[row] + r - (col > row ? 1 : 0);
+      if (pos < bindx[row] || pos >= bindx[row + 1] || bindx[pos] != col) {
+        atomicOr(&flag[1], 1);
+        continue;
+      }
+    }
+    if (!touched[pos]) {
+      touched[pos] = 1;
+      *rp = (unsigned short)(r | 0x8000);
+    }
+  }
+}
+
 // Element-to-nonzero map: for every local entry of every element, the MSR
 // index of the global (row, column) pair; the diagonal lives at index row.
 template <int DIM>
@@ -726,12 +1331,42 @@ int gm_fill_setup_device(gm_ctx *c) {
     gm_set_error("element-to-nonzero map: entry missing from the MSR graph");
     return GM_ERR_ARG;
   }
+  if (c->dim == 3) {
+    // rank map and first touches (the HEX27 assembly kernel's scatter)
+    const size_t nr = ne * 2 * (size_t)L.nve * L.nve;
+    GM_HIP(hipMalloc(&c->d_rank, nr * sizeof(unsigned short)));
+    k_build_ranks<3><<<c->num_elems, 256, 0, c->stream>>>(c->d_conn, c->d_fu, c->d_bindx, c->energy,
+                                                         c->d_rank, c->d_flag);
+    GM_HIP(hipGetLastError());
+    unsigned char *touched = nullptr;
+    GM_HIP(hipMalloc(&touched, (size_t)c->nval));
+    GM_HIP(hipMemsetAsync(touched, 0, (size_t)c->nval, c->stream));
+    for (int col = 0; col < c->n_ecol; col++) {
+      const int cnt = c->ecol_ptr[col + 1] - c->ecol_ptr[col];
+      if (cnt)
+        k_first_touch<3><<<cnt, 256, 0, c->stream>>>(c->d_ecol_elems + c->ecol_ptr[col], c->d_conn,
+                                                    c->d_fu, c->d_bindx, c->d_rank, touched,
+                                                    c->d_flag);
+    }
+    const hipError_t le = hipGetLastError();
+    GM_HIP(hipMemcpyAsync(flag, c->d_flag, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    GM_HIP(hipStreamSynchronize(c->stream));
+    (void)hipFree(touched);
+    GM_HIP(le);
+    if (flag[1]) {
+      gm_set_error("rank map: a node pair's position does not match the MSR graph");
+      return GM_ERR_ARG;
+    }
+  }
   return GM_OK;
 }
 
 template <int DIM, int EN, int VISC, typename JT>
 static void launch_fill(gm_ctx *c, FillArgs &a) {
   constexpr int NT = DIM == 2 ? 128 : 256;
+  void (*kern)(FillArgs, const int *);
+  if constexpr (DIM == 3) kern = k_fill3m<EN, VISC, JT>;
+  else kern = k_fill<DIM, EN, VISC, JT, NT>;
   for (int col = 0; col < c->n_ecol; col++) {
     const int cnt = c->ecol_ptr[col + 1] - c->ecol_ptr[col];
     if (cnt == 0) continue;
@@ -741,8 +1376,7 @@ static void launch_fill(gm_ctx *c, FillArgs &a) {
                         8.0 * DIM * c->num_nodes + 4.0 * El<DIM>::NVE * c->num_elems;
     const int id = a.resid_only ? GM_K_FILL_RESID : GM_K_FILL;
     gm_launch(c, id, share * (a.resid_only ? basm - (double)(c->nval - 1) * sizeof(JT) : basm),
-              k_fill<DIM, EN, VISC, JT, NT>, dim3(cnt), dim3(NT), 0, c->stream, a,
-              c->d_ecol_elems + c->ecol_ptr[col]);
+              kern, dim3(cnt), dim3(NT), 0, c->stream, a, c->d_ecol_elems + c->ecol_ptr[col]);
   }
 }
 
@@ -784,6 +1418,9 @@ int gm_fill_launch_ex(gm_ctx *c, const double *x, double *resid, double *val, fl
   a.qs_side = c->d_qs_side;
   a.qs_q = c->d_qs_q;
   a.stab = c->d_stab;
+  a.bindx = c->d_bindx;
+  a.rank = c->d_rank;
+  a.overwrite = c->fill_overwrite && c->d_rank ? 1 : 0;
   a.num_nodes = c->num_nodes;
   a.E = c->lay.total;
   a.off_mp = c->lay.off_mp;

@@ -146,6 +146,9 @@ struct gm_ctx {
   int *d_flag = nullptr;        // device error flags
   double *d_tables = nullptr;   // reference basis tables
   double *d_stab = nullptr;     // side basis tables (surface integrals)
+  unsigned short *d_rank = nullptr;  // HEX27: per element and node pair, the rank of the
+                                     // column node in the row node's rows (+ first-touch bit)
+  int fill_overwrite = 0;       // the next assembly stores first touches instead of adding
 
   // theta-method state of the next assemblies (tr_x_old == nullptr: steady):
   // xdot = tr_cmass (x - x_old) - tr_cold xdot_old

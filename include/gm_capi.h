@@ -179,6 +179,12 @@ int gm_set_physics(gm_ctx *ctx, const gm_physics *phys);
  * doubles).  Returns GM_OK or GM_ERR_ELEM. */
 int gm_matrix_fill_full_dev(gm_ctx *ctx, const double *x_dev, double *resid_dev, double *val_dev);
 int gm_matrix_fill_full(gm_ctx *ctx, const double *x, double *resid, double *val);
+/* The same assembly, with the Jacobian stored rather than accumulated: val
+ * needs no zeroing beforehand (the HEX27 kernel writes each entry's first
+ * element contribution and adds the others; QUAD9 zeroes val first).  resid
+ * is still accumulated.  Saves the caller's pass over val (8 GB at 48^3). */
+int gm_matrix_fill_full_assign_dev(gm_ctx *ctx, const double *x_dev, double *resid_dev,
+                                   double *val_dev);
 /* The same assembly with the Jacobian values in FP32 (the mixed-precision
  * path, GM_PREC_MIXED): val32 holds gm_msr_size floats and is accumulated
  * (+=); the residual stays FP64.  Replaces matrix_fill_full

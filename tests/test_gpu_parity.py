@@ -90,6 +90,16 @@ def test_fill_matches_oracle(name, capi, oracle_mod):
     # accumulate semantics: a second fill doubles both arrays
     assert ctx.fill_dev(xd.data_ptr(), rd.data_ptr(), vd.data_ptr()) == 0
     assert np.abs(vd.cpu().numpy() - 2 * v_ref).max() <= 4e-12 * np.abs(v_ref).max()
+    # stored, not accumulated: every entry written over a NaN-filled array
+    # (val[N] is not part of the MSR matrix and is never written)
+    vn = torch.full((ctx.nval,), float("nan"), dtype=torch.float64, device="cuda")
+    rn = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    assert ctx.fill_assign_dev(xd.data_ptr(), rn.data_ptr(), vn.data_ptr()) == 0
+    va = vn.cpu().numpy()
+    va[ctx.N] = v_ref[ctx.N]
+    assert np.isfinite(va).all()
+    assert np.abs(va - v_ref).max() <= 1e-12 * np.abs(v_ref).max()
+    np.testing.assert_array_equal(rn.cpu().numpy(), r)
     # host-pointer drop-in
     r2, v2 = np.zeros(ctx.N), np.zeros(ctx.nval)
     assert ctx.fill_host(x, r2, v2) == 0
@@ -111,8 +121,10 @@ def test_fill_matches_oracle(name, capi, oracle_mod):
     ctx.close()
 
 
-def test_fill_determinism(capi):
-    prob = problems.natural_convection(n=8, Ra=1e4)
+@pytest.mark.parametrize("dim", [2, 3])
+def test_fill_determinism(dim, capi):
+    prob = (problems.natural_convection(n=8, Ra=1e4) if dim == 2 else
+            problems.lid_cavity(n=4, Re=400.0, dim=3, energy=True, Ra=1e4, jitter=0.1))
     ctx = capi.Context(prob)
     x = _dev(np.random.default_rng(3).standard_normal(ctx.N))
     outs = []

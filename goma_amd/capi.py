@@ -137,7 +137,7 @@ EXPORTS = [
     "gm_mlp_set_basis", "gm_mlp_clear_basis", "gm_matrix_fill_full_transient_dev",
     "gm_add_qside", "gm_clear_qside", "gm_solve_nonlinear_transient_dev", "gm_solve_transient_dev",
     "gm_num_nodes", "gm_num_elems", "gm_mg_build", "gm_mg_level", "gm_mg_copy_prolongation",
-    "gm_matrix_fill_full_assign_dev",
+    "gm_matrix_fill_full_assign_dev", "gm_mlp_predict_noalloc_dev",
 ]
 
 _lib = None
@@ -213,6 +213,7 @@ def lib():
     L.gm_mlp_loss_grad.argtypes = [vp, ip, vp, vp, vp, C.POINTER(C.c_double), vp]
     L.gm_mlp_predict_dev.argtypes = [vp, C.c_longlong, vp, vp]
     L.gm_mlp_predict.argtypes = [vp, C.c_longlong, vp, vp]
+    L.gm_mlp_predict_noalloc_dev.argtypes = [vp, C.c_longlong, vp, vp]
     L.gm_mlp_guess_dev.argtypes = [vp, vp, C.POINTER(GmMlpQuery), C.POINTER(GmMlpQuery), vp, vp]
     L.gm_mlp_set_basis.argtypes = [vp, vp, ip, vp, vp]
     L.gm_mlp_clear_basis.argtypes = [vp]

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "gm_launch.h"
@@ -402,10 +403,48 @@ __global__ void k_mlp_qapply(int N, int nb, const double *__restrict__ Q, const 
   x[i] = xprev[i] + (w[i] > 0.0 ? s / w[i] : 0.0);
 }
 
+// ---------------------------------------------- allocation-free inference
+// Scratch of gm_mlp_predict_noalloc_dev, one copy per GPU for the life of
+// the process: two BF16 activation buffers of NA_PTS points x NA_W features
+// (ping-pong through the layers) and the FP32 outputs, NA_PTS x OUT_PAD.
+// Points go through it in chunks of NA_PTS.  Every call on a GPU shares that
+// GPU's copy: calls that overlap on one GPU race.  Each chunk overwrites every
+// entry it reads (inputs and their zero padding, each layer's full output),
+// so nothing needs clearing between calls.
+static constexpr int NA_PTS = 2048, NA_W = 512;
+static constexpr size_t NA_ACT = (size_t)NA_PTS * NA_W * sizeof(bf16_t);
+static constexpr size_t NA_BYTES = 2 * NA_ACT + (size_t)NA_PTS * OUT_PAD * sizeof(float);
+static_assert(NA_PTS % PT == 0 && NA_W % MT == 0, "scratch shape");
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+static constexpr int NA_MAX_DEV = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[NA_MAX_DEV] = {};  // device address of `scratch` on each GPU
+
+// Device address of this GPU's scratch, taken once per device with that
+// device current (host code must not pass the symbol itself to a kernel).
+static int scratch_init(int dev) {
+  if (dev < 0 || dev >= NA_MAX_DEV) {
+    gm_set_error("device %d outside the %d devices the inference scratch tracks", dev, NA_MAX_DEV);
+    return GM_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return GM_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    gm_set_error("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    return GM_ERR_HIP;
+  }
+  g_scratch_ptr[dev] = p;
+  return GM_OK;
+}
+
 // ------------------------------------------------------------------ host side
 struct gm_mlp {
   gm_mlp_shape s{};
   int L = 0;
+  int dev = -1;  // the GPU the network lives on (current device at creation)
   std::vector<int> K, N, Kr, Nr;
   hipStream_t stream = nullptr;
   std::vector<float *> W, b, mW, vW, mb, vb, gW, gb;
@@ -611,6 +650,9 @@ static int create_impl(const gm_mlp_shape *s, gm_mlp *m) {
   }
   m->s = *s;
   m->L = s->hidden + 1;
+  GM_HIP(hipGetDevice(&m->dev));
+  int rc0 = scratch_init(m->dev);
+  if (rc0) return rc0;
   for (int l = 0; l < m->L; l++) {
     m->Kr.push_back(l == 0 ? s->n_in : s->width);
     m->K.push_back(l == 0 ? IN_PAD : s->width);
@@ -862,6 +904,45 @@ extern "C" int gm_mlp_predict_dev(gm_mlp *m, long long npts, const float *in_dev
     gather(m, M, Mpad, nullptr, in_dev + off * m->s.n_in, nullptr, nullptr);
     forward(m, Mpad);
     k_mlp_copy_out<<<(M * m->s.n_out + 255) / 256, 256, 0, m->stream>>>(M, m->s.n_out, m->y,
+                                                                        out_dev + off * m->s.n_out);
+  }
+  GM_HIP(hipGetLastError());
+  GM_HIP(hipStreamSynchronize(m->stream));
+  return GM_OK;
+}
+
+extern "C" int gm_mlp_predict_noalloc_dev(gm_mlp *m, long long npts, const float *in_dev,
+                                          float *out_dev) {
+  if (!m || npts < 0 || (npts && (!in_dev || !out_dev))) return GM_ERR_ARG;
+  if (maxdim(m) > NA_W) {
+    gm_set_error("gm_mlp_predict_noalloc_dev: layer width %d exceeds the scratch's %d features",
+                 maxdim(m), NA_W);
+    return GM_ERR_ARG;
+  }
+  void *base = (m->dev >= 0 && m->dev < NA_MAX_DEV) ? g_scratch_ptr[m->dev] : nullptr;
+  if (!base) {
+    gm_set_error("gm_mlp_predict_noalloc_dev: no scratch address for device %d", m->dev);
+    return GM_ERR_HIP;
+  }
+  bf16_t *buf[2] = {(bf16_t *)base, (bf16_t *)((char *)base + NA_ACT)};
+  float *y = (float *)((char *)base + 2 * NA_ACT);
+  for (long long off = 0; off < npts; off += NA_PTS) {
+    const int M = (int)std::min<long long>(NA_PTS, npts - off);
+    const int Mpad = (M + PT - 1) / PT * PT;
+    const long long total = (long long)Mpad * IN_PAD;
+    const int blocks = (int)std::min<long long>((total + 255) / 256, 4096);
+    k_mlp_gather<<<blocks, 256, 0, m->stream>>>(M, Mpad, nullptr, in_dev + off * m->s.n_in, nullptr,
+                                                nullptr, m->s.n_in, m->s.n_out, in_norm(m), buf[0],
+                                                nullptr, nullptr);
+    for (int l = 0; l < m->L; l++) {
+      if (l < m->L - 1)
+        gemm<EPI_TANH>(m->stream, Mpad, m->N[l], m->K[l], buf[l & 1], m->Wb[l], m->b[l], nullptr,
+                       buf[(l + 1) & 1], nullptr);
+      else
+        gemm<EPI_LIN>(m->stream, Mpad, m->N[l], m->K[l], buf[l & 1], m->Wb[l], m->b[l], nullptr,
+                      nullptr, y);
+    }
+    k_mlp_copy_out<<<(M * m->s.n_out + 255) / 256, 256, 0, m->stream>>>(M, m->s.n_out, y,
                                                                         out_dev + off * m->s.n_out);
   }
   GM_HIP(hipGetLastError());

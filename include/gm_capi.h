@@ -414,6 +414,17 @@ int gm_mlp_loss_grad(gm_mlp *m, int npts, const float *in, const float *targ, co
 /* y [npts][n_out] = network(in [npts][n_in]); device or host arrays. */
 int gm_mlp_predict_dev(gm_mlp *m, long long npts, const float *in_dev, float *out_dev);
 int gm_mlp_predict(gm_mlp *m, long long npts, const float *in, float *out);
+/* Allocation-free gm_mlp_predict_dev: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Activations and
+ * outputs pass through a scratch array the library keeps on each GPU for the
+ * life of the process (two 2048-point x 512-feature BF16 buffers and
+ * 2048 x 64 FP32 outputs, 4.5 MiB), 2048 points at a time, so any npts is
+ * accepted; a network with a layer wider than 512 features is refused with
+ * GM_ERR_ARG.  Calls on one GPU share its scratch: calls that overlap on one
+ * GPU race.  Makes no hipMalloc or hipFree call.  Same outputs as
+ * gm_mlp_predict_dev, bit for bit. */
+int gm_mlp_predict_noalloc_dev(gm_mlp *m, long long npts, const float *in_dev, float *out_dev);
 
 /* One query of the network for a context: the parameter inputs that follow
  * the coordinates, and the affine map from network outputs to fields. */

@@ -137,7 +137,7 @@ EXPORTS = [
     "gm_mlp_set_basis", "gm_mlp_clear_basis", "gm_matrix_fill_full_transient_dev",
     "gm_add_qside", "gm_clear_qside", "gm_solve_nonlinear_transient_dev", "gm_solve_transient_dev",
     "gm_num_nodes", "gm_num_elems", "gm_mg_build", "gm_mg_level", "gm_mg_copy_prolongation",
-    "gm_matrix_fill_full_assign_dev", "gm_mlp_predict_noalloc_dev",
+    "gm_matrix_fill_full_assign_dev", "gm_mlp_predict_noalloc_dev", "gm_mg_copy_block_inverse",
 ]
 
 _lib = None
@@ -187,6 +187,7 @@ def lib():
     L.gm_mg_copy_prolongation.argtypes = [vp, ip, vp, vp, vp]
     L.gm_mg_setup_dev.argtypes = [vp, vp, vp, vp]
     L.gm_mg_copy_level_matrix.argtypes = [vp, ip, vp]
+    L.gm_mg_copy_block_inverse.argtypes = [vp, ip, ip, vp]
     L.gm_prof_enable.argtypes = [vp, ip]
     L.gm_prof_get.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.gm_solve_nonlinear_guess_dev.argtypes = [vp, vp, vp, C.POINTER(GmSolverOpts), ip,
@@ -459,6 +460,14 @@ class Context:
         out = np.zeros(self._coarse[level - 1].nval)
         _check(lib().gm_mg_copy_level_matrix(self.h, level, out.ctypes.data), "gm_mg_copy_level_matrix")
         return out
+
+    def mg_block_inverse(self, level: int, elem: int) -> np.ndarray:
+        """(A_e^-1) of element ``elem`` on multigrid level ``level`` (0 = this
+        mesh) from the last set-up, as an n x n array."""
+        out = np.zeros(128 * 128)  # the largest block: 112 unknowns (HEX27 with energy)
+        n = _check(lib().gm_mg_copy_block_inverse(self.h, level, elem, out.ctypes.data),
+                   "gm_mg_copy_block_inverse")
+        return out[: n * n].reshape(n, n).T.copy()
 
     def mg_setup_dev(self, x_ptr, val_ptr, scale_ptr=None):
         _check(lib().gm_mg_setup_dev(self.h, x_ptr, val_ptr, scale_ptr), "gm_mg_setup_dev")

@@ -17,6 +17,13 @@ sensitivity from J dx/dRa = -dR/dRa at the converged x_n, dR/dRa by central
 differences of the residual (soln_sens, src/mm_sol_nonlinear.c:3638-3930).
 
 Steps are uniform in log10 Ra, the natural spacing of the sweep.
+
+``continue_problem`` is the adaptive form of the same loop: the path
+parameter s = log10 Ra advances by a step that path_step_control
+(src/rf_util.c:1312-1365) rescales after every step that converged at the
+first try, from the number of Newton passes it took, between Delta_s_min and
+Delta_s_max (ac_conti.c:1091-1100); a failed step is halved and retried
+(ac_conti.c:968-1045).
 """
 from __future__ import annotations
 
@@ -174,3 +181,94 @@ def sweep(ctx, prob, ras, opts: dict, x=None, max_retries: int = 5, keep: bool =
         if not ok:
             break
     return steps, sols
+
+
+def path_step_control(delta_s_old: float, inewton: int, max_newton: int) -> float:
+    """Goma's path_step_control (src/rf_util.c:1312-1365): the next step from
+    the last one and the Newton passes it needed, aiming at half of
+    Max_Newton_Steps.  beta = (2^(d - inewton) - 1) / d with d = max_newton // 2;
+    beta > 1 gives the largest growth, 3; otherwise the factor is 10^beta,
+    clipped to [0.5, 3]."""
+    d = max_newton // 2
+    if d < 1:
+        return delta_s_old
+    beta = (2.0 ** float(d - inewton) - 1.0) / d
+    beta = 3.0 if beta > 1.0 else 10.0 ** beta
+    return min(max(beta, 0.5), 3.0) * delta_s_old
+
+
+@dataclass
+class PathStep:
+    Ra: float
+    delta_s: float      # log10 Ra increment of the accepted step
+    newton_its: int
+    tries: int          # attempts at this step (1: converged at once)
+    seconds: float
+
+
+def continue_problem(ctx, prob, ra_start: float, ra_end: float, delta_s: float, opts: dict,
+                     delta_s_min: float = 1e-3, delta_s_max: float = 0.5, order: int = 1,
+                     x=None, max_steps: int = 200, log=None):
+    """Adaptive natural-parameter continuation in s = log10 Ra from ra_start
+    to ra_end (continue_problem, src/ac_conti.c:106-1100, steady branch).
+
+    The first solve is at ra_start from ``x`` (zero if omitted).  Each step
+    predicts the next state (zeroth order: the last solution; first order:
+    plus the sensitivity times the increment), solves, and on failure halves
+    the step and retries from the last solution, giving up after five tries
+    or below delta_s_min.  After a step that converged at its first try, the
+    next increment is path_step_control's, capped at delta_s_max.  Returns
+    (accepted steps, converged: bool); ``x`` holds the last solution.
+    """
+    import torch
+
+    if x is None:
+        x = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    s, s_end = float(np.log10(ra_start)), float(np.log10(ra_end))
+    sign = 1.0 if s_end >= s else -1.0
+    max_newton = int(opts.get("max_newton", 25))
+    rep, dt = _solve(ctx, prob, x, ra_start, opts)
+    steps = [PathStep(ra_start, 0.0, rep.newton_its, 1, dt)]
+    if not rep.converged:
+        return steps, False
+    last = x.clone()
+    tan = sensitivity(ctx, prob, last, ra_start, opts) if order == 1 else None
+    ds = abs(delta_s)
+    n = 0
+    while sign * (s_end - s) > 1e-12 and n < max_steps:
+        ds = min(ds, abs(s_end - s))
+        tries, secs = 0, 0.0
+        while True:
+            tries += 1
+            s1 = s + sign * ds
+            if abs(s1 - s_end) < 1e-12:
+                s1 = s_end
+            ra0, ra1 = 10.0 ** s, 10.0 ** s1
+            x.copy_(last)
+            if tan is not None:
+                x.add_(tan, alpha=ra1 - ra0)
+            rep, dt = _solve(ctx, prob, x, ra1, opts)
+            secs += dt
+            if rep.converged:
+                break
+            if tries > 5:
+                return steps, False
+            ds *= 0.5
+            if log:
+                log(f"[conti] Ra={ra1:.4g}: Newton failed, step {ds:.4g} in log10 Ra")
+            if ds < delta_s_min:
+                return steps, False
+        steps.append(PathStep(ra1, ds, rep.newton_its, tries, secs))
+        if log:
+            log(f"[conti] Ra={ra1:.4g} step {ds:.4g} newton {rep.newton_its} tries {tries}")
+        last.copy_(x)
+        if order == 1:
+            t0 = time.perf_counter()
+            tan = sensitivity(ctx, prob, last, ra1, opts)
+            steps[-1].seconds += time.perf_counter() - t0
+        # path_step_control after a step that converged at once (ac_conti.c:1091-1100)
+        if tries == 1 and n != 0:
+            ds = min(path_step_control(ds, rep.newton_its, max_newton), delta_s_max)
+        s = s1
+        n += 1
+    return steps, abs(s - s_end) < 1e-12

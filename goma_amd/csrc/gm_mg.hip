@@ -882,6 +882,28 @@ extern "C" int gm_mg_copy_level_matrix(const gm_ctx *c, int level, double *val) 
   return GM_OK;
 }
 
+extern "C" int gm_mg_copy_block_inverse(const gm_ctx *c, int level, int elem, double *out) {
+  if (!c || !out || level < 0 || level > (int)c->mg.size()) return GM_ERR_ARG;
+  const MgLevel &L = level == 0 ? c->mg0 : c->mg[level - 1];
+  const gm_ctx *lc = level == 0 ? c : L.ctx;
+  if (elem < 0 || elem >= lc->num_elems) return GM_ERR_ARG;
+  if (!L.d_ainv) {
+    gm_set_error("multigrid level %d has no element-block inverses yet", level);
+    return GM_ERR_ARG;
+  }
+  const size_t n = nloc_of(lc), off = (size_t)elem * n * n;
+  if (L.d_ainv == L.d_ainv_p[1]) {
+    std::vector<float> h(n * n);
+    GM_HIP(hipMemcpy(h.data(), (const float *)L.d_ainv + off, n * n * sizeof(float),
+                     hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n * n; i++) out[i] = h[i];
+  } else {
+    GM_HIP(hipMemcpy(out, (const double *)L.d_ainv + off, n * n * sizeof(double),
+                     hipMemcpyDeviceToHost));
+  }
+  return (int)n;
+}
+
 extern "C" int gm_mg_add_level(gm_ctx *fine, gm_ctx *coarse, const int *P_rowptr, const int *P_col,
                                const double *P_val, const int *inject) {
   if (!fine || !coarse || !P_rowptr || !P_col || !P_val || !inject) return GM_ERR_ARG;
@@ -961,9 +983,10 @@ static void launch_invert(gm_ctx *prof, gm_ctx *lc, hipStream_t s, const VT *val
   // gathered entries + map + inverses written
   const double bytes = (double)lc->num_elems * n * n * (sizeof(VT) + 4 + sizeof(AT));
   if constexpr (DIM == 3) {
-    // 3D blocks (n = 85 / 112): in registers, 32 x 16 threads, at most 4 x 7 entries each
-    gm_launch(prof, GM_K_VANKA_INV, bytes, k_vanka_invert_reg<DIM, EN, VT, AT, AT, 32, 16>,
-              dim3(lc->num_elems), dim3(512), 0, s, level_dev(lc), val, ainv);
+    // 3D blocks (n = 85 / 112) in registers, 16 x 16 threads: three
+    // workgroups per CU (164 VGPRs) overlap their pivot steps' barriers
+    gm_launch(prof, GM_K_VANKA_INV, bytes, k_vanka_invert_reg<DIM, EN, VT, AT, AT, 16, 16>,
+              dim3(lc->num_elems), dim3(256), 0, s, level_dev(lc), val, ainv);
   } else {
     gm_launch(prof, GM_K_VANKA_INV, bytes, k_vanka_invert<DIM, EN, VT, AT, AT, 128>,
               dim3(lc->num_elems), dim3(128), 0, s, level_dev(lc), val, ainv);

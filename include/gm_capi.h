@@ -321,6 +321,12 @@ int gm_mg_copy_prolongation(const gm_ctx *ctx, int level, int *rowptr, int *col,
 /* Coarse Jacobians at the injected state x_dev and element-block inverses for
  * the level-0 matrix val_dev (row-sum scaled by scale_dev, or NULL). */
 int gm_mg_setup_dev(gm_ctx *ctx, const double *x_dev, const double *val_dev, const double *scale_dev);
+/* Host copy (as FP64) of the element-block inverse of element `elem` on
+ * multigrid level `level` (0 = the context's own mesh) from the last set-up:
+ * n x n with n = the element's unknowns (local order: each nodal variable
+ * over the element's nodes, then its P1 dofs), stored transposed
+ * (out[j * n + i] = (A_e^-1)[i][j]).  Returns n. */
+int gm_mg_copy_block_inverse(const gm_ctx *ctx, int level, int elem, double *out);
 /* Host copy (as FP64) of the Jacobian of multigrid level `level` (1 = first
  * coarse level) from the last set-up, in that level's MSR layout. */
 int gm_mg_copy_level_matrix(const gm_ctx *ctx, int level, double *val);

@@ -237,6 +237,23 @@ class Oracle:
         return rc, x, rep
 
 
+def sensitivity(prob_at, x, lam, h):
+    """soln_sens (src/mm_sol_nonlinear.c:3638-3930) restated on the oracle:
+    dx/dlambda = -J^-1 dR/dlambda at the state x, with dR/dlambda the central
+    difference (R(lam + h) - R(lam - h)) / 2h of the residuals of the problems
+    prob_at(lam +- h), and J the Jacobian of prob_at(lam), solved exactly
+    (SciPy sparse LU).  Test infrastructure only."""
+    import scipy.sparse.linalg as spla
+
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    rp = Oracle(prob_at(lam + h)).fill(x)[1]
+    rm = Oracle(prob_at(lam - h)).fill(x)[1]
+    o = Oracle(prob_at(lam))
+    rc, r, v = o.fill(x)
+    A = msr_to_scipy(o.ija, v, o.N).tocsc()
+    return -spla.spsolve(A, (rp - rm) / (2.0 * h))
+
+
 def msr_to_scipy(ija, val, N):
     """MSR (Goma ams->bindx/val) -> scipy CSR, for direct solves in tests."""
     import scipy.sparse as sp

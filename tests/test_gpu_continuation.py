@@ -46,6 +46,14 @@ def test_sensitivity_matches_difference_quotient(capi):
     assert np.abs(t - fd).max() <= 1e-5 * np.abs(fd).max()
     # a wrong sign or scale would be far off
     assert np.abs(-t - fd).max() > 0.5 * np.abs(fd).max()
+    # the oracle's restatement of soln_sens at the same state, exact solve
+    from dataclasses import replace
+
+    import oracle
+
+    ref = oracle.sensitivity(lambda ra: replace(prob, phys=problems.physics_at(prob, ra)),
+                             x0.cpu().numpy(), Ra, 1e-3 * Ra)
+    assert np.abs(t - ref).max() <= 1e-7 * np.abs(ref).max()
     ctx.close()
 
 
@@ -66,3 +74,32 @@ def test_first_order_sweep_saves_newton_passes(capi):
         ctx.close()
     assert its[1] < its[0]
     assert np.abs(sols[1] - sols[0]).max() <= 1e-8 * np.abs(sols[0]).max()
+
+
+def test_adaptive_continuation_reaches_the_end_with_growing_steps(capi):
+    """continue_problem with path_step_control (src/rf_util.c:1312): from a
+    small first step the increments grow while Newton needs few passes, the
+    end value is reached, and the last state is the solution there (the same
+    fixed point as a uniform sweep's last step)."""
+    from goma_amd import continuation
+
+    prob = problems.natural_convection(n=16, Ra=1e3, carreau=True, jitter=0.15)
+    print("[case] adaptive continuation carreau n=16", flush=True)
+    ctx = capi.Context(prob)
+    x = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    opts = dict(OPTS, max_newton=20)
+    steps, ok = continuation.continue_problem(ctx, prob, 1e3, 3e4, 0.05, opts, x=x, order=1,
+                                              delta_s_max=0.5)
+    assert ok
+    assert steps[-1].Ra == pytest.approx(3e4)
+    ds = [st.delta_s for st in steps[1:]]
+    assert max(ds) > 2 * ds[0]  # the step grew
+    assert len(steps) - 1 < (np.log10(3e4) - 3.0) / 0.05  # fewer steps than at the first size
+    ref = capi.Context(prob)
+    xr = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    s, sols = continuation.sweep(ref, prob, problems.sweep_ra(12, 1e3, 3e4), dict(opts), x=xr)
+    assert all(st.converged for st in s)
+    xa = x.cpu().numpy()
+    assert np.abs(xa - sols[-1]).max() <= 1e-8 * np.abs(sols[-1]).max()
+    ctx.close()
+    ref.close()

@@ -45,7 +45,8 @@ class GmPhysics(C.Structure):
                 ("mu0", C.c_double), ("muinf", C.c_double), ("lam", C.c_double),
                 ("nexp", C.c_double), ("aexp", C.c_double), ("Cp", C.c_double),
                 ("kcond", C.c_double), ("hsrc", C.c_double), ("g", C.c_double * 3),
-                ("beta", C.c_double), ("Tref", C.c_double)]
+                ("beta", C.c_double), ("Tref", C.c_double), ("pspg", C.c_int),
+                ("ps_scaling", C.c_double)]
 
 
 class GmBC(C.Structure):
@@ -138,6 +139,7 @@ EXPORTS = [
     "gm_add_qside", "gm_clear_qside", "gm_solve_nonlinear_transient_dev", "gm_solve_transient_dev",
     "gm_num_nodes", "gm_num_elems", "gm_mg_build", "gm_mg_level", "gm_mg_copy_prolongation",
     "gm_matrix_fill_full_assign_dev", "gm_mlp_predict_noalloc_dev", "gm_mg_copy_block_inverse",
+    "gm_h_elem_avg", "gm_velocity_norm_dev", "gm_set_velocity_norm",
 ]
 
 _lib = None
@@ -165,6 +167,9 @@ def lib():
     L.gm_copy_dirichlet.argtypes = [vp, vp, vp]
     L.gm_copy_patch_permutation.argtypes = [vp, ip, vp, vp]
     L.gm_set_physics.argtypes = [vp, C.POINTER(GmPhysics)]
+    L.gm_h_elem_avg.argtypes = [vp, C.POINTER(C.c_double)]
+    L.gm_velocity_norm_dev.argtypes = [vp, vp, C.POINTER(C.c_double)]
+    L.gm_set_velocity_norm.argtypes = [vp, C.c_double]
     L.gm_matrix_fill_full_dev.argtypes = [vp, vp, vp, vp]
     L.gm_matrix_fill_full.argtypes = [vp, vp, vp, vp]
     L.gm_matrix_fill_full32_dev.argtypes = [vp, vp, vp, vp]
@@ -246,7 +251,8 @@ def solver_opts(**kw) -> GmSolverOpts:
 
 def physics_struct(ph) -> GmPhysics:
     return GmPhysics(ph.energy, ph.visc_model, ph.rho, ph.mu0, ph.muinf, ph.lam, ph.nexp, ph.aexp,
-                     ph.Cp, ph.kcond, ph.hsrc, (C.c_double * 3)(*ph.g), ph.beta, ph.Tref)
+                     ph.Cp, ph.kcond, ph.hsrc, (C.c_double * 3)(*ph.g), ph.beta, ph.Tref,
+                     int(getattr(ph, "pspg", 0)), float(getattr(ph, "ps_scaling", 0.0)))
 
 
 class Context:
@@ -319,6 +325,23 @@ class Context:
     def set_physics(self, phys):
         ph = physics_struct(phys)
         _check(lib().gm_set_physics(self.h, C.byref(ph)), "gm_set_physics")
+
+    def h_elem_avg(self) -> float:
+        """global_h_elem_siz of the mesh (the PSPG element size)."""
+        h = C.c_double()
+        _check(lib().gm_h_elem_avg(self.h, C.byref(h)), "gm_h_elem_avg")
+        return h.value
+
+    def velocity_norm_dev(self, x_ptr: int) -> float:
+        """global_velocity_norm of a device state: mean square of the velocities."""
+        u = C.c_double()
+        _check(lib().gm_velocity_norm_dev(self.h, x_ptr, C.byref(u)), "gm_velocity_norm_dev")
+        return u.value
+
+    def set_velocity_norm(self, U: float):
+        """The PSPG velocity norm the assembly entry points use (the Newton
+        drivers set it from x at every pass)."""
+        _check(lib().gm_set_velocity_norm(self.h, float(U)), "gm_set_velocity_norm")
 
     # ---- compute (device pointers)
     def fill_dev(self, x_ptr: int, resid_ptr: int, val_ptr: int) -> int:

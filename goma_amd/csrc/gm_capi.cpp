@@ -153,6 +153,7 @@ static int create_impl(const gm_mesh *m, const gm_physics *ph, const gm_bc *bc, 
     }
   int rc = gm_build_host(c);
   if (rc) return rc;
+  c->h_elem_avg = gm_h_elem_size(c);
   // Dirichlet map per unknown; pressure datum on the datum element's first
   // P dof (find_and_set_Dirichlet, src/mm_bc.c:132, 386-457).
   c->dbc.assign(c->N, 0);
@@ -271,6 +272,27 @@ extern "C" int gm_set_physics(gm_ctx *c, const gm_physics *ph) {
   c->phys = *ph;
   // the multigrid levels re-assemble the same physics on coarser meshes
   for (MgLevel &L : c->mg) L.ctx->phys = *ph;
+  return GM_OK;
+}
+
+extern "C" int gm_h_elem_avg(gm_ctx *c, double *h) {
+  if (!c || !h) return GM_ERR_ARG;
+  *h = c->h_elem_avg;
+  return GM_OK;
+}
+
+extern "C" int gm_velocity_norm_dev(gm_ctx *c, const double *x, double *U) {
+  if (!c || !x || !U) return GM_ERR_ARG;
+  return gm_velocity_norm(c, x, U);
+}
+
+extern "C" int gm_set_velocity_norm(gm_ctx *c, double U) {
+  if (!c || !(U >= 0.0)) {
+    gm_set_error("the velocity norm must be a non-negative number");
+    return GM_ERR_ARG;
+  }
+  c->U_norm = U;
+  for (MgLevel &L : c->mg) L.ctx->U_norm = U;
   return GM_OK;
 }
 
@@ -537,6 +559,12 @@ static int solve_impl(gm_ctx *c, double *x, const gm_solver_opts *o, gm_newton_r
   if (galerkin && (rc = gm_mg_kel_alloc(c))) return rc;
   while (!converged && inewton < o->max_newton) {
     double ta = now_s();
+    // the PSPG velocity norm of this pass (src/mm_sol_nonlinear.c:1183-1192);
+    // the line search's residuals keep it
+    if (c->phys.pspg) {
+      if ((rc = gm_velocity_norm(c, x, &c->U_norm))) return rc;
+      for (MgLevel &L : c->mg) L.ctx->U_norm = c->U_norm;
+    }
     if ((rc = gm_zero(c, c->d_resid, c->N))) return rc;
     if ((rc = gm_zero(c, c->d_dx, c->N))) return rc;
     // the HEX27 assembly stores each node pair's first contribution, so the

@@ -24,6 +24,10 @@
 //     R_a,i = int[ -rho phi_i (v.grad v_a) - grad phi_i . Pi[a][.] + phi_i f_a ] dA
 //     Pi = -P I + mu (grad v + grad v^T),  f_a = -rho g_a beta (T - Tref)
 //   continuity (src/mm_fill_continuity.c:435-700):  R_k = int psi_k div v dA
+//     with PSPG (global tau, src/mm_fill_continuity.c:600-612; calc_pspg
+//     src/mm_fill_stabilization.c:1037-1049, 1100-1465, no polymer stress):
+//     R_k += int grad psi_k . tau (rho v_dot + rho v.grad v + grad P - f) dA,
+//     grad psi_0 = 0 and grad psi_k = B[.][k-1] for the P1 basis {1, xi_k}
 //   energy   (src/mm_fill_energy.c:280-700, heat_flux 1629):
 //     R_i = int[ -rho Cp phi_i (v.grad T) - k grad phi_i . grad T + phi_i h ] dA
 #include <cmath>
@@ -66,6 +70,7 @@ struct FillArgs {
   const unsigned short *rank;       // rank map [num_elems][2][nve * nve] (k_build_ranks)
   int overwrite;                    // 1: first touch of a node pair stores (no zeroing needed)
   int num_nodes;
+  double tau_pspg;  // global PSPG tau, 0: no pressure stabilisation
   int resid_only;  // 1: residual only (Assemble_Jacobian = FALSE)
   long long E;
   long long off_mp, off_cv, off_cp;
@@ -124,6 +129,7 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
   __shared__ double s_ud[NV][NVE];       // nodal time derivatives (transient)
   __shared__ double s_vd[NQ][DIM];       // v_dot at the quadrature points
   __shared__ double s_Td[NQ];            // T_dot
+  __shared__ double s_ps[NQ][DIM];       // PSPG: tau dA (rho v_dot + rho v.grad v + grad P - f)
   constexpr int NSP = STab<DIM>::NSP, SREC = STab<DIM>::REC;
   __shared__ double s_qw[GM_QS_MAX][NSP];  // QSIDE: weight * surface det * flux per point
   __shared__ int s_qsd[GM_QS_MAX];         // QSIDE: side index (0-based) of each entry
@@ -287,6 +293,16 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
     s_mu[q] = mu;
     s_dmu[q] = dmu;
     s_div[q] = div;
+    const double w = A.tau_pspg * s_dA[q];
+#pragma unroll
+    for (int a = 0; a < DIM; a++) {
+      double adv = 0.0, gP = 0.0;
+#pragma unroll
+      for (int p = 0; p < DIM; p++) adv += v[p] * gv[p][a];
+#pragma unroll
+      for (int k = 1; k < NP; k++) gP += s_p[k] * s_B[q][a][k - 1];
+      s_ps[q][a] = w * (A.rho * (adv + vd[a]) + gP - s_f[q][a]);
+    }
   } else if (EN && tid >= NT / 2 && tid < NT / 2 + nqs * NSP) {
     // QSIDE (src/bc_integ.c:1518-1519, 2156-2157): w * |surface Jacobian| * q
     // at one surface Gauss point of one side entry; the side's free reference
@@ -365,6 +381,10 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
       const int k = t - NV * NVE;
       row = s_gp[k];
       for (int q = 0; q < NQ; q++) R += s_dA[q] * tb[T::PSI + q * NP + k] * s_div[q];
+      if (A.tau_pspg != 0.0 && k > 0)
+        for (int q = 0; q < NQ; q++)
+#pragma unroll
+          for (int a = 0; a < DIM; a++) R += s_B[q][a][k - 1] * s_ps[q][a];
     }
     if (A.dbc[row]) R = A.x[row] - A.dbc_val[row];
     A.resid[row] += R;
@@ -478,18 +498,39 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
     }
   }
 
-  // 5d. continuity x velocity (J_c_v): int psi_k grad_b phi_j dA
+  // 5d. continuity x velocity (J_c_v): int psi_k grad_b phi_j dA; PSPG adds
+  // tau rho [phi_j grad psi_k . d_b v + grad_b psi_k (v.grad phi_j + c phi_j)]
+  // and continuity x temperature tau rho beta (grad psi_k . g) phi_j
+  const bool ps = A.tau_pspg != 0.0;
   for (int t = tid; t < NP * NVE; t += NT) {
     const int k = t / NVE, j = t - k * NVE;
     const bool dir = A.dbc[s_gp[k]] != 0;
     if (dir && !A.kel) continue;
-    double acc[DIM];
+    double acc[DIM], accT = 0.0;
 #pragma unroll
     for (int b = 0; b < DIM; b++) acc[b] = 0.0;
     for (int q = 0; q < NQ; q++) {
       const double w = s_dA[q] * tb[T::PSI + q * NP + k];
 #pragma unroll
       for (int b = 0; b < DIM; b++) acc[b] += w * s_g[q][j][b];
+      if (ps && k > 0) {
+        const double wp = A.tau_pspg * s_dA[q] * rho, phj = tb[T::PHI + q * NVE + j];
+        double vg = 0.0, gk[DIM], gkg = 0.0;
+#pragma unroll
+        for (int p = 0; p < DIM; p++) {
+          vg += s_v[q][p] * s_g[q][j][p];
+          gk[p] = s_B[q][p][k - 1];
+          gkg += gk[p] * A.g[p];
+        }
+#pragma unroll
+        for (int b = 0; b < DIM; b++) {
+          double gv = 0.0;
+#pragma unroll
+          for (int a = 0; a < DIM; a++) gv += gk[a] * s_gv[q][b][a];
+          acc[b] += wp * (phj * gv + gk[b] * (vg + cm * phj));
+        }
+        accT += wp * A.beta * gkg * phj;
+      }
     }
 #pragma unroll
     for (int b = 0; b < DIM; b++) {
@@ -498,16 +539,40 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
       const int idx = map[A.off_cv + b * NP * NVE + t];
       out[idx] += (JT)acc[b];
     }
+    if (EN && ps && k > 0 && !dir) {
+      // the element layout holds no continuity x temperature entries: find
+      // the column in the row (sorted off-diagonal list)
+      const int row = s_gp[k], col = s_gun[DIM][j];
+      int lo = A.bindx[row], hi = A.bindx[row + 1] - 1;
+      while (lo <= hi) {
+        const int mid = (lo + hi) >> 1, v = A.bindx[mid];
+        if (v == col) {
+          out[mid] += (JT)accT;
+          break;
+        }
+        if (v < col) lo = mid + 1;
+        else hi = mid - 1;
+      }
+    }
   }
 
-  // 5e. continuity x pressure: zero for Galerkin Q2-P1, unit diagonal on a
-  // Dirichlet (datum) row
+  // 5e. continuity x pressure: zero for Galerkin Q2-P1; PSPG adds
+  // tau grad psi_k . grad psi_l; a unit diagonal on a Dirichlet (datum) row
   if (tid < NP * NP) {
     const int k = tid / NP, k2 = tid - k * NP;
-    if (A.kel) A.kel[(size_t)e * A.E + A.off_cp + tid] = 0.0;
-    if (k == k2 && A.dbc[s_gp[k]]) {
-      const int idx = map[A.off_cp + tid];
-      out[idx] += (JT)1.0;
+    double acc = 0.0;
+    if (ps && k > 0 && k2 > 0)
+      for (int q = 0; q < NQ; q++) {
+        double g2 = 0.0;
+#pragma unroll
+        for (int a = 0; a < DIM; a++) g2 += s_B[q][a][k - 1] * s_B[q][a][k2 - 1];
+        acc += A.tau_pspg * s_dA[q] * g2;
+      }
+    if (A.kel) A.kel[(size_t)e * A.E + A.off_cp + tid] = acc;
+    if (A.dbc[s_gp[k]]) {
+      if (k == k2) out[map[A.off_cp + tid]] += (JT)1.0;
+    } else if (acc != 0.0) {
+      out[map[A.off_cp + tid]] += (JT)acc;
     }
   }
 }
@@ -827,6 +892,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VISC ? 3 : 
       row = s_gp[k];
 #pragma unroll 1
       for (int q = 0; q < NQ; q++) R += s_dA[q] * tb[T::PSI + q * NP + k] * s_div[q];
+      if (A.tau_pspg != 0.0 && k > 0) {
+        // PSPG: grad psi_k . tau (rho v_dot + rho v.grad v + grad P - f) dA,
+        // formed per point from the staged fields (four threads do this)
+#pragma unroll 1
+        for (int q = 0; q < NQ; q++) {
+          double r = 0.0;
+#pragma unroll
+          for (int a = 0; a < DIM; a++) {
+            double adv = 0.0, gP = 0.0;
+#pragma unroll
+            for (int p = 0; p < DIM; p++) adv += s_v[q][p] * s_gv[q][p][a];
+#pragma unroll
+            for (int l = 1; l < NP; l++) gP += s_p[l] * s_B[q][a][l - 1];
+            r += s_B[q][a][k - 1] * (rho * (adv + s_vd[q][a]) + gP - s_f[q][a]);
+          }
+          R += A.tau_pspg * s_dA[q] * r;
+        }
+      }
       if (s_dirp[k]) R = s_p[k] - A.dbc_val[row];
     }
     A.resid[row] += R;
@@ -987,16 +1070,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VISC ? 3 : 
     }
   }
 
-  // 5d. continuity x velocity (J_c_v): int psi_k grad_b phi_j dA
+  // 5d. continuity x velocity (J_c_v): int psi_k grad_b phi_j dA; PSPG adds
+  // tau rho [phi_j grad psi_k . d_b v + grad_b psi_k (v.grad phi_j + c phi_j)]
+  // and continuity x temperature tau rho beta (grad psi_k . g) phi_j
+  const bool ps = A.tau_pspg != 0.0;
   for (int t = tid; t < NP * NVE; t += NT) {
     const int k = t / NVE, j = t - k * NVE;
     const bool dir = s_dirp[k] != 0;
-    double acc[DIM] = {0.0, 0.0, 0.0};
+    double acc[DIM] = {0.0, 0.0, 0.0}, accT = 0.0;
 #pragma unroll 1
     for (int q = 0; q < NQ; q++) {
       const double w = s_dA[q] * tb[T::PSI + q * NP + k];
 #pragma unroll
       for (int b = 0; b < DIM; b++) acc[b] += w * s_G[b][q][j];
+    }
+    if (ps && k > 0) {
+#pragma unroll 1
+      for (int q = 0; q < NQ; q++) {
+        const double wp = A.tau_pspg * s_dA[q] * rho, phj = s_Phi[q][j];
+        double vg = 0.0, gk[DIM], gkg = 0.0;
+#pragma unroll
+        for (int p = 0; p < DIM; p++) {
+          vg += s_v[q][p] * s_G[p][q][j];
+          gk[p] = s_B[q][p][k - 1];
+          gkg += gk[p] * A.g[p];
+        }
+#pragma unroll
+        for (int b = 0; b < DIM; b++) {
+          double gv = 0.0;
+#pragma unroll
+          for (int a = 0; a < DIM; a++) gv += gk[a] * s_gv[q][b][a];
+          acc[b] += wp * (phj * gv + gk[b] * (vg + cm * phj));
+        }
+        accT += wp * A.beta * gkg * phj;
+      }
     }
     const unsigned rm = rk[CL * NVE + j];
     const bool first = ow && (rm & 0x8000u);
@@ -1015,21 +1122,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VISC ? 3 : 
       else out[pos] += (JT)v;
     }
     // continuity x temperature: in the graph (the continuity rows couple to
-    // every unknown of the element's nodes) but zero in the weak form
-    if (EN && first) {
+    // every unknown of the element's nodes); zero in the Galerkin weak form,
+    // PSPG's Boussinesq term otherwise
+    if (EN && (first || (ps && !dir))) {
       const int col = s_gun[DIM][j];
-      out[s_bp[k] + (int)(rm & 0x7fffu) + DIM - (col > row ? 1 : 0)] = (JT)0.0;
+      const long long pos = s_bp[k] + (int)(rm & 0x7fffu) + DIM - (col > row ? 1 : 0);
+      const double v = dir ? 0.0 : accT;
+      if (first) out[pos] = (JT)v;
+      else out[pos] += (JT)v;
     }
   }
 
-  // 5e. continuity x pressure: zero for Galerkin Q2-P1, unit diagonal on a
-  // Dirichlet (datum) row
+  // 5e. continuity x pressure: zero for Galerkin Q2-P1; PSPG adds
+  // tau grad psi_k . grad psi_l; a unit diagonal on a Dirichlet (datum) row
   if (tid < NP * NP) {
     const int k = tid / NP, k2 = tid - k * NP;
-    if (A.kel) A.kel[(size_t)e * A.E + A.off_cp + tid] = 0.0;
+    double acc = 0.0;
+    if (ps && k > 0 && k2 > 0)
+#pragma unroll 1
+      for (int q = 0; q < NQ; q++) {
+        double g2 = 0.0;
+#pragma unroll
+        for (int a = 0; a < DIM; a++) g2 += s_B[q][a][k - 1] * s_B[q][a][k2 - 1];
+        acc += A.tau_pspg * s_dA[q] * g2;
+      }
+    if (A.kel) A.kel[(size_t)e * A.E + A.off_cp + tid] = acc;
     const unsigned rm = rk[CL * NVE + CL];
     const bool first = ow && (rm & 0x8000u);
-    const double v = (k == k2 && s_dirp[k]) ? 1.0 : 0.0;
+    const double v = s_dirp[k] ? (k == k2 ? 1.0 : 0.0) : acc;
     const long long pos = k == k2 ? (long long)s_gp[k] : s_bp[k] + (int)(rm & 0x7fffu) + NV + k2 - (k2 > k ? 1 : 0);
     if (first) out[pos] = (JT)v;
     else if (v != 0.0) out[pos] += (JT)v;
@@ -1422,6 +1542,7 @@ int gm_fill_launch_ex(gm_ctx *c, const double *x, double *resid, double *val, fl
   a.rank = c->d_rank;
   a.overwrite = c->fill_overwrite && c->d_rank ? 1 : 0;
   a.num_nodes = c->num_nodes;
+  a.tau_pspg = c->phys.pspg ? gm_pspg_tau(c) : 0.0;
   a.E = c->lay.total;
   a.off_mp = c->lay.off_mp;
   a.off_cv = c->lay.off_cv;

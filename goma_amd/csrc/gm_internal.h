@@ -120,6 +120,8 @@ struct MgLevel {
 struct gm_ctx {
   int dim = 0, nve = 0, np = 0, nv = 0, num_nodes = 0, num_elems = 0, energy = 0;
   gm_physics phys{};
+  double h_elem_avg = 0.0;  // PSPG element size (global_h_elem_siz), from the mesh
+  double U_norm = 0.0;      // PSPG velocity norm (global_velocity_norm) the fills use
   int N = 0;
   long long nval = 0;  // bindx[N]
   std::vector<double> coor;
@@ -215,6 +217,11 @@ int gm_zero(gm_ctx *c, double *p, long long n);
 int gm_zero32(gm_ctx *c, float *p, long long n);
 int gm_scale_launch(gm_ctx *c, double *val, float *val32, double *b, double *scale);
 int gm_norms(gm_ctx *c, const double *v, long long n, double *l2, double *loo);
+// PSPG: mean square of the velocity unknowns of x (device), the mesh's mean
+// element size (host) and the global tau at c->U_norm
+int gm_velocity_norm(gm_ctx *c, const double *x, double *U);
+double gm_h_elem_size(const gm_ctx *c);
+double gm_pspg_tau(const gm_ctx *c);
 int gm_corr_norms(gm_ctx *c, const double *dx, const double *x, double *l2r, double *loor);
 int gm_axpy_update(gm_ctx *c, double *x, const double *dx, double damp);
 int gm_spmv_launch(gm_ctx *c, const double *val, const double *x, double *y);

@@ -172,6 +172,36 @@ int gm_corr_norms(gm_ctx *c, const double *dx, const double *x, double *l2r, dou
   return finish_norms(c, l2r, loor);
 }
 
+// global_velocity_norm (src/mm_fill_aux.c:612-672): partial sums of x^2 over
+// the velocity unknowns, the first dim of every node
+__global__ __launch_bounds__(RED_THREADS) void k_velocity_sq(int num_nodes, int dim,
+                                                             const int *__restrict__ fu,
+                                                             const double *__restrict__ x,
+                                                             double *__restrict__ part) {
+  __shared__ double sh[RED_THREADS / 64];
+  double s = 0.0;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < (long long)num_nodes * dim;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int n = (int)(t / dim), a = (int)(t - (long long)n * dim);
+    const double v = x[fu[n] + a];
+    s += v * v;
+  }
+  const double ss = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = ss;
+}
+
+int gm_velocity_norm(gm_ctx *c, const double *x, double *U) {
+  k_velocity_sq<<<RED_BLOCKS, RED_THREADS, 0, c->stream>>>(c->num_nodes, c->dim, c->d_fu, x, c->d_red);
+  GM_HIP(hipGetLastError());
+  GM_HIP(hipMemcpyAsync(c->h_red, c->d_red, RED_BLOCKS * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+  GM_HIP(hipStreamSynchronize(c->stream));
+  double s = 0.0;
+  for (int i = 0; i < RED_BLOCKS; i++) s += c->h_red[i];
+  *U = s / ((double)c->num_nodes * c->dim);
+  return GM_OK;
+}
+
 // x -= damp * dx (NLS_FULL_STEP update, src/mm_sol_nonlinear.c:2553-2557)
 __global__ void k_newton_update(double *__restrict__ x, const double *__restrict__ dx, int n,
                                 double damp) {

@@ -165,6 +165,56 @@ int gm_build_host(gm_ctx *c) {
   return GM_OK;
 }
 
+// global_h_elem_siz (src/mm_fill_aux.c:1128-1200) with the QUAD / HEX
+// widths of h_elem_siz (src/mm_fill_aux.c:844-1125): along each reference
+// direction the squared distance between the centroids of the two opposite
+// faces, from the vertex nodes (local 0-3 or 0-7); an element contributes
+// sqrt(mean of the squared widths), the result is the mean over elements.
+double gm_h_elem_size(const gm_ctx *c) {
+  // vertex sets of the faces at xi_r = -1 and xi_r = +1 (Goma's local order)
+  static const int q_lo[2][2] = {{0, 3}, {0, 1}}, q_hi[2][2] = {{1, 2}, {2, 3}};
+  static const int h_lo[3][4] = {{0, 3, 4, 7}, {0, 1, 4, 5}, {4, 5, 6, 7}};
+  static const int h_hi[3][4] = {{1, 2, 5, 6}, {2, 3, 6, 7}, {0, 1, 2, 3}};
+  const int d = c->dim, nf = d == 2 ? 2 : 4;
+  double sum = 0.0;
+  for (int e = 0; e < c->num_elems; e++) {
+    const int *en = &c->conn[(size_t)e * c->nve];
+    double hsq = 0.0;
+    for (int r = 0; r < d; r++) {
+      double w2 = 0.0;
+      for (int a = 0; a < d; a++) {
+        const double *xa = &c->coor[(size_t)a * c->num_nodes];
+        double lo = 0.0, hi = 0.0;
+        for (int k = 0; k < nf; k++) {
+          lo += xa[en[d == 2 ? q_lo[r][k] : h_lo[r][k]]];
+          hi += xa[en[d == 2 ? q_hi[r][k] : h_hi[r][k]]];
+        }
+        const double w = (hi - lo) / nf;
+        w2 += w * w;
+      }
+      hsq += w2;
+    }
+    sum += std::sqrt(hsq / d);
+  }
+  return sum / c->num_elems;
+}
+
+// tau of the global PSPG scheme (calc_pspg, src/mm_fill_stabilization.c:
+// 1037-1049) at c->U_norm; mu_avg is element_viscosity's (src/mm_fill_aux.c:
+// 687-751): the viscosity model at a dim x dim shear-rate tensor of 1e-5
+// entries, so gammadot = sqrt(dim^2 / 2) 1e-5.
+double gm_pspg_tau(const gm_ctx *c) {
+  const gm_physics &p = c->phys;
+  double mu = p.mu0;
+  if (p.visc_model == 1) {
+    const double gd = std::sqrt(0.5 * c->dim * c->dim) * 1e-5;
+    mu = p.muinf + (p.mu0 - p.muinf) * std::pow(1.0 + std::pow(p.lam * gd, p.aexp), (p.nexp - 1.0) / p.aexp);
+  }
+  const double h = c->h_elem_avg, U = c->U_norm;
+  if (p.rho * U * h / (2.0 * mu) <= 3.0) return p.ps_scaling * h * h / (12.0 * mu);
+  return p.ps_scaling * h / (2.0 * p.rho * U);
+}
+
 int gm_default_bins(const gm_ctx *c) {
   // aim at patches of about 8x8 elements in 2D and 4x4x4 in 3D
   double per_dir = std::pow((double)c->num_elems, 1.0 / c->dim);

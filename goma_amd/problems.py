@@ -44,6 +44,8 @@ class Physics:
     g: tuple = (0.0, 0.0, 0.0)
     beta: float = 0.0
     Tref: float = 0.0
+    pspg: int = 0          # Pressure Stabilization: 0 no, 1 global
+    ps_scaling: float = 0.0  # Pressure Stabilization Scaling
 
 
 @dataclass

@@ -79,6 +79,8 @@ typedef struct gm_physics {
   double g[3];         /* momentum_source (gravity vector) */
   double beta;         /* Volume_Expansion (Boussinesq) */
   double Tref;         /* reference[TEMPERATURE] */
+  int pspg;            /* Pressure Stabilization: 0 no, 1 global (Goma's PSPG = 1) */
+  double ps_scaling;   /* Pressure Stabilization Scaling */
 } gm_physics;
 
 typedef struct gm_bc {
@@ -172,6 +174,25 @@ int gm_copy_patch_permutation(gm_ctx *ctx, int patch_bins, int *perm, int *colou
  * update_parameterC, src/ac_update_parameter.c:48), on this context and on
  * the multigrid levels attached to it. */
 int gm_set_physics(gm_ctx *ctx, const gm_physics *phys);
+
+/* Pressure stabilisation (PSPG, global tau; calc_pspg,
+ * src/mm_fill_stabilization.c:1037-1049): with phys.pspg = 1 the continuity
+ * rows gain int grad psi_k . tau (rho v_dot + rho v.grad v + grad P - f) dA
+ * (src/mm_fill_continuity.c:600-612) and its Jacobian, with
+ * Re = rho U h / (2 mu_avg), tau = s h^2 / (12 mu_avg) (Re <= 3) or
+ * s h / (2 rho U), mu_avg the viscosity at a shear-rate tensor of 1e-5
+ * entries (element_viscosity, src/mm_fill_aux.c:687).
+ * gm_h_elem_avg replaces global_h_elem_siz (src/mm_fill_aux.c:1128): the
+ * mean over elements of sqrt(sum_p hsquared[p] / dim), from the mesh.
+ * gm_velocity_norm_dev replaces global_velocity_norm (src/mm_fill_aux.c:612):
+ * the mean of x^2 over the velocity unknowns of x_dev.
+ * gm_set_velocity_norm is matrix_fill_full's U_norm argument
+ * (src/mm_fill.c:174) for the assembly entry points; the Newton drivers set
+ * it themselves from x at the top of every pass (src/mm_sol_nonlinear.c:
+ * 1183-1192), and their line-search residuals keep it. */
+int gm_h_elem_avg(gm_ctx *ctx, double *h);
+int gm_velocity_norm_dev(gm_ctx *ctx, const double *x_dev, double *U_norm);
+int gm_set_velocity_norm(gm_ctx *ctx, double U_norm);
 
 /* matrix_fill_full (src/mm_fill.c:158-312): accumulates the residual and the
  * analytic Jacobian of every element, with Dirichlet rows replaced per element

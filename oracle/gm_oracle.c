@@ -37,6 +37,8 @@ struct om_problem {
   int n_qs;              /* QSIDE heat-flux sides: element, side (1-based), flux */
   int *qs_elem, *qs_side;
   double *qs_q;
+  double h_elem_avg; /* PSPG: global mean element size */
+  double U_norm;     /* PSPG: global velocity norm (mean square) */
 };
 
 static double now_s(void) {
@@ -303,6 +305,7 @@ om_problem *om_create(const om_mesh *m, const om_physics *ph, const om_bc *bc) {
     p->dbc[u] = 1;
     p->dbc_val[u] = bc->datum_value;
   }
+  p->h_elem_avg = om_h_elem_avg(p);
   return p;
 }
 
@@ -421,6 +424,94 @@ static void qside_elem(const om_problem *p, int e, double xc[3][MDE], lec_t *L) 
   }
 }
 
+/* h_elem_siz (src/mm_fill_aux.c:844-1125), QUAD and HEX branches: the
+ * squared widths between the centroids of opposite faces, from the vertex
+ * nodes (local 0-3 / 0-7). */
+static void h_elem_siz(const om_problem *p, int e, double hsq[3]) {
+  const int d = p->dim, nve = p->nve;
+  double xn[3][8];
+  const int nvert = d == 2 ? 4 : 8;
+  for (int a = 0; a < d; a++)
+    for (int i = 0; i < nvert; i++)
+      xn[a][i] = p->coor[(size_t)a * p->num_nodes + p->conn[(size_t)e * nve + i]];
+  if (d == 2) {
+    double hh0[2], hh1[2];
+    for (int a = 0; a < 2; a++) {
+      const double p1 = 0.5 * (xn[a][0] + xn[a][3]), p2 = 0.5 * (xn[a][1] + xn[a][2]);
+      const double p3 = 0.5 * (xn[a][0] + xn[a][1]), p4 = 0.5 * (xn[a][2] + xn[a][3]);
+      hh0[a] = p2 - p1;
+      hh1[a] = p3 - p4;
+    }
+    hsq[0] = hh0[0] * hh0[0] + hh0[1] * hh0[1];
+    hsq[1] = hh1[0] * hh1[0] + hh1[1] * hh1[1];
+    hsq[2] = 0.0;
+  } else {
+    double hh[3][3];
+    for (int a = 0; a < 3; a++) {
+      const double f1 = 0.25 * (xn[a][0] + xn[a][1] + xn[a][2] + xn[a][3]);
+      const double f2 = 0.25 * (xn[a][1] + xn[a][2] + xn[a][5] + xn[a][6]);
+      const double f3 = 0.25 * (xn[a][2] + xn[a][3] + xn[a][6] + xn[a][7]);
+      const double f4 = 0.25 * (xn[a][0] + xn[a][1] + xn[a][4] + xn[a][5]);
+      const double f5 = 0.25 * (xn[a][0] + xn[a][3] + xn[a][4] + xn[a][7]);
+      const double f6 = 0.25 * (xn[a][4] + xn[a][5] + xn[a][6] + xn[a][7]);
+      hh[0][a] = f2 - f5;
+      hh[1][a] = f3 - f4;
+      hh[2][a] = f1 - f6;
+    }
+    for (int r = 0; r < 3; r++) hsq[r] = hh[r][0] * hh[r][0] + hh[r][1] * hh[r][1] + hh[r][2] * hh[r][2];
+  }
+}
+
+double om_h_elem_avg(const om_problem *p) {
+  /* global_h_elem_siz (src/mm_fill_aux.c:1128-1200), one process */
+  double h = 0.0;
+  const double w = 1.0 / (double)p->num_elems;
+  for (int e = 0; e < p->num_elems; e++) {
+    double hsq[3];
+    h_elem_siz(p, e, hsq);
+    double s = 0.0;
+    for (int r = 0; r < p->dim; r++) s += hsq[r];
+    h += sqrt(s / p->dim) * w;
+  }
+  return h;
+}
+
+double om_velocity_norm(const om_problem *p, const double *x) {
+  /* global_velocity_norm (src/mm_fill_aux.c:612-672): every node carries
+   * the velocity components */
+  double s = 0.0;
+  long long cnt = 0;
+  for (int a = 0; a < p->dim; a++)
+    for (int n = 0; n < p->num_nodes; n++) {
+      const double v = x[p->first_unknown[n] + a];
+      s += v * v;
+      cnt++;
+    }
+  return s / (double)cnt;
+}
+
+void om_set_velocity_norm(om_problem *p, double U_norm) { p->U_norm = U_norm; }
+
+/* element_viscosity (src/mm_fill_aux.c:687-751): the viscosity model at a
+ * shear-rate tensor with every one of its dim x dim entries 1e-5 (VIM =
+ * Num_Dim in Cartesian coordinates, src/mm_prob_def.c:96). */
+static double element_viscosity(const om_physics *ph, int dim) {
+  if (ph->visc_model != 1) return ph->mu0;
+  const double gd = sqrt(0.5 * fabs((double)(dim * dim) * 1e-5 * 1e-5));
+  const double val2 = pow(ph->lam * gd, ph->aexp);
+  return ph->muinf + (ph->mu0 - ph->muinf) * pow(1. + val2, (ph->nexp - 1.) / ph->aexp);
+}
+
+double om_pspg_tau(const om_problem *p) {
+  /* calc_pspg, pspg_global (src/mm_fill_stabilization.c:1037-1049); the
+   * global scheme's tau carries no Jacobian dependence */
+  const om_physics *ph = &p->ph;
+  const double mu_avg = element_viscosity(ph, p->dim), h = p->h_elem_avg, U = p->U_norm;
+  const double Re = ph->rho * U * h / (2.0 * mu_avg);
+  if (Re <= 3.0) return ph->ps_scaling * h * h / (12.0 * mu_avg);
+  return ph->ps_scaling * h / (2.0 * ph->rho * U);
+}
+
 /* Assembles one element into lec.  Returns 0, or -1 on detJ <= 0.
  * Follows matrix_fill's quadrature loop (src/mm_fill.c:1253-2791).  With
  * xdot != NULL the transient mass terms are added: xdot is the time
@@ -430,6 +521,7 @@ static int elem_fill(const om_problem *p, int e, const double *x, const double *
   const int d = p->dim, nve = p->nve, np = d + 1, en = p->ph.energy ? 1 : 0;
   const om_physics *ph = &p->ph;
   const int nq = (d == 2) ? 9 : 27;
+  const double tau_pspg = ph->pspg ? om_pspg_tau(p) : 0.0;
   double xc[3][MDE], vel[3][MDE], Tn[MDE], Pn[4], veld[3][MDE], Tdn[MDE];
   memset(L->ndof, 0, sizeof(L->ndof));
   for (int a = 0; a < d; a++) L->ndof[a] = nve;
@@ -665,6 +757,54 @@ static int elem_fill(const om_problem *p, int e, const double *x, const double *
           double div_phi_j_e_b = gphi[j][b];
           L->J[SLOT_P][b][k][j] += psi[k] * div_phi_j_e_b * d_area;
         }
+    }
+    /* PSPG, global tau (src/mm_fill_continuity.c:305-307, 600-612, 742-749,
+     * 792-809, 890-905; calc_pspg src/mm_fill_stabilization.c:1100-1465 with
+     * no polymer stress, so div_s = div_G = 0): R_c[k] += grad psi_k . pspg dA,
+     * pspg_a = tau (rho v_dot_a + rho v . grad v_a + dP/dx_a - f_a).  The P1
+     * gradients: grad psi_0 = 0, grad psi_k[a] = B[a][k-1]. */
+    if (ph->pspg) {
+      double gpsi[4][3], gP[3], mom[3];
+      for (int a = 0; a < 3; a++) gpsi[0][a] = 0.0;
+      for (int k = 1; k < np; k++)
+        for (int a = 0; a < 3; a++) gpsi[k][a] = a < d ? B[a][k - 1] : 0.0;
+      for (int a = 0; a < d; a++) {
+        gP[a] = 0.0;
+        for (int k = 0; k < np; k++) gP[a] += Pn[k] * gpsi[k][a];
+        double adv = 0.0;
+        for (int pp = 0; pp < d; pp++) adv += v[pp] * grad_v[pp][a];
+        mom[a] = ph->rho * adv + gP[a] - f[a] + (xdot ? ph->rho * v_dot[a] : 0.0);
+      }
+      const double w = tau_pspg * d_area;
+      for (int k = 0; k < np; k++) {
+        double r = 0.0;
+        for (int a = 0; a < d; a++) r += gpsi[k][a] * mom[a];
+        L->R[SLOT_P][k] += w * r;
+        /* J_c_v: d pspg_a / d v_b,j = tau rho (phi_j dv_a/dx_b + delta_ab
+         * (v . grad phi_j + c phi_j)), c = (1 + 2 theta) / dt (transient) */
+        for (int b = 0; b < d; b++)
+          for (int j = 0; j < nve; j++) {
+            double vg = 0.0;
+            for (int pp = 0; pp < d; pp++) vg += v[pp] * gphi[j][pp];
+            double acc = 0.0;
+            for (int a = 0; a < d; a++) acc += gpsi[k][a] * phi[j] * grad_v[b][a];
+            acc += gpsi[k][b] * (vg + (xdot ? cmass * phi[j] : 0.0));
+            L->J[SLOT_P][b][k][j] += w * ph->rho * acc;
+          }
+        /* J_c_T: d pspg_a / d T_j = -tau df_a/dT_j = tau rho g_a beta phi_j */
+        if (en)
+          for (int j = 0; j < nve; j++) {
+            double acc = 0.0;
+            for (int a = 0; a < d; a++) acc += gpsi[k][a] * ph->rho * ph->g[a] * ph->beta;
+            L->J[SLOT_P][SLOT_T][k][j] += w * acc * phi[j];
+          }
+        /* J_c_P: d pspg_a / d P_l = tau grad psi_l[a] */
+        for (int l = 0; l < np; l++) {
+          double acc = 0.0;
+          for (int a = 0; a < d; a++) acc += gpsi[k][a] * gpsi[l][a];
+          L->J[SLOT_P][SLOT_P][k][l] += w * acc;
+        }
+      }
     }
 
     /* ---------------- assemble_energy (src/mm_fill_energy.c:280-700) */
@@ -1348,7 +1488,7 @@ static double backtrack(const om_problem *p, double *x, const double *dx, const 
 
 /* solve_nonlinear_problem, steady, NLS_FULL_STEP or NLS_BACKTRACK
  * (src/mm_sol_nonlinear.c:1108-2913). */
-int om_newton(const om_problem *p, double *x, const om_solver_opts *o, om_report *rep) {
+int om_newton(om_problem *p, double *x, const om_solver_opts *o, om_report *rep) {
   const int N = p->N;
   const int nv = p->ija[N];
   double *resid = (double *)malloc(N * sizeof(double));
@@ -1363,6 +1503,9 @@ int om_newton(const om_problem *p, double *x, const om_solver_opts *o, om_report
     memset(resid, 0, N * sizeof(double));
     memset(dx, 0, N * sizeof(double));
     memset(val, 0, (size_t)nv * sizeof(double));
+    /* the PSPG velocity norm from this pass's x (src/mm_sol_nonlinear.c:
+     * 1183-1192); the line search's residual fills keep it */
+    if (p->ph.pspg) p->U_norm = om_velocity_norm(p, x);
     double ta = now_s();
     if (om_fill(p, x, resid, val) < 0) {
       ret = -1;

@@ -68,6 +68,8 @@ typedef struct om_physics {
   double g[3];    /* momentum_source (gravity) */
   double beta;    /* Volume_Expansion */
   double Tref;
+  int pspg;           /* Pressure Stabilization: 0 no, 1 global (PSPG = 1) */
+  double ps_scaling;  /* Pressure Stabilization Scaling */
 } om_physics;
 
 typedef struct om_bc {
@@ -180,7 +182,20 @@ int om_ilu0_csr(int N, const double *val, const int *ija, const int *perm, int *
 
 /* Newton loop (solve_nonlinear_problem, steady, full step).  x is in/out.
  * Returns Goma's return value (passes through the loop) or -1. */
-int om_newton(const om_problem *p, double *x, const om_solver_opts *opt, om_report *rep);
+int om_newton(om_problem *p, double *x, const om_solver_opts *opt, om_report *rep);
+/* PSPG inputs.  global_h_elem_siz (src/mm_fill_aux.c:1128-1200): the mean
+ * over elements of sqrt(sum_p hsquared[p] / dim), hsquared from h_elem_siz's
+ * face-centroid widths (src/mm_fill_aux.c:844-1125).  global_velocity_norm
+ * (src/mm_fill_aux.c:612-672): the mean of x^2 over the velocity unknowns
+ * (a mean square, not its root).  om_newton sets the velocity norm from x at
+ * the top of each pass (src/mm_sol_nonlinear.c:1183-1192); a standalone
+ * om_fill uses the last value set. */
+double om_h_elem_avg(const om_problem *p);
+double om_velocity_norm(const om_problem *p, const double *x);
+void om_set_velocity_norm(om_problem *p, double U_norm);
+/* tau_pspg of the global scheme at the current velocity norm
+ * (src/mm_fill_stabilization.c:1037-1049). */
+double om_pspg_tau(const om_problem *p);
 
 #ifdef __cplusplus
 }

@@ -29,7 +29,8 @@ class OmPhysics(C.Structure):
                 ("mu0", C.c_double), ("muinf", C.c_double), ("lam", C.c_double),
                 ("nexp", C.c_double), ("aexp", C.c_double), ("Cp", C.c_double),
                 ("kcond", C.c_double), ("hsrc", C.c_double), ("g", C.c_double * 3),
-                ("beta", C.c_double), ("Tref", C.c_double)]
+                ("beta", C.c_double), ("Tref", C.c_double), ("pspg", C.c_int),
+                ("ps_scaling", C.c_double)]
 
 
 class OmBC(C.Structure):
@@ -105,6 +106,12 @@ def lib():
         L.om_gmres_perm.argtypes = [C.c_int, _dp, _ip, _dp, _dp, C.c_void_p, C.c_void_p, C.c_int,
                                     C.c_int, C.c_double, C.POINTER(C.c_double)]
         L.om_newton.argtypes = [C.c_void_p, _dp, C.POINTER(OmSolverOpts), C.POINTER(OmReport)]
+        for nm in ("om_h_elem_avg", "om_pspg_tau"):
+            getattr(L, nm).restype = C.c_double
+            getattr(L, nm).argtypes = [C.c_void_p]
+        L.om_velocity_norm.restype = C.c_double
+        L.om_velocity_norm.argtypes = [C.c_void_p, _dp]
+        L.om_set_velocity_norm.argtypes = [C.c_void_p, C.c_double]
         _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
         L.omlp_num_params.restype = C.c_longlong
         L.omlp_num_params.argtypes = [C.POINTER(OmlpShape)]
@@ -133,7 +140,8 @@ class Oracle:
         om = OmMesh(m.dim, m.num_nodes, m.num_elems, self._coor.ctypes.data, self._conn.ctypes.data)
         ph = prob.phys
         op = OmPhysics(ph.energy, ph.visc_model, ph.rho, ph.mu0, ph.muinf, ph.lam, ph.nexp, ph.aexp,
-                       ph.Cp, ph.kcond, ph.hsrc, (C.c_double * 3)(*ph.g), ph.beta, ph.Tref)
+                       ph.Cp, ph.kcond, ph.hsrc, (C.c_double * 3)(*ph.g), ph.beta, ph.Tref,
+                       int(getattr(ph, "pspg", 0)), float(getattr(ph, "ps_scaling", 0.0)))
         ob = OmBC(len(self._bn), self._bn.ctypes.data, self._bv.ctypes.data, self._bx.ctypes.data,
                   prob.datum_elem, prob.datum_value)
         self.h = L.om_create(C.byref(om), C.byref(op), C.byref(ob))
@@ -162,6 +170,21 @@ class Oracle:
         e1 = self.prob.mesh.num_elems if e1 is None else e1
         rc = lib().om_fill_range(self.h, x, r, v, e0, e1)
         return rc, r, v
+
+    def h_elem_avg(self) -> float:
+        """global_h_elem_siz: the PSPG element size."""
+        return lib().om_h_elem_avg(self.h)
+
+    def velocity_norm(self, x) -> float:
+        """global_velocity_norm: the mean square of the velocity unknowns."""
+        return lib().om_velocity_norm(self.h, np.ascontiguousarray(x, dtype=np.float64))
+
+    def set_velocity_norm(self, U: float):
+        """The PSPG velocity norm the next fills use (newton sets it per pass)."""
+        lib().om_set_velocity_norm(self.h, float(U))
+
+    def pspg_tau(self) -> float:
+        return lib().om_pspg_tau(self.h)
 
     def add_qside(self, elems, sides, q):
         """QSIDE heat flux q (into the domain) on element sides (Exodus

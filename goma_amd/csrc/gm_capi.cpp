@@ -109,7 +109,7 @@ static void free_ctx(gm_ctx *c) {
                   c->d_resid, c->d_dx,    c->d_scale, c->d_val32,    c->d_red,   c->d_xwork,
                   c->d_xsave, c->d_rtrial, c->d_w, c->d_mg_rows, c->d_mg_wmul, c->d_kel,
                   c->d_mg_gptr, c->d_mg_gent, c->d_mg_corr, c->d_stab, c->d_qs_ptr,
-                  c->d_qs_side, c->d_qs_q, c->d_rank};
+                  c->d_qs_side, c->d_qs_q, c->d_rank, c->d_nn_ptr, c->d_nn_list};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->h_red) hipHostFree(c->h_red);
@@ -195,6 +195,15 @@ static int create_impl(const gm_mesh *m, const gm_physics *ph, const gm_bc *bc, 
   if ((rc = upload(&c->d_dbc, c->dbc.data(), c->dbc.size()))) return rc;
   if ((rc = upload(&c->d_dbc_val, c->dbc_val.data(), c->dbc_val.size()))) return rc;
   if ((rc = upload(&c->d_bindx, c->bindx.data(), c->bindx.size()))) return rc;
+  c->nn_cap = 0;
+  for (int n = 0; n < c->num_nodes; n++) {
+    int len = 0;
+    for (int q = c->nn_ptr[n]; q < c->nn_ptr[n + 1]; q++)
+      len += c->first_unknown[c->nn_list[q] + 1] - c->first_unknown[c->nn_list[q]];
+    c->nn_cap = std::max(c->nn_cap, len);
+  }
+  if ((rc = upload(&c->d_nn_ptr, c->nn_ptr.data(), c->nn_ptr.size()))) return rc;
+  if ((rc = upload(&c->d_nn_list, c->nn_list.data(), c->nn_list.size()))) return rc;
   if ((rc = upload(&c->d_ecol_elems, c->ecol_elems.data(), c->ecol_elems.size()))) return rc;
   GM_HIP(hipMalloc(&c->d_flag, 8 * sizeof(int)));
   GM_HIP(hipMemset(c->d_flag, 0, 8 * sizeof(int)));

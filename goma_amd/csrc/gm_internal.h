@@ -116,6 +116,7 @@ struct MgLevel {
 };
 
 #define GM_MG_DIRECT_MAX 4096 /* largest coarsest level solved exactly */
+#define GM_NODE_CAP_MAX 2048 /* longest node pattern the per-node row kernels take (LDS) */
 
 struct gm_ctx {
   int dim = 0, nve = 0, np = 0, nv = 0, num_nodes = 0, num_elems = 0, energy = 0;
@@ -143,6 +144,8 @@ struct gm_ctx {
   signed char *d_dbc = nullptr;
   double *d_dbc_val = nullptr;
   int *d_bindx = nullptr;
+  int *d_nn_ptr = nullptr, *d_nn_list = nullptr;  // node-node graph (the row kernels' runs)
+  int nn_cap = 0;  // longest column pattern of a node (sum of its neighbours' unknowns)
   int *d_ecol_elems = nullptr;
   int *d_e2nz = nullptr;        // [num_elems * lay.total]
   int *d_flag = nullptr;        // device error flags

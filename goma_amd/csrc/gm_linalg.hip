@@ -237,8 +237,21 @@ __global__ __launch_bounds__(256) void k_spmv_msr(int N, const VT *__restrict__ 
 
 template <typename VT, typename XT, typename YT>
 static void spmv(gm_ctx *c, const VT *a, const XT *x, YT *y) {
-  const double bytes = (double)(c->nval - 1) * (sizeof(VT) + 4) + 4.0 * (c->N + 1) +
-                       (double)c->N * (sizeof(XT) + sizeof(YT));
+  using AT = typename std::conditional<sizeof(VT) == 8 || sizeof(XT) == 8, double, float>::type;
+  const double vec = (double)c->N * (sizeof(XT) + sizeof(YT));
+  if (c->dim == 3 && c->nn_cap <= GM_NODE_CAP_MAX) {
+    // HEX27: one wave per node (k_msr_nodes), values, node graph and row
+    // pointers (QUAD9 rows, ~60 entries, are faster per row: 0.156 vs 0.345 ms)
+    const double bytes = (double)(c->nval - 1) * sizeof(VT) +
+                         4.0 * (2.0 * (c->num_nodes + 1) + c->N + c->nn_list.size()) + vec;
+    gm_launch(c, GM_K_SPMV, bytes, k_msr_nodes<AT, VT, XT, YT, false>, dim3((c->num_nodes + 3) / 4),
+              dim3(256), (size_t)4 * 2 * c->nn_cap * sizeof(int), c->stream, c->num_nodes, c->dim,
+              c->nv, c->nn_cap, c->d_fu, c->d_nn_ptr, c->d_nn_list, c->d_bindx, a, x,
+              (const XT *)nullptr, y);
+    return;
+  }
+  // G lanes per row
+  const double bytes = (double)(c->nval - 1) * (sizeof(VT) + 4) + 4.0 * (c->N + 1) + vec;
   if (c->dim == 2) {
     constexpr int G = 8;  // 2D rows hold ~50-250 entries: measured 156 vs 172 us at 256^2 (G = 16)
     const int blocks = (int)(((long long)c->N * G + 255) / 256);

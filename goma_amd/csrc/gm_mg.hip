@@ -1201,8 +1201,20 @@ static void resid_g(gm_ctx *lc, hipStream_t s, const VT *a, const BT *x, const B
 
 template <typename VT, typename BT>
 static void resid(gm_ctx *lc, hipStream_t s, const VT *a, const BT *x, const BT *b, BT *r) {
-  if (lc->dim == 3) resid_g<VT, BT, 64>(lc, s, a, x, b, r);
-  else resid_g<VT, BT, 8>(lc, s, a, x, b, r);
+  if (lc->dim == 2 || lc->nn_cap > GM_NODE_CAP_MAX) {  // G lanes per row
+    if (lc->dim == 3) resid_g<VT, BT, 64>(lc, s, a, x, b, r);
+    else resid_g<VT, BT, 8>(lc, s, a, x, b, r);
+    return;
+  }
+  // HEX27: one wave per node (k_msr_nodes)
+  using AT = typename std::conditional<sizeof(BT) == 8, double, float>::type;
+  const double bytes = (double)(lc->nval - 1) * sizeof(VT) +
+                       4.0 * (2.0 * (lc->num_nodes + 1) + lc->N + lc->nn_list.size()) +
+                       3.0 * lc->N * sizeof(BT);
+  gm_launch(g_prof_ctx, GM_K_MG_RESID, bytes, k_msr_nodes<AT, VT, BT, BT, true>,
+            dim3((lc->num_nodes + 3) / 4), dim3(256), (size_t)4 * 2 * lc->nn_cap * sizeof(int), s,
+            lc->num_nodes, lc->dim, lc->nv, lc->nn_cap, lc->d_fu, lc->d_nn_ptr, lc->d_nn_list,
+            lc->d_bindx, a, x, b, r);
 }
 
 template <typename VT, typename BT, int G>

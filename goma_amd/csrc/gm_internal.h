@@ -107,7 +107,7 @@ struct MgLevel {
   int nnz = 0;                // entries of P
   // coarsest level only: dense FP64 inverse for the exact coarse solve
   double *d_dense = nullptr;  // [n][n]
-  double *d_pcol = nullptr, *d_prow = nullptr;  // pivot column / row of one elimination step
+  double *d_pcol = nullptr;  // pivot rows of one panel of the coarsest inverse [GJ_NB][n]
   int *d_perm = nullptr;      // row interchange of each step
   int dense_n = 0;
   // host copy of P, kept by gm_mg_build (gm_mg_copy_prolongation)

@@ -368,66 +368,199 @@ __global__ void k_msr_to_dense(int n, const VT *__restrict__ a, const int *__res
 
 // step k, one block: pivot row p = argmax_{i>=k} |D[i][k]| (first on ties),
 // swap rows k and p, keep copies of the pivot column and row
-__global__ __launch_bounds__(256) void k_gj_pivot(int n, int k, double *__restrict__ D,
-                                                  int *__restrict__ perm, double *__restrict__ pcol,
-                                                  double *__restrict__ prow, int *__restrict__ flag) {
-  __shared__ double sv[256];
-  __shared__ int si[256];
-  const int tid = threadIdx.x;
-  double best = -1.0;
-  int bi = n;
-  for (int i = k + tid; i < n; i += 256) {
-    const double v = fabs(D[(size_t)i * n + k]);
-    if (v > best) {
-      best = v;
-      bi = i;
-    }
+// Dense Gauss-Jordan inverse of the coarsest level with partial pivoting,
+// blocked in panels of at most GJ_NB columns (same pivots and interchanges as
+// the column-at-a-time algorithm).  For panel K:
+//   k_gjb_panel    one workgroup runs the panel's column steps on the panel
+//                  columns only, the panel in registers (pivot: largest
+//                  |entry| at or below the diagonal, first on ties; whole
+//                  panel rows interchanged);
+//   k_gjb_swap     the panel's interchanges on every other column, in order,
+//                  and a copy of the pivot rows B_K of those columns;
+//   k_gjb_update   every other column: B <- B' + Pan B_K, B' = B with the
+//                  pivot rows zeroed (the panel now holds the K columns of
+//                  the accumulated elimination, so pivot rows become
+//                  A_KK^-1 B_K and the others B_R - A_RK A_KK^-1 B_K).
+// D is row-major n x n.  perm[k] = the row interchanged with k at step k.
+constexpr int GJ_NB = 32;
+
+// The panel lives in registers for its column steps: thread t of 512 owns
+// rows t + 512 r (r < RPT) of the NB panel columns (RPT NB <= 64 doubles),
+// so D is read and written once per panel; the pivot row and the
+// interchange go through LDS.
+template <int NB, int RPT>
+__global__ __launch_bounds__(512) void k_gjb_panel(int n, int kb0, int nb, double *__restrict__ D,
+                                                    int *__restrict__ perm, int *__restrict__ flag) {
+  __shared__ double s_rk[NB], s_rp[NB];
+  __shared__ double sv[8];
+  __shared__ int si[8];
+  __shared__ int s_p;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  double P[RPT][NB];
+#pragma unroll
+  for (int r = 0; r < RPT; r++) {
+    const int i = tid + 512 * r;
+#pragma unroll
+    for (int j = 0; j < NB; j++) P[r][j] = (i < n && j < nb) ? D[(size_t)i * n + kb0 + j] : 0.0;
   }
-  sv[tid] = best;
-  si[tid] = bi;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (tid < s) {
-      const double v = sv[tid + s];
-      const int i = si[tid + s];
-      if (v > sv[tid] || (v == sv[tid] && i < si[tid])) {
-        sv[tid] = v;
-        si[tid] = i;
+#pragma unroll
+  for (int kk = 0; kk < NB; kk++) {
+    if (kk < nb) {
+    const int k = kb0 + kk;
+    double best = -1.0;
+    int bi = n;
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+      const int i = tid + 512 * r;
+      const double v = fabs(P[r][kk]);
+      if (i >= k && i < n && v > best) {
+        best = v;
+        bi = i;
       }
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ov > best || (ov == best && oi < bi)) {
+        best = ov;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      sv[w] = best;
+      si[w] = bi;
+    }
     __syncthreads();
+    if (tid == 0) {
+      double bv = sv[0];
+      int bix = si[0];
+      for (int q = 1; q < 8; q++)
+        if (sv[q] > bv || (sv[q] == bv && si[q] < bix)) {
+          bv = sv[q];
+          bix = si[q];
+        }
+      if (!(bv > 0.0)) flag[3] = 1;  // singular coarse matrix
+      if (bix >= n) bix = k;
+      s_p = bix;
+      perm[k] = bix;
+    }
+    __syncthreads();
+    const int p = s_p;
+    // rows k and p to LDS (row p is the new pivot row), then interchanged
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+      const int i = tid + 512 * r;
+      if (i == p)
+#pragma unroll
+        for (int j = 0; j < NB; j++) s_rp[j] = P[r][j];
+      if (i == k && p != k)
+#pragma unroll
+        for (int j = 0; j < NB; j++) s_rk[j] = P[r][j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+      const int i = tid + 512 * r;
+      if (p != k && i == k)
+#pragma unroll
+        for (int j = 0; j < NB; j++) P[r][j] = s_rp[j];
+      if (p != k && i == p)
+#pragma unroll
+        for (int j = 0; j < NB; j++) P[r][j] = s_rk[j];
+    }
+    const double piv = s_rp[kk];
+    const double inv = piv != 0.0 ? 1.0 / piv : 0.0;
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+      const int i = tid + 512 * r;
+      const double f = P[r][kk];
+      if (i == k) {
+#pragma unroll
+        for (int j = 0; j < NB; j++) P[r][j] = j == kk ? inv : s_rp[j] * inv;
+      } else {
+        const double g = f * inv;
+#pragma unroll
+        for (int j = 0; j < NB; j++) P[r][j] = j == kk ? -g : P[r][j] - g * s_rp[j];
+      }
+    }
+    __syncthreads();  // s_rp / s_rk reused by the next step
+    }
   }
-  const int p = si[0];
-  if (tid == 0) {
-    perm[k] = p;
-    if (!(sv[0] > 0.0)) flag[3] = 1;  // singular coarse matrix
+#pragma unroll
+  for (int r = 0; r < RPT; r++) {
+    const int i = tid + 512 * r;
+    if (i < n)
+#pragma unroll
+      for (int j = 0; j < NB; j++)
+        if (j < nb) D[(size_t)i * n + kb0 + j] = P[r][j];
   }
-  if (p != k)
-    for (int j = tid; j < n; j += 256) {
+}
+
+__global__ __launch_bounds__(256) void k_gjb_swap(int n, int kb0, int nb, double *__restrict__ D,
+                                                  const int *__restrict__ perm,
+                                                  double *__restrict__ BK) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n || (j >= kb0 && j < kb0 + nb)) return;
+  for (int kk = 0; kk < nb; kk++) {
+    const int k = kb0 + kk, p = perm[k];
+    if (p != k) {
       const double t = D[(size_t)k * n + j];
       D[(size_t)k * n + j] = D[(size_t)p * n + j];
       D[(size_t)p * n + j] = t;
     }
+  }
+  for (int kk = 0; kk < nb; kk++) BK[(size_t)kk * n + j] = D[(size_t)(kb0 + kk) * n + j];
+}
+
+// 64 x 64 output tile per workgroup, 4 x 4 per thread; columns of the panel
+// are left alone
+__global__ __launch_bounds__(256) void k_gjb_update(int n, int kb0, int nb, double *__restrict__ D,
+                                                    const double *__restrict__ BK) {
+  __shared__ double sP[64][GJ_NB + 1];
+  __shared__ double sB[GJ_NB][64];
+  const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64, tid = threadIdx.x;
+  for (int idx = tid; idx < 64 * GJ_NB; idx += 256) {
+    const int r = idx / GJ_NB, t = idx - r * GJ_NB, i = i0 + r;
+    sP[r][t] = (i < n && t < nb) ? D[(size_t)i * n + kb0 + t] : 0.0;
+  }
+  for (int idx = tid; idx < GJ_NB * 64; idx += 256) {
+    const int t = idx >> 6, c = idx & 63, j = j0 + c;
+    sB[t][c] = (t < nb && j < n) ? BK[(size_t)t * n + j] : 0.0;
+  }
   __syncthreads();
-  for (int i = tid; i < n; i += 256) {
-    pcol[i] = D[(size_t)i * n + k];
-    prow[i] = D[(size_t)k * n + i];
+  const int tr = tid >> 4, tc = tid & 15;
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[a][c] = 0.0;
+  for (int t = 0; t < nb; t++) {
+    double pa[4], pb[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) pa[a] = sP[tr + 16 * a][t];
+#pragma unroll
+    for (int c = 0; c < 4; c++) pb[c] = sB[t][tc + 16 * c];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) acc[a][c] += pa[a] * pb[c];
+  }
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    const int i = i0 + tr + 16 * a;
+    if (i >= n) continue;
+    const bool krow = i >= kb0 && i < kb0 + nb;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int j = j0 + tc + 16 * c;
+      if (j >= n || (j >= kb0 && j < kb0 + nb)) continue;
+      const size_t at = (size_t)i * n + j;
+      D[at] = (krow ? 0.0 : D[at]) + acc[a][c];
+    }
   }
 }
 
-__global__ void k_gj_elim(int n, int k, double *__restrict__ D, const double *__restrict__ pcol,
-                          const double *__restrict__ prow) {
-  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (idx >= (long long)n * n) return;
-  const int i = (int)(idx / n), j = (int)(idx - (long long)i * n);
-  const double a = prow[k];
-  if (a == 0.0) return;  // singular: flagged by k_gj_pivot
-  if (i == k) D[idx] = j == k ? 1.0 / a : prow[j] / a;
-  else if (j == k) D[idx] = -pcol[i] / a;
-  else D[idx] -= pcol[i] * prow[j] / a;
-}
-
-// inverse of A = inverse of (P A) with the columns swapped back, last step first
 __global__ void k_gj_unpermute(int n, const int *__restrict__ perm, double *__restrict__ D) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1076,14 +1209,13 @@ static int coarse_inverse(gm_ctx *c, MgLevel &L, int prec) {
     return GM_ERR_ARG;
   }
   if (L.dense_n != n) {
-    for (void *p : {(void *)L.d_dense, (void *)L.d_pcol, (void *)L.d_prow, (void *)L.d_perm})
+    for (void *p : {(void *)L.d_dense, (void *)L.d_pcol, (void *)L.d_perm})
       if (p) (void)hipFree(p);
-    L.d_dense = L.d_pcol = L.d_prow = nullptr;
+    L.d_dense = L.d_pcol = nullptr;
     L.d_perm = nullptr;
     L.dense_n = 0;
     GM_HIP(hipMalloc(&L.d_dense, (size_t)n * n * sizeof(double)));
-    GM_HIP(hipMalloc(&L.d_pcol, (size_t)n * sizeof(double)));
-    GM_HIP(hipMalloc(&L.d_prow, (size_t)n * sizeof(double)));
+    GM_HIP(hipMalloc(&L.d_pcol, (size_t)GJ_NB * n * sizeof(double)));  // pivot rows B_K
     GM_HIP(hipMalloc(&L.d_perm, (size_t)n * sizeof(int)));
     L.dense_n = n;
   }
@@ -1095,10 +1227,19 @@ static int coarse_inverse(gm_ctx *c, MgLevel &L, int prec) {
   else
     k_msr_to_dense<double><<<n, 64, 0, s>>>(n, (const double *)L.d_val, lc->d_bindx, L.d_dense);
   GM_HIP(hipMemsetAsync(c->d_flag + 3, 0, sizeof(int), s));
-  const int eb = (int)((nn + 255) / 256);
-  for (int k = 0; k < n; k++) {
-    k_gj_pivot<<<1, 256, 0, s>>>(n, k, L.d_dense, L.d_perm, L.d_pcol, L.d_prow, c->d_flag);
-    k_gj_elim<<<eb, 256, 0, s>>>(n, k, L.d_dense, L.d_pcol, L.d_prow);
+  const int nt = (n + 63) / 64;
+  // panel width x rows per thread (512 threads; wider panels left the step
+  // loop partly rolled, the panel in scratch)
+  const int pw = n <= 2048 ? 16 : 8;
+  for (int kb0 = 0; kb0 < n; kb0 += pw) {
+    const int nb = std::min(pw, n - kb0);
+    if (n <= 512) k_gjb_panel<16, 1><<<1, 512, 0, s>>>(n, kb0, nb, L.d_dense, L.d_perm, c->d_flag);
+    else if (n <= 1024) k_gjb_panel<16, 2><<<1, 512, 0, s>>>(n, kb0, nb, L.d_dense, L.d_perm, c->d_flag);
+    else if (n <= 1536) k_gjb_panel<16, 3><<<1, 512, 0, s>>>(n, kb0, nb, L.d_dense, L.d_perm, c->d_flag);
+    else if (n <= 2048) k_gjb_panel<16, 4><<<1, 512, 0, s>>>(n, kb0, nb, L.d_dense, L.d_perm, c->d_flag);
+    else k_gjb_panel<8, 8><<<1, 512, 0, s>>>(n, kb0, nb, L.d_dense, L.d_perm, c->d_flag);
+    k_gjb_swap<<<(n + 255) / 256, 256, 0, s>>>(n, kb0, nb, L.d_dense, L.d_perm, L.d_pcol);
+    k_gjb_update<<<dim3(nt, nt), 256, 0, s>>>(n, kb0, nb, L.d_dense, L.d_pcol);
   }
   k_gj_unpermute<<<(n + 255) / 256, 256, 0, s>>>(n, L.d_perm, L.d_dense);
   GM_HIP(hipGetLastError());
@@ -1387,7 +1528,7 @@ void gm_mg_free(gm_ctx *c) {
     void *p[] = {L.d_Prow, L.d_Pcol, L.d_Pval, L.d_Rrow, L.d_Rcol, L.d_Rval, L.d_inject,
                  L.d_state, L.d_rdummy, L.d_x, L.d_b, L.d_r, L.d_val_p[0], L.d_val_p[1],
                  L.d_ainv_p[0], L.d_ainv_p[1],
-                 L.d_dense, L.d_pcol, L.d_prow, L.d_perm, L.d_child, L.d_W, L.d_Q};
+                 L.d_dense, L.d_pcol, L.d_perm, L.d_child, L.d_W, L.d_Q};
     for (void *q : p)
       if (q) (void)hipFree(q);
   };

@@ -82,14 +82,19 @@ __global__ __launch_bounds__(256) void k_row_sum_scale(int N, VT *__restrict__ a
   }
 }
 
-int gm_scale_launch(gm_ctx *c, double *val, float *val32, double *b, double *scale) {
-  const int G = 16;
-  const long long thr = (long long)c->N * G;
-  const int blocks = (int)((thr + 255) / 256);
+template <int G>
+static void scale_g(gm_ctx *c, double *val, float *val32, double *b, double *scale) {
+  const int blocks = (int)(((long long)c->N * G + 255) / 256);
   if (val32)
     k_row_sum_scale<float, G><<<blocks, 256, 0, c->stream>>>(c->N, val32, c->d_bindx, b, scale);
   else
     k_row_sum_scale<double, G><<<blocks, 256, 0, c->stream>>>(c->N, val, c->d_bindx, b, scale);
+}
+
+int gm_scale_launch(gm_ctx *c, double *val, float *val32, double *b, double *scale) {
+  // a wave per HEX27 row (~250 entries), 16 lanes per QUAD9 row (~60)
+  if (c->dim == 3) scale_g<64>(c, val, val32, b, scale);
+  else scale_g<16>(c, val, val32, b, scale);
   GM_HIP(hipGetLastError());
   return GM_OK;
 }

@@ -799,7 +799,7 @@ static std::string build_galerkin_maps(const gm_ctx *f, const gm_ctx *c, const i
 // Coarse element matrices P_e^T K_e P_e summed over the children of each coarse
 // element (one workgroup per coarse element), in the e2nz layout: nodal blocks
 // (a, b) as [i][j], then momentum x P, continuity x velocity, P x P.
-template <int DIM, int EN>
+template <int DIM, int EN, typename CT>
 __global__ __launch_bounds__(256) void k_galerkin(const double *__restrict__ kf,
                                                   const int *__restrict__ child,
                                                   const double *__restrict__ W,
@@ -811,12 +811,12 @@ __global__ __launch_bounds__(256) void k_galerkin(const double *__restrict__ kf,
   constexpr int NMP = DIM * NVE * NP;          // momentum x P (= continuity x velocity) entries
   constexpr int NPC = 2 * NMP + NP * NP;       // pressure couplings, contiguous from off_mp
   constexpr int RP = (NPC + NT - 1) / NT;
-  __shared__ double sW[NVE2], sK[NVE2], sT[NVE2], sQ[NP * NP], sKp[NPC];
+  __shared__ CT sW[NVE2], sK[NVE2], sT[NVE2], sQ[NP * NP], sKp[NPC];
   const int ec = blockIdx.x, tid = threadIdx.x;
   const int *ch = child + (size_t)ec * NCH;
   double *__restrict__ out = kc + (size_t)ec * E;
   for (int blk = 0; blk < NV * NV; blk++) {
-    double acc[RN];
+    CT acc[RN];
 #pragma unroll
     for (int r = 0; r < RN; r++) acc[r] = 0.0;
     for (int sidx = 0; sidx < NCH; sidx++) {
@@ -824,13 +824,13 @@ __global__ __launch_bounds__(256) void k_galerkin(const double *__restrict__ kf,
       const double *Kf = kf + (size_t)f * E + (size_t)blk * NVE2;
       const double *Wf = W + (size_t)f * NVE2;
       for (int t = tid; t < NVE2; t += NT) {
-        sK[t] = Kf[t];
-        sW[t] = Wf[t];
+        sK[t] = (CT)Kf[t];
+        sW[t] = (CT)Wf[t];
       }
       __syncthreads();
       for (int t = tid; t < NVE2; t += NT) {  // T = K W
         const int i = t / NVE, n = t - i * NVE;
-        double v = 0.0;
+        CT v = 0;
         for (int j = 0; j < NVE; j++) v += sK[i * NVE + j] * sW[j * NVE + n];
         sT[t] = v;
       }
@@ -840,7 +840,7 @@ __global__ __launch_bounds__(256) void k_galerkin(const double *__restrict__ kf,
         const int t = tid + r * NT;
         if (t < NVE2) {
           const int m = t / NVE, n = t - m * NVE;
-          double v = 0.0;
+          CT v = 0;
           for (int i = 0; i < NVE; i++) v += sW[i * NVE + m] * sT[i * NVE + n];
           acc[r] += v;
         }
@@ -853,25 +853,25 @@ __global__ __launch_bounds__(256) void k_galerkin(const double *__restrict__ kf,
       if (t < NVE2) out[(size_t)blk * NVE2 + t] = acc[r];
     }
   }
-  double accp[RP];
+  CT accp[RP];
 #pragma unroll
   for (int r = 0; r < RP; r++) accp[r] = 0.0;
   for (int sidx = 0; sidx < NCH; sidx++) {
     const int f = ch[sidx];
     const double *Kf = kf + (size_t)f * E + off_mp;
-    for (int t = tid; t < NVE2; t += NT) sW[t] = W[(size_t)f * NVE2 + t];
-    if (tid < NP * NP) sQ[tid] = Q[(size_t)f * NP * NP + tid];
-    for (int t = tid; t < NPC; t += NT) sKp[t] = Kf[t];
+    for (int t = tid; t < NVE2; t += NT) sW[t] = (CT)W[(size_t)f * NVE2 + t];
+    if (tid < NP * NP) sQ[tid] = (CT)Q[(size_t)f * NP * NP + tid];
+    for (int t = tid; t < NPC; t += NT) sKp[t] = (CT)Kf[t];
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < RP; r++) {
       const int t = tid + r * NT;
       if (t >= NPC) continue;
-      double v = 0.0;
+      CT v = 0;
       if (t < NMP) {  // row (a, m) momentum, column l pressure
         const int a = t / (NVE * NP), rem = t - a * NVE * NP, m = rem / NP, l = rem - m * NP;
         for (int i = 0; i < NVE; i++) {
-          double q = 0.0;
+          CT q = 0;
 #pragma unroll
           for (int k = 0; k < NP; k++) q += sKp[a * NVE * NP + i * NP + k] * sQ[k * NP + l];
           v += sW[i * NVE + m] * q;
@@ -881,7 +881,7 @@ __global__ __launch_bounds__(256) void k_galerkin(const double *__restrict__ kf,
         const int l = rem / NVE, n = rem - l * NVE;
 #pragma unroll
         for (int k = 0; k < NP; k++) {
-          double q = 0.0;
+          CT q = 0;
           for (int j = 0; j < NVE; j++) q += sKp[NMP + b * NP * NVE + k * NVE + j] * sW[j * NVE + n];
           v += sQ[k * NP + l] * q;
         }
@@ -964,21 +964,12 @@ static void galerkin_level(gm_ctx *prof, const gm_ctx *above, MgLevel &L, JT *va
   const int nch = 1 << lc->dim;
   const double gbytes = ((double)above->num_elems * (ly.total + lc->nve * lc->nve + lc->np * lc->np) +
                          (double)lc->num_elems * ly.total) * 8.0 + 4.0 * nch * lc->num_elems;
-  if (lc->dim == 2) {
-    if (lc->energy)
-      gm_launch(prof, GM_K_GALERKIN, gbytes, k_galerkin<2, 1>, dim3(lc->num_elems), dim3(256), 0,
-                lc->stream, above->d_kel, L.d_child, L.d_W, L.d_Q, ly.total, ly.off_mp, lc->d_kel);
-    else
-      gm_launch(prof, GM_K_GALERKIN, gbytes, k_galerkin<2, 0>, dim3(lc->num_elems), dim3(256), 0,
-                lc->stream, above->d_kel, L.d_child, L.d_W, L.d_Q, ly.total, ly.off_mp, lc->d_kel);
-  } else {
-    if (lc->energy)
-      gm_launch(prof, GM_K_GALERKIN, gbytes, k_galerkin<3, 1>, dim3(lc->num_elems), dim3(256), 0,
-                lc->stream, above->d_kel, L.d_child, L.d_W, L.d_Q, ly.total, ly.off_mp, lc->d_kel);
-    else
-      gm_launch(prof, GM_K_GALERKIN, gbytes, k_galerkin<3, 0>, dim3(lc->num_elems), dim3(256), 0,
-                lc->stream, above->d_kel, L.d_child, L.d_W, L.d_Q, ly.total, ly.off_mp, lc->d_kel);
-  }
+  // products in the precision of the coarse matrices (FP32 under mixed precision)
+  using CT = JT;
+  auto kern = lc->dim == 2 ? (lc->energy ? k_galerkin<2, 1, CT> : k_galerkin<2, 0, CT>)
+                           : (lc->energy ? k_galerkin<3, 1, CT> : k_galerkin<3, 0, CT>);
+  gm_launch(prof, GM_K_GALERKIN, gbytes, kern, dim3(lc->num_elems), dim3(256), 0, lc->stream,
+            above->d_kel, L.d_child, L.d_W, L.d_Q, ly.total, ly.off_mp, lc->d_kel);
   for (int col = 0; col < lc->n_ecol; col++) {
     const int cnt = lc->ecol_ptr[col + 1] - lc->ecol_ptr[col];
     if (!cnt) continue;

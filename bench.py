@@ -238,8 +238,10 @@ def kernel_roofline(name, k, prob, prec, workload):
         flops = k["bytes_per_launch"] * 2.0 * nloc / (2 * s + 4)
         tf = flops / (k["avg_ms"] * 1e-3) / 1e12
         peak = FP64_PEAK_TFS if prec == 0 else FP32_PEAK_TFS
+        tr = recorded_traffic(workload, name) if prec == 0 else None
         return dict(kernel=name, bound="mfma", achieved=round(tf, 2), peak=peak, unit="TFLOP/s",
-                    frac=round(tf / peak, 4), traffic=None, flops_per_launch=round(flops),
+                    frac=round(tf / peak, 4), traffic=round(tr) if tr else None,
+                    flops_per_launch=round(flops),
                     avg_launch_ms=round(k["avg_ms"], 5),
                     precision="fp64" if prec == 0 else "fp32")
     a = k["GBps"] or 0.0

@@ -611,6 +611,35 @@ def sweep_bench(args, torch, capi):
     return 0
 
 
+def ilu_config3(args, torch, capi):
+    """Config 3's solver as specified: FP32 Jacobian values and ILU(0), inner
+    GMRES(50), FP64 residual and Newton update (precision = mixed,
+    preconditioner ILU(0) in patch order), from the zero guess on the 48^3
+    cavity, GMRES capped at max-its iterations per pass and the fixed linear
+    tolerance 1e-6 (Goma's Residual Ratio Tolerance default).  Prints one JSON
+    line with the per-pass GMRES iterations and scaled residual norms; not a
+    timed benchmark (the multigrid solve is the headline)."""
+    prob, _ = make_problem("cavity3d", args.n)
+    ctx = capi.Context(prob)
+    x = torch.zeros(ctx.N, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rc, rep = ctx.solve_dev(x.data_ptr(), precond=1, precision=1, kspace=50, max_its=args.max_its,
+                            max_newton=15, line_search=1, lin_tol=1e-6, lin_tol_adapt=0,
+                            patch_bins=args.patch_bins, verbose=1 if args.verbose else 0)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    n = min(rep.newton_its, 64)
+    print(json.dumps({"workload": f"{prob.name}: FP32 ILU(0) + GMRES(50), FP64 residual",
+                      "converged": bool(rep.converged), "newton_passes": rep.newton_its,
+                      "gmres_its_total": rep.lin_its_total, "seconds": round(dt, 2),
+                      "gmres_its": [int(rep.lin_its[i]) for i in range(n)],
+                      "resid_L2": [float(rep.resid_L2[i]) for i in range(n)],
+                      "max_its_per_pass": args.max_its}))
+    ctx.close()
+    return 0
+
+
 def copy_calibration(torch):
     """A plain device copy of known size (1 GiB read, 1 GiB written, three
     times): the reference for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE
@@ -637,7 +666,8 @@ def main(argv=None):
     ap.add_argument("--workload", default="cavity3d",
                     help="cavity3d (48^3 HEX27 lid cavity + energy, the default), natconv2d "
                          "(256^2 Boussinesq), carreau2d, cavity2d, sweep (config-4 Ra sweep and "
-                         "learned guess), copycal (PMC counter calibration copy)")
+                         "learned guess), copycal (PMC counter calibration copy), cavity3d-ilu "
+                         "(config 3's FP32 ILU(0) + GMRES(50) solver at 48^3, one solve)")
     ap.add_argument("--n", type=int, default=None, help="elements per side (default: workload's)")
     ap.add_argument("--kspace", type=int, default=50)
     ap.add_argument("--max-its", type=int, default=200)
@@ -701,6 +731,8 @@ def main(argv=None):
 
     if args.workload == "copycal":
         return copy_calibration(torch)
+    if args.workload == "cavity3d-ilu":
+        return ilu_config3(args, torch, capi)
     if args.workload == "sweep":
         if args.sweep_count == 8:
             args.sweep_count = 64

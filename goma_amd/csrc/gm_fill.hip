@@ -29,7 +29,11 @@
 //     R_k += int grad psi_k . tau (rho v_dot + rho v.grad v + grad P - f) dA,
 //     grad psi_0 = 0 and grad psi_k = B[.][k-1] for the P1 basis {1, xi_k}
 //   energy   (src/mm_fill_energy.c:280-700, heat_flux 1629):
-//     R_i = int[ -rho Cp phi_i (v.grad T) - k grad phi_i . grad T + phi_i h ] dA
+//     R_i = int[ -rho Cp w_i (v.grad T) - k grad phi_i . grad T + phi_i h ] dA
+//     w_i = phi_i, or with SUPG (src/mm_fill_energy.c:237-259, 332-339)
+//     w_i = phi_i + s/h (v.grad phi_i), h = sqrt(sum_p vc_p^2 / hsq_p) / 2
+//     from the squared face-centroid widths hsq and the velocity vc at the
+//     element's last local node (element_velocity, src/mm_fill_aux.c:798)
 #include <cmath>
 #include <vector>
 
@@ -71,6 +75,7 @@ struct FillArgs {
   int overwrite;                    // 1: first touch of a node pair stores (no zeroing needed)
   int num_nodes;
   double tau_pspg;  // global PSPG tau, 0: no pressure stabilisation
+  double supg_e;    // energy SUPG coefficient, 0: Galerkin weights
   int resid_only;  // 1: residual only (Assemble_Jacobian = FALSE)
   long long E;
   long long off_mp, off_cv, off_cp;
@@ -99,6 +104,56 @@ struct STab {
   static constexpr int REC = NVE + NVE * DIM + 1;
   static constexpr int SIZE = NSIDE * NSP * REC;
 };
+
+// Energy SUPG element parameters (src/mm_fill_energy.c:250-259, 644-651):
+// out[0] = s / h, out[1 + b] = s d(1/h)/d vc_b (the derivative in the
+// velocity component b at the last local node), all zero when h = 0.  x are
+// the element's nodal coordinates (local order), vc the velocity at the last
+// local node.  Face-centroid widths as h_elem_siz (src/mm_fill_aux.c:844).
+template <int DIM, typename XA>
+__device__ inline void supg_params(double s, const XA &x, const double *vc, double *out) {
+  double hsq[3];
+  if constexpr (DIM == 2) {
+    double w0 = 0.0, w1 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+      const double d0 = 0.5 * (x[a][1] + x[a][2]) - 0.5 * (x[a][0] + x[a][3]);
+      const double d1 = 0.5 * (x[a][0] + x[a][1]) - 0.5 * (x[a][2] + x[a][3]);
+      w0 += d0 * d0;
+      w1 += d1 * d1;
+    }
+    hsq[0] = w0;
+    hsq[1] = w1;
+  } else {
+    double w[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const double f1 = 0.25 * (x[a][0] + x[a][1] + x[a][2] + x[a][3]);
+      const double f2 = 0.25 * (x[a][1] + x[a][2] + x[a][5] + x[a][6]);
+      const double f3 = 0.25 * (x[a][2] + x[a][3] + x[a][6] + x[a][7]);
+      const double f4 = 0.25 * (x[a][0] + x[a][1] + x[a][4] + x[a][5]);
+      const double f5 = 0.25 * (x[a][0] + x[a][3] + x[a][4] + x[a][7]);
+      const double f6 = 0.25 * (x[a][4] + x[a][5] + x[a][6] + x[a][7]);
+      w[0] += (f2 - f5) * (f2 - f5);
+      w[1] += (f3 - f4) * (f3 - f4);
+      w[2] += (f1 - f6) * (f1 - f6);
+    }
+    hsq[0] = w[0];
+    hsq[1] = w[1];
+    hsq[2] = w[2];
+  }
+  double he = 0.0;
+#pragma unroll
+  for (int p = 0; p < DIM; p++) he += vc[p] * vc[p] / hsq[p];
+  he = sqrt(he) / 2.0;
+  const double hinv = he == 0.0 ? 0.0 : 1.0 / he;
+  out[0] = s * hinv;
+#pragma unroll
+  for (int b = 0; b < DIM; b++) {
+    const double dh = hsq[b] != 0.0 ? vc[b] * hinv / 4.0 / hsq[b] : 0.0;
+    out[1 + b] = he != 0.0 ? s * (-dh / he / he) : 0.0;
+  }
+}
 
 template <int DIM, int EN, int VISC, typename JT, int NT>
 __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__ elems) {
@@ -130,6 +185,7 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
   __shared__ double s_vd[NQ][DIM];       // v_dot at the quadrature points
   __shared__ double s_Td[NQ];            // T_dot
   __shared__ double s_ps[NQ][DIM];       // PSPG: tau dA (rho v_dot + rho v.grad v + grad P - f)
+  __shared__ double s_su[4];             // energy SUPG: s/h, s d(1/h)/d vc_b
   constexpr int NSP = STab<DIM>::NSP, SREC = STab<DIM>::REC;
   __shared__ double s_qw[GM_QS_MAX][NSP];  // QSIDE: weight * surface det * flux per point
   __shared__ int s_qsd[GM_QS_MAX];         // QSIDE: side index (0-based) of each entry
@@ -193,6 +249,13 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
     }
     if (!(det > 0.0)) atomicOr(&A.flag[0], 1);
     s_dA[q] = det * tb[T::WT + q];
+  } else if (tid == NQ) {
+    double vc[DIM], su[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int p = 0; p < DIM; p++) vc[p] = s_u[p][NVE - 1];
+    if (EN && A.supg_e != 0.0) supg_params<DIM>(A.supg_e, s_x, vc, su);
+#pragma unroll
+    for (int k = 0; k < 4; k++) s_su[k] = su[k];
   }
   __syncthreads();
 
@@ -364,15 +427,17 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
           R += s_dA[q] * (-rho * ph * adv - dif + ph * s_f[q][a] - rho * ph * s_vd[q][a]);
         }
       } else {
+        const double sw = s_su[0];
         for (int q = 0; q < NQ; q++) {
           const double ph = tb[T::PHI + q * NVE + i];
-          double adv = 0.0, dif = 0.0;
+          double adv = 0.0, dif = 0.0, vgi = 0.0;
 #pragma unroll
           for (int p = 0; p < DIM; p++) {
             adv += s_v[q][p] * s_gT[q][p];
             dif += s_g[q][i][p] * s_gT[q][p];
+            vgi += s_v[q][p] * s_g[q][i][p];
           }
-          R += s_dA[q] * (-rCp * ph * adv - kc * dif + ph * A.hsrc - rCp * ph * s_Td[q]);
+          R += s_dA[q] * (-rCp * (ph + sw * vgi) * adv - kc * dif + ph * A.hsrc - rCp * ph * s_Td[q]);
         }
         for (int k = 0; k < nqs; k++)
           for (int g = 0; g < NSP; g++) R += s_qw[k][g] * A.stab[(size_t)(s_qsd[k] * NSP + g) * SREC + i];
@@ -446,12 +511,24 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
         }
       }
       if constexpr (EN) {
+        // SUPG weight w_i = phi_i + sw (v.G_i) on the advection terms, and
+        // the weight's own derivative in v (d_wt_func)
+        const double sw = s_su[0];
+        double vgi = 0.0, vgT = 0.0;
+#pragma unroll
+        for (int p = 0; p < DIM; p++) {
+          vgi += s_v[q][p] * Gi[p];
+          vgT += s_v[q][p] * s_gT[q][p];
+        }
+        const double wi = phi_i + sw * vgi;
 #pragma unroll
         for (int a = 0; a < DIM; a++) {
           acc[a][DIM] += dA * (-rho * A.g[a] * A.beta * pij);
-          acc[DIM][a] += dA * (-rCp * pij * s_gT[q][a]);
+          double dw = sw * phi_j * Gi[a];
+          if (j == NVE - 1) dw += s_su[1 + a] * s_v[q][a] * Gi[a];
+          acc[DIM][a] += dA * (-rCp * (wi * phi_j * s_gT[q][a] + dw * vgT));
         }
-        acc[DIM][DIM] += dA * (-rCp * phi_i * advj - kc * GG - rCp * cm * pij);
+        acc[DIM][DIM] += dA * (-rCp * wi * advj - kc * GG - rCp * cm * pij);
       }
     }
     if (A.kel) {
@@ -637,6 +714,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VISC ? 3 : 
   __shared__ double s_gT[KP][DIM];
   __shared__ double s_P[KP], s_mu[KP], s_dmu[KP], s_div[KP], s_Td[KP];
   __shared__ double s_f[KP][DIM], s_vd[KP][DIM];
+  __shared__ double s_su[4];  // energy SUPG: s/h, s d(1/h)/d vc_b
   __shared__ double s_qw[GM_QS_MAX][NSP];
   __shared__ int s_qsd[GM_QS_MAX];
   const int qs0 = A.qs_ptr ? A.qs_ptr[e] : 0;
@@ -700,6 +778,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VISC ? 3 : 
     s_dA[q] = det * tb[T::WT + q];
   } else if (tid == NQ) {
     s_dA[NQ] = 0.0;  // the padding point carries no weight
+  } else if (tid == NQ + 1) {
+    double vc[DIM], su[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int p = 0; p < DIM; p++) vc[p] = s_u[p][NVE - 1];
+    if (EN && A.supg_e != 0.0) supg_params<DIM>(A.supg_e, s_x, vc, su);
+#pragma unroll
+    for (int k = 0; k < 4; k++) s_su[k] = su[k];
   } else if (EN && tid >= 64 && tid < 64 + nqs * NSP) {
     // QSIDE (src/bc_integ.c:1518-1519, 2156-2157): w |surface Jacobian| q at
     // one surface Gauss point of one side entry
@@ -872,16 +957,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VISC ? 3 : 
           R += s_dA[q] * (-rho * ph * adv - dif + ph * s_f[q][a] - rho * ph * s_vd[q][a]);
         }
       } else {
+        const double sw = s_su[0];
 #pragma unroll 1
         for (int q = 0; q < NQ; q++) {
           const double ph = s_Phi[q][i];
-          double adv = 0.0, dif = 0.0;
+          double adv = 0.0, dif = 0.0, vgi = 0.0;
 #pragma unroll
           for (int p = 0; p < DIM; p++) {
             adv += s_v[q][p] * s_gT[q][p];
             dif += s_G[p][q][i] * s_gT[q][p];
+            vgi += s_v[q][p] * s_G[p][q][i];
           }
-          R += s_dA[q] * (-rCp * ph * adv - kc * dif + ph * A.hsrc - rCp * ph * s_Td[q]);
+          R += s_dA[q] * (-rCp * (ph + sw * vgi) * adv - kc * dif + ph * A.hsrc - rCp * ph * s_Td[q]);
         }
         for (int k = 0; k < nqs; k++)
           for (int g = 0; g < NSP; g++) R += s_qw[k][g] * A.stab[(size_t)(s_qsd[k] * NSP + g) * SREC + i];
@@ -929,7 +1016,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VISC ? 3 : 
     // shared products, four independent chains per k-step: E(phi, dA, phi),
     // E(phi, dA, v.G), sum_p E(G_p, dA, G_p) and (Carreau) its mu-weighted
     // form, folded into the diagonal-block terms
-    d4v A1 = z, Dadv = z, L1 = z, Lmu = z;
+    d4v A1 = z, Dadv = z, L1 = z, Lmu = z, Dsu = z;
+    const bool supg = EN && s_su[0] != 0.0;
 #pragma unroll 1
     for (int q0 = 0; q0 < KP; q0 += 4) {
       const int q = q0 + kk;
@@ -940,6 +1028,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VISC ? 3 : 
       for (int p = 0; p < DIM; p++) vB += s_v[q][p] * s_G[p][q][jj];
       A1 = GM_MFMA64(phA, phB, A1);
       Dadv = GM_MFMA64(phA, vB, Dadv);
+      if (supg) {  // energy SUPG: E(v.G_i dA, v.G_j)
+        double vA = 0.0;
+#pragma unroll
+        for (int p = 0; p < DIM; p++) vA += s_v[q][p] * s_G[p][q][ii];
+        Dsu = GM_MFMA64(vA * dA, vB, Dsu);
+      }
 #pragma unroll
       for (int p = 0; p < DIM; p++) {
         const double gA = s_G[p][q][ii] * dA, gB = s_G[p][q][jj];
@@ -949,7 +1043,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VISC ? 3 : 
     }
     if constexpr (VISC == 0) Lmu = A.mu0 * L1;
     const d4v Dm = -(rho * Dadv + Lmu + (rho * cm) * A1);
-    const d4v Tt = -(rCp * Dadv + kc * L1 + (rCp * cm) * A1);
+    const d4v Tt = -(rCp * Dadv + kc * L1 + (rCp * cm) * A1 + (rCp * s_su[0]) * Dsu);
 
     // this lane's rows i = 16 (wave >> 1) + kk + 4 r, column j = jj: rank
     // words of the four node pairs
@@ -1029,12 +1123,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VISC ? 3 : 
       d4v Jr[NV];
 #pragma unroll
       for (int b = 0; b < DIM; b++) Jr[b] = z;
+      const double sw = s_su[0];
 #pragma unroll 1
       for (int q0 = 0; q0 < KP; q0 += 4) {
         const int q = q0 + kk;
-        const double phA = s_Phi[q][ii] * (-rCp * s_dA[q]), phB = s_Phi[q][jj];
+        const double w = -rCp * s_dA[q], phB = s_Phi[q][jj];
+        // (phi_i + sw v.G_i) d_b T + sw (v.grad T) G_b,i: the SUPG weight on
+        // the advection and its derivative in v at the column node
+        double vA = 0.0, vgT = 0.0;
 #pragma unroll
-        for (int b = 0; b < DIM; b++) Jr[b] = GM_MFMA64(phA * s_gT[q][b], phB, Jr[b]);
+        for (int p = 0; p < DIM; p++) {
+          vA += s_v[q][p] * s_G[p][q][ii];
+          vgT += s_v[q][p] * s_gT[q][p];
+        }
+        const double wi = s_Phi[q][ii] + sw * vA;
+#pragma unroll
+        for (int b = 0; b < DIM; b++)
+          Jr[b] = GM_MFMA64(w * (wi * s_gT[q][b] + sw * vgT * s_G[b][q][ii]), phB, Jr[b]);
+      }
+      if (supg && jj == NVE - 1) {
+        // d(1/h)/dv at the last local node: -rho Cp s_d,b sum_q dA (v.grad T) v_b G_b,i
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = 16 * (wave >> 1) + kk + 4 * r;
+          if (i >= NVE) continue;
+#pragma unroll
+          for (int b = 0; b < DIM; b++) {
+            double c = 0.0;
+            for (int q = 0; q < NQ; q++) {
+              double vgT = 0.0;
+#pragma unroll
+              for (int p = 0; p < DIM; p++) vgT += s_v[q][p] * s_gT[q][p];
+              c += s_dA[q] * vgT * s_v[q][b] * s_G[b][q][i];
+            }
+            Jr[b][r] += -rCp * s_su[1 + b] * c;
+          }
+        }
       }
       Jr[DIM] = Tt;
       scatter_row(DIM, Jr);
@@ -1543,6 +1667,7 @@ int gm_fill_launch_ex(gm_ctx *c, const double *x, double *resid, double *val, fl
   a.overwrite = c->fill_overwrite && c->d_rank ? 1 : 0;
   a.num_nodes = c->num_nodes;
   a.tau_pspg = c->phys.pspg ? gm_pspg_tau(c) : 0.0;
+  a.supg_e = c->energy ? c->phys.energy_supg : 0.0;
   a.E = c->lay.total;
   a.off_mp = c->lay.off_mp;
   a.off_cv = c->lay.off_cv;

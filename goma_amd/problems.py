@@ -46,6 +46,7 @@ class Physics:
     Tref: float = 0.0
     pspg: int = 0          # Pressure Stabilization: 0 no, 1 global
     ps_scaling: float = 0.0  # Pressure Stabilization Scaling
+    energy_supg: float = 0.0  # energy weight function SUPG coefficient, 0: Galerkin
 
 
 @dataclass

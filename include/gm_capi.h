@@ -81,6 +81,8 @@ typedef struct gm_physics {
   double Tref;         /* reference[TEMPERATURE] */
   int pspg;            /* Pressure Stabilization: 0 no, 1 global (Goma's PSPG = 1) */
   double ps_scaling;   /* Pressure Stabilization Scaling */
+  double energy_supg;  /* energy weight function SUPG coefficient (Ewt_funcModel = SUPG,
+                          Ewt_func; src/mm_fill_energy.c:237-259), 0: Galerkin */
 } gm_physics;
 
 typedef struct gm_bc {

@@ -523,6 +523,12 @@ static int elem_fill(const om_problem *p, int e, const double *x, const double *
   const int nq = (d == 2) ? 9 : 27;
   const double tau_pspg = ph->pspg ? om_pspg_tau(p) : 0.0;
   double xc[3][MDE], vel[3][MDE], Tn[MDE], Pn[4], veld[3][MDE], Tdn[MDE];
+  /* energy SUPG (src/mm_fill_energy.c:237-259): h_elem from the element's
+   * face-centroid widths and the element velocity of element_velocity for Q2
+   * (src/mm_fill_aux.c:798-806: the velocity at local dof dofs - 1, with
+   * dv_dnode = 1 there); supg_w = Ewt_func / h_elem, supg_d[b] its derivative
+   * in the velocity component b at that dof (src/mm_fill_energy.c:644-651) */
+  double supg_w = 0.0, supg_d[3] = {0.0, 0.0, 0.0};
   memset(L->ndof, 0, sizeof(L->ndof));
   for (int a = 0; a < d; a++) L->ndof[a] = nve;
   if (en) L->ndof[SLOT_T] = nve;
@@ -547,6 +553,19 @@ static int elem_fill(const om_problem *p, int e, const double *x, const double *
       Pn[k] = x[f + k];
       L->gun[SLOT_P][k] = f + k;
     }
+  }
+  if (en && ph->energy_supg != 0.0) {
+    double hsq[3], he = 0.0;
+    h_elem_siz(p, e, hsq);
+    for (int pp = 0; pp < d; pp++) he += vel[pp][nve - 1] * vel[pp][nve - 1] / hsq[pp];
+    he = sqrt(he) / 2.0;
+    const double hinv = he == 0.0 ? 0.0 : 1.0 / he;
+    supg_w = ph->energy_supg * hinv;
+    if (he != 0.0)
+      for (int b = 0; b < d; b++) {
+        const double dh = hsq[b] != 0.0 ? vel[b][nve - 1] * hinv / 4.0 / hsq[b] : 0.0;
+        supg_d[b] = ph->energy_supg * (-dh / he / he);
+      }
   }
   /* zero_lec (src/mm_fill.c:5641), restricted to the active blocks */
   for (int A = 0; A < NSLOT; A++) {
@@ -812,11 +831,17 @@ static int elem_fill(const om_problem *p, int e, const double *x, const double *
       const double k = ph->kcond, rCp = ph->rho * ph->Cp;
       double qf[3]; /* heat_flux, Fourier (src/mm_fill_energy.c:1652) */
       for (int pp = 0; pp < 3; pp++) qf[pp] = -k * grad_T[pp];
+      double vgT = 0.0; /* v . grad T */
+      for (int pp = 0; pp < d; pp++) vgT += v[pp] * grad_T[pp];
       for (int i = 0; i < nve; i++) {
         double phi_i = phi[i];
-        double advection = 0.0;
-        for (int pp = 0; pp < d; pp++) advection += v[pp] * grad_T[pp];
-        advection *= -phi_i * rCp * detJ * wt;
+        /* weight of the advection term: phi_i + supg / h_elem (v . grad phi_i)
+         * (src/mm_fill_energy.c:332-339, 403-411) */
+        double vgi = 0.0;
+        for (int pp = 0; pp < d; pp++) vgi += v[pp] * gphi[i][pp];
+        const double wt_func = phi_i + supg_w * vgi;
+        double advection = vgT;
+        advection *= -wt_func * rCp * detJ * wt;
         double diffusion = 0.0;
         for (int pp = 0; pp < d; pp++) diffusion += gphi[i][pp] * qf[pp];
         diffusion *= detJ * wt;
@@ -833,16 +858,24 @@ static int elem_fill(const om_problem *p, int e, const double *x, const double *
         for (int j = 0; j < nve; j++) {
           double adv = 0.0, dif = 0.0;
           for (int pp = 0; pp < d; pp++) adv += rCp * v[pp] * gphi[j][pp];
-          adv *= -phi_i * detJ * wt;
+          adv *= -wt_func * detJ * wt;
           for (int pp = 0; pp < d; pp++) dif += (-k * gphi[j][pp]) * gphi[i][pp];
           dif *= detJ * wt;
           L->J[SLOT_T][SLOT_T][i][j] += adv + dif;
         }
-        /* J_e_v (src/mm_fill_energy.c:620-690): d_vconv->v[b][b][j] = phi_j */
+        /* J_e_v (src/mm_fill_energy.c:620-690): d_vconv->v[b][b][j] = phi_j;
+         * with SUPG the weight's own derivative: d_wt_func = supg/h phi_j
+         * d_b phi_i + supg d(1/h)/dv_b,j v_b d_b phi_i (the second only at
+         * the element-velocity dof) times the advection rho Cp v . grad T */
         for (int b = 0; b < d; b++)
           for (int j = 0; j < nve; j++) {
-            double adv = phi_i * rCp * phi[j] * grad_T[b];
+            double adv = wt_func * rCp * phi[j] * grad_T[b];
             adv *= -detJ * wt;
+            if (supg_w != 0.0 || supg_d[b] != 0.0) {
+              double dwt = supg_w * phi[j] * gphi[i][b];
+              if (j == nve - 1) dwt += supg_d[b] * v[b] * gphi[i][b];
+              adv += -dwt * rCp * vgT * detJ * wt;
+            }
             L->J[SLOT_T][b][i][j] += adv;
           }
       }

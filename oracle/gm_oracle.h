@@ -70,6 +70,7 @@ typedef struct om_physics {
   double Tref;
   int pspg;           /* Pressure Stabilization: 0 no, 1 global (PSPG = 1) */
   double ps_scaling;  /* Pressure Stabilization Scaling */
+  double energy_supg; /* energy weight function SUPG coefficient (Ewt_func), 0: Galerkin */
 } om_physics;
 
 typedef struct om_bc {

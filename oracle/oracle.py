@@ -30,7 +30,7 @@ class OmPhysics(C.Structure):
                 ("nexp", C.c_double), ("aexp", C.c_double), ("Cp", C.c_double),
                 ("kcond", C.c_double), ("hsrc", C.c_double), ("g", C.c_double * 3),
                 ("beta", C.c_double), ("Tref", C.c_double), ("pspg", C.c_int),
-                ("ps_scaling", C.c_double)]
+                ("ps_scaling", C.c_double), ("energy_supg", C.c_double)]
 
 
 class OmBC(C.Structure):
@@ -141,7 +141,8 @@ class Oracle:
         ph = prob.phys
         op = OmPhysics(ph.energy, ph.visc_model, ph.rho, ph.mu0, ph.muinf, ph.lam, ph.nexp, ph.aexp,
                        ph.Cp, ph.kcond, ph.hsrc, (C.c_double * 3)(*ph.g), ph.beta, ph.Tref,
-                       int(getattr(ph, "pspg", 0)), float(getattr(ph, "ps_scaling", 0.0)))
+                       int(getattr(ph, "pspg", 0)), float(getattr(ph, "ps_scaling", 0.0)),
+                       float(getattr(ph, "energy_supg", 0.0)))
         ob = OmBC(len(self._bn), self._bn.ctypes.data, self._bv.ctypes.data, self._bx.ctypes.data,
                   prob.datum_elem, prob.datum_value)
         self.h = L.om_create(C.byref(om), C.byref(op), C.byref(ob))

@@ -961,14 +961,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VISC ? 3 : 
 #pragma unroll 1
         for (int q = 0; q < NQ; q++) {
           const double ph = s_Phi[q][i];
-          double adv = 0.0, dif = 0.0, vgi = 0.0;
+          double adv = 0.0, dif = 0.0;
 #pragma unroll
           for (int p = 0; p < DIM; p++) {
             adv += s_v[q][p] * s_gT[q][p];
             dif += s_G[p][q][i] * s_gT[q][p];
-            vgi += s_v[q][p] * s_G[p][q][i];
           }
-          R += s_dA[q] * (-rCp * (ph + sw * vgi) * adv - kc * dif + ph * A.hsrc - rCp * ph * s_Td[q]);
+          double wi = ph;
+          if (sw != 0.0) {
+            double vgi = 0.0;
+#pragma unroll
+            for (int p = 0; p < DIM; p++) vgi += s_v[q][p] * s_G[p][q][i];
+            wi += sw * vgi;
+          }
+          R += s_dA[q] * (-rCp * wi * adv - kc * dif + ph * A.hsrc - rCp * ph * s_Td[q]);
         }
         for (int k = 0; k < nqs; k++)
           for (int g = 0; g < NSP; g++) R += s_qw[k][g] * A.stab[(size_t)(s_qsd[k] * NSP + g) * SREC + i];
@@ -1016,7 +1022,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VISC ? 3 : 
     // shared products, four independent chains per k-step: E(phi, dA, phi),
     // E(phi, dA, v.G), sum_p E(G_p, dA, G_p) and (Carreau) its mu-weighted
     // form, folded into the diagonal-block terms
-    d4v A1 = z, Dadv = z, L1 = z, Lmu = z, Dsu = z;
+    d4v A1 = z, Dadv = z, L1 = z, Lmu = z;
     const bool supg = EN && s_su[0] != 0.0;
 #pragma unroll 1
     for (int q0 = 0; q0 < KP; q0 += 4) {
@@ -1028,12 +1034,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VISC ? 3 : 
       for (int p = 0; p < DIM; p++) vB += s_v[q][p] * s_G[p][q][jj];
       A1 = GM_MFMA64(phA, phB, A1);
       Dadv = GM_MFMA64(phA, vB, Dadv);
-      if (supg) {  // energy SUPG: E(v.G_i dA, v.G_j)
-        double vA = 0.0;
-#pragma unroll
-        for (int p = 0; p < DIM; p++) vA += s_v[q][p] * s_G[p][q][ii];
-        Dsu = GM_MFMA64(vA * dA, vB, Dsu);
-      }
 #pragma unroll
       for (int p = 0; p < DIM; p++) {
         const double gA = s_G[p][q][ii] * dA, gB = s_G[p][q][jj];
@@ -1043,7 +1043,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VISC ? 3 : 
     }
     if constexpr (VISC == 0) Lmu = A.mu0 * L1;
     const d4v Dm = -(rho * Dadv + Lmu + (rho * cm) * A1);
-    const d4v Tt = -(rCp * Dadv + kc * L1 + (rCp * cm) * A1 + (rCp * s_su[0]) * Dsu);
+    const d4v Tt = -(rCp * Dadv + kc * L1 + (rCp * cm) * A1);
 
     // this lane's rows i = 16 (wave >> 1) + kk + 4 r, column j = jj: rank
     // words of the four node pairs
@@ -1124,46 +1124,65 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VISC ? 3 : 
 #pragma unroll
       for (int b = 0; b < DIM; b++) Jr[b] = z;
       const double sw = s_su[0];
-#pragma unroll 1
-      for (int q0 = 0; q0 < KP; q0 += 4) {
-        const int q = q0 + kk;
-        const double w = -rCp * s_dA[q], phB = s_Phi[q][jj];
-        // (phi_i + sw v.G_i) d_b T + sw (v.grad T) G_b,i: the SUPG weight on
-        // the advection and its derivative in v at the column node
-        double vA = 0.0, vgT = 0.0;
-#pragma unroll
-        for (int p = 0; p < DIM; p++) {
-          vA += s_v[q][p] * s_G[p][q][ii];
-          vgT += s_v[q][p] * s_gT[q][p];
-        }
-        const double wi = s_Phi[q][ii] + sw * vA;
-#pragma unroll
-        for (int b = 0; b < DIM; b++)
-          Jr[b] = GM_MFMA64(w * (wi * s_gT[q][b] + sw * vgT * s_G[b][q][ii]), phB, Jr[b]);
-      }
-      if (supg && jj == NVE - 1) {
-        // d(1/h)/dv at the last local node: -rho Cp s_d,b sum_q dA (v.grad T) v_b G_b,i
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int i = 16 * (wave >> 1) + kk + 4 * r;
-          if (i >= NVE) continue;
-#pragma unroll
-          for (int b = 0; b < DIM; b++) {
-            double c = 0.0;
-            for (int q = 0; q < NQ; q++) {
-              double vgT = 0.0;
-#pragma unroll
-              for (int p = 0; p < DIM; p++) vgT += s_v[q][p] * s_gT[q][p];
-              c += s_dA[q] * vgT * s_v[q][b] * s_G[b][q][i];
-            }
-            Jr[b][r] += -rCp * s_su[1 + b] * c;
-          }
-        }
-      }
       Jr[DIM] = Tt;
+      if (supg) {
+#pragma unroll 1
+        for (int q0 = 0; q0 < KP; q0 += 4) {
+          const int q = q0 + kk;
+          const double w = -rCp * s_dA[q], phB = s_Phi[q][jj];
+          // (phi_i + sw v.G_i) d_b T + sw (v.grad T) G_b,i: the SUPG weight on
+          // the advection and its derivative in v at the column node; and
+          // the weight's part of J_TT, -rho Cp sw E(v.G_i dA, v.G_j)
+          double vA = 0.0, vgT = 0.0, vB = 0.0;
+#pragma unroll
+          for (int p = 0; p < DIM; p++) {
+            vA += s_v[q][p] * s_G[p][q][ii];
+            vgT += s_v[q][p] * s_gT[q][p];
+            vB += s_v[q][p] * s_G[p][q][jj];
+          }
+          const double wi = s_Phi[q][ii] + sw * vA;
+#pragma unroll
+          for (int b = 0; b < DIM; b++)
+            Jr[b] = GM_MFMA64(w * (wi * s_gT[q][b] + sw * vgT * s_G[b][q][ii]), phB, Jr[b]);
+          Jr[DIM] = GM_MFMA64(w * sw * vA, vB, Jr[DIM]);
+        }
+      } else {
+#pragma unroll 1
+        for (int q0 = 0; q0 < KP; q0 += 4) {
+          const int q = q0 + kk;
+          const double phA = s_Phi[q][ii] * (-rCp * s_dA[q]), phB = s_Phi[q][jj];
+#pragma unroll
+          for (int b = 0; b < DIM; b++) Jr[b] = GM_MFMA64(phA * s_gT[q][b], phB, Jr[b]);
+        }
+      }
       scatter_row(DIM, Jr);
     }
 #undef GM_MFMA64
+  }
+
+  // energy SUPG: d(1/h)/dv at the last local node adds
+  // -rho Cp s_d,b sum_q dA (v.grad T) v_b G_b,i to the energy rows' columns of
+  // that node's velocities, on top of the entries scattered above
+  if (EN && s_su[0] != 0.0) {
+    __syncthreads();
+    for (int t = tid; t < DIM * NVE; t += NT) {
+      const int b = t / NVE, i = t - b * NVE;
+      double c = 0.0;
+#pragma unroll 1
+      for (int q = 0; q < NQ; q++) {
+        double vgT = 0.0;
+#pragma unroll
+        for (int p = 0; p < DIM; p++) vgT += s_v[q][p] * s_gT[q][p];
+        c += s_dA[q] * vgT * s_v[q][b] * s_G[b][q][i];
+      }
+      const double v = -rCp * s_su[1 + b] * c;
+      const int ij = i * NVE + (NVE - 1);
+      if (A.kel) A.kel[(size_t)e * A.E + (size_t)(DIM * NV + b) * NVE2 + ij] += v;
+      if (s_dir[DIM][i]) continue;  // a Dirichlet row keeps its unit diagonal only
+      const int row = s_gun[DIM][i], col = s_gun[b][NVE - 1];
+      const long long pos = s_brow[DIM][i] + (int)rk[NVE2 + ij] + b - (col > row ? 1 : 0);
+      out[pos] += (JT)v;
+    }
   }
 
   // 5c. momentum x pressure (J_m_P): int grad_a phi_i psi_k dA

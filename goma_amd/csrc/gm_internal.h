@@ -92,7 +92,13 @@ struct MgLevel {
   // inside each coarse element, and per finer element the nodal weights
   // W[i][m] (finer local node i <- coarse local node m) and P1 weights Q[k][l]
   int *d_child = nullptr;     // [coarse elems][2^dim]
-  double *d_W = nullptr;      // [finer elems][nve * nve]
+  // nodal weight matrix W of each finer element by columns (coarse local
+  // node m: the finer local nodes interpolating from it, ascending, and
+  // their weights): pointers [finer elems][nve + 1], rows / weights
+  // [finer elems][W_stride]
+  int *d_Wp = nullptr, *d_Wi = nullptr;
+  double *d_Wv = nullptr;
+  int W_stride = 0;
   double *d_Q = nullptr;      // [finer elems][np * np]
   bool gal_ok = false;
   std::string gal_err;

@@ -790,7 +790,32 @@ static std::string build_galerkin_maps(const gm_ctx *f, const gm_ctx *c, const i
   }
   for (int e = 0; e < c->num_elems; e++)
     if (nchild[e] != nch) return "a coarse element without 2^dim children";
-  if (upv(&L.d_child, child.data(), child.size()) || upv(&L.d_W, W.data(), W.size()) ||
+  // W by columns, nonzeros only (a Q2 child of a Q2 parent: 125 of 729 in 3D)
+  int stride = 1;
+  for (int e = 0; e < f->num_elems; e++) {
+    int nz = 0;
+    for (int t = 0; t < nve * nve; t++) nz += W[(size_t)e * nve * nve + t] != 0.0;
+    stride = std::max(stride, nz);
+  }
+  std::vector<int> Wp((size_t)f->num_elems * (nve + 1)), Wi((size_t)f->num_elems * stride, 0);
+  std::vector<double> Wv((size_t)f->num_elems * stride, 0.0);
+  for (int e = 0; e < f->num_elems; e++) {
+    const double *We = &W[(size_t)e * nve * nve];
+    int k = 0;
+    for (int m = 0; m < nve; m++) {
+      Wp[(size_t)e * (nve + 1) + m] = k;
+      for (int i = 0; i < nve; i++)
+        if (We[i * nve + m] != 0.0) {
+          Wi[(size_t)e * stride + k] = i;
+          Wv[(size_t)e * stride + k] = We[i * nve + m];
+          k++;
+        }
+    }
+    Wp[(size_t)e * (nve + 1) + nve] = k;
+  }
+  L.W_stride = stride;
+  if (upv(&L.d_child, child.data(), child.size()) || upv(&L.d_Wp, Wp.data(), Wp.size()) ||
+      upv(&L.d_Wi, Wi.data(), Wi.size()) || upv(&L.d_Wv, Wv.data(), Wv.size()) ||
       upv(&L.d_Q, Q.data(), Q.size()))
     return "device allocation failed";
   return "";
@@ -802,7 +827,9 @@ static std::string build_galerkin_maps(const gm_ctx *f, const gm_ctx *c, const i
 template <int DIM, int EN, typename CT>
 __global__ __launch_bounds__(256) void k_galerkin(const double *__restrict__ kf,
                                                   const int *__restrict__ child,
-                                                  const double *__restrict__ W,
+                                                  const int *__restrict__ Wp,
+                                                  const int *__restrict__ Wi,
+                                                  const double *__restrict__ Wv, int ws,
                                                   const double *__restrict__ Q, long long E,
                                                   long long off_mp, double *__restrict__ kc) {
   constexpr int NVE = DIM == 2 ? 9 : 27, NP = DIM + 1, NV = DIM + EN, NCH = 1 << DIM, NT = 256;
@@ -811,10 +838,21 @@ __global__ __launch_bounds__(256) void k_galerkin(const double *__restrict__ kf,
   constexpr int NMP = DIM * NVE * NP;          // momentum x P (= continuity x velocity) entries
   constexpr int NPC = 2 * NMP + NP * NP;       // pressure couplings, contiguous from off_mp
   constexpr int RP = (NPC + NT - 1) / NT;
-  __shared__ CT sW[NVE2], sK[NVE2], sT[NVE2], sQ[NP * NP], sKp[NPC];
+  // W by columns: s_wp[m] .. s_wp[m + 1] index the finer nodes s_wi (ascending)
+  // interpolating from coarse node m and their weights s_wv; only these terms
+  // of the dense sums are nonzero, taken in the same order
+  __shared__ CT sK[NVE2], sT[NVE2], sQ[NP * NP], sKp[NPC], s_wv[NVE2];
+  __shared__ int s_wp[NVE + 1], s_wi[NVE2];
   const int ec = blockIdx.x, tid = threadIdx.x;
   const int *ch = child + (size_t)ec * NCH;
   double *__restrict__ out = kc + (size_t)ec * E;
+  auto load_w = [&](int f) {
+    if (tid <= NVE) s_wp[tid] = Wp[(size_t)f * (NVE + 1) + tid];
+    for (int t = tid; t < ws; t += NT) {
+      s_wi[t] = Wi[(size_t)f * ws + t];
+      s_wv[t] = (CT)Wv[(size_t)f * ws + t];
+    }
+  };
   for (int blk = 0; blk < NV * NV; blk++) {
     CT acc[RN];
 #pragma unroll
@@ -822,17 +860,14 @@ __global__ __launch_bounds__(256) void k_galerkin(const double *__restrict__ kf,
     for (int sidx = 0; sidx < NCH; sidx++) {
       const int f = ch[sidx];
       const double *Kf = kf + (size_t)f * E + (size_t)blk * NVE2;
-      const double *Wf = W + (size_t)f * NVE2;
-      for (int t = tid; t < NVE2; t += NT) {
-        sK[t] = (CT)Kf[t];
-        sW[t] = (CT)Wf[t];
-      }
+      for (int t = tid; t < NVE2; t += NT) sK[t] = (CT)Kf[t];
+      load_w(f);
       __syncthreads();
-      for (int t = tid; t < NVE2; t += NT) {  // T = K W
-        const int i = t / NVE, n = t - i * NVE;
+      for (int t = tid; t < NVE2; t += NT) {  // T = K W, column m of W
+        const int m = t / NVE, i = t - m * NVE;
         CT v = 0;
-        for (int j = 0; j < NVE; j++) v += sK[i * NVE + j] * sW[j * NVE + n];
-        sT[t] = v;
+        for (int k = s_wp[m]; k < s_wp[m + 1]; k++) v += sK[i * NVE + s_wi[k]] * s_wv[k];
+        sT[i * NVE + m] = v;
       }
       __syncthreads();
 #pragma unroll
@@ -841,7 +876,7 @@ __global__ __launch_bounds__(256) void k_galerkin(const double *__restrict__ kf,
         if (t < NVE2) {
           const int m = t / NVE, n = t - m * NVE;
           CT v = 0;
-          for (int i = 0; i < NVE; i++) v += sW[i * NVE + m] * sT[i * NVE + n];
+          for (int k = s_wp[m]; k < s_wp[m + 1]; k++) v += s_wv[k] * sT[s_wi[k] * NVE + n];
           acc[r] += v;
         }
       }
@@ -859,7 +894,7 @@ __global__ __launch_bounds__(256) void k_galerkin(const double *__restrict__ kf,
   for (int sidx = 0; sidx < NCH; sidx++) {
     const int f = ch[sidx];
     const double *Kf = kf + (size_t)f * E + off_mp;
-    for (int t = tid; t < NVE2; t += NT) sW[t] = (CT)W[(size_t)f * NVE2 + t];
+    load_w(f);
     if (tid < NP * NP) sQ[tid] = (CT)Q[(size_t)f * NP * NP + tid];
     for (int t = tid; t < NPC; t += NT) sKp[t] = (CT)Kf[t];
     __syncthreads();
@@ -870,11 +905,12 @@ __global__ __launch_bounds__(256) void k_galerkin(const double *__restrict__ kf,
       CT v = 0;
       if (t < NMP) {  // row (a, m) momentum, column l pressure
         const int a = t / (NVE * NP), rem = t - a * NVE * NP, m = rem / NP, l = rem - m * NP;
-        for (int i = 0; i < NVE; i++) {
+        for (int k2 = s_wp[m]; k2 < s_wp[m + 1]; k2++) {
+          const int i = s_wi[k2];
           CT q = 0;
 #pragma unroll
           for (int k = 0; k < NP; k++) q += sKp[a * NVE * NP + i * NP + k] * sQ[k * NP + l];
-          v += sW[i * NVE + m] * q;
+          v += s_wv[k2] * q;
         }
       } else if (t < 2 * NMP) {  // row l continuity, column (b, n) velocity
         const int t2 = t - NMP, b = t2 / (NP * NVE), rem = t2 - b * NP * NVE;
@@ -882,7 +918,8 @@ __global__ __launch_bounds__(256) void k_galerkin(const double *__restrict__ kf,
 #pragma unroll
         for (int k = 0; k < NP; k++) {
           CT q = 0;
-          for (int j = 0; j < NVE; j++) q += sKp[NMP + b * NP * NVE + k * NVE + j] * sW[j * NVE + n];
+          for (int k2 = s_wp[n]; k2 < s_wp[n + 1]; k2++)
+            q += sKp[NMP + b * NP * NVE + k * NVE + s_wi[k2]] * s_wv[k2];
           v += sQ[k * NP + l] * q;
         }
       } else {  // P x P
@@ -962,14 +999,16 @@ static void galerkin_level(gm_ctx *prof, const gm_ctx *above, MgLevel &L, JT *va
   gm_ctx *lc = L.ctx;
   const ElemLayout &ly = lc->lay;
   const int nch = 1 << lc->dim;
-  const double gbytes = ((double)above->num_elems * (ly.total + lc->nve * lc->nve + lc->np * lc->np) +
-                         (double)lc->num_elems * ly.total) * 8.0 + 4.0 * nch * lc->num_elems;
+  const double gbytes = ((double)above->num_elems * (ly.total + lc->np * lc->np) +
+                         (double)lc->num_elems * ly.total) * 8.0 + 4.0 * nch * lc->num_elems +
+                        (double)above->num_elems * (L.W_stride * 12.0 + (lc->nve + 1) * 4.0);
   // products in the precision of the coarse matrices (FP32 under mixed precision)
   using CT = JT;
   auto kern = lc->dim == 2 ? (lc->energy ? k_galerkin<2, 1, CT> : k_galerkin<2, 0, CT>)
                            : (lc->energy ? k_galerkin<3, 1, CT> : k_galerkin<3, 0, CT>);
   gm_launch(prof, GM_K_GALERKIN, gbytes, kern, dim3(lc->num_elems), dim3(256), 0, lc->stream,
-            above->d_kel, L.d_child, L.d_W, L.d_Q, ly.total, ly.off_mp, lc->d_kel);
+            above->d_kel, L.d_child, L.d_Wp, L.d_Wi, L.d_Wv, L.W_stride, L.d_Q, ly.total, ly.off_mp,
+            lc->d_kel);
   for (int col = 0; col < lc->n_ecol; col++) {
     const int cnt = lc->ecol_ptr[col + 1] - lc->ecol_ptr[col];
     if (!cnt) continue;
@@ -1406,19 +1445,23 @@ static void vanka_additive(gm_ctx *lc, hipStream_t s, const AT *ainv, const BT *
 // Multiplicative: coloured block Gauss-Seidel, the residual recomputed on each
 // colour's rows.  Additive: one residual per sweep, every element's correction
 // from it (one launch), then each unknown sums its corrections weighted by
-// 1 / elements sharing it (a second launch; no colours needed).
+// 1 / elements sharing it (a second launch; no colours needed).  x_zero: x is
+// zero on entry (every level's pre-smoothing in the V-cycle), so the first
+// additive sweep's residual is b itself.
 template <typename VT, typename BT>
 static void smooth(gm_ctx *lc, hipStream_t s, const VT *a, const VT *ainv, const BT *b, BT *x, BT *r,
-                   int sweeps, double omega, int additive) {
+                   int sweeps, double omega, int additive, bool x_zero) {
   for (int it = 0; it < sweeps; it++) {
     if (additive) {
-      resid<VT, BT>(lc, s, a, x, b, r);
+      const BT *rr = r;
+      if (it == 0 && x_zero) rr = b;
+      else resid<VT, BT>(lc, s, a, x, b, r);
       if (lc->dim == 2) {
-        if (lc->energy) vanka_additive<2, 1, VT, BT>(lc, s, ainv, r, x, omega);
-        else vanka_additive<2, 0, VT, BT>(lc, s, ainv, r, x, omega);
+        if (lc->energy) vanka_additive<2, 1, VT, BT>(lc, s, ainv, rr, x, omega);
+        else vanka_additive<2, 0, VT, BT>(lc, s, ainv, rr, x, omega);
       } else {
-        if (lc->energy) vanka_additive<3, 1, VT, BT>(lc, s, ainv, r, x, omega);
-        else vanka_additive<3, 0, VT, BT>(lc, s, ainv, r, x, omega);
+        if (lc->energy) vanka_additive<3, 1, VT, BT>(lc, s, ainv, rr, x, omega);
+        else vanka_additive<3, 0, VT, BT>(lc, s, ainv, rr, x, omega);
       }
       continue;
     }
@@ -1481,7 +1524,7 @@ static void vcycle(gm_ctx *c, const VT *a0, const BT *v, BT *z, const gm_solver_
     }
     k_set<BT><<<1024, 256, 0, s>>>(x, lc->N, (BT)0);
     smooth<VT, BT>(lc, s, amat(l), (const VT *)L.d_ainv, b, x, (BT *)L.d_r,
-                   l == nl ? (cs > 0 ? cs : 20) : nu, om, o->mg_smoother == 1);
+                   l == nl ? (cs > 0 ? cs : 20) : nu, om, o->mg_smoother == 1, true);
     if (l == nl) break;
     resid<VT, BT>(lc, s, amat(l), x, b, (BT *)L.d_r);
     MgLevel &C = c->mg[l];
@@ -1498,7 +1541,7 @@ static void vcycle(gm_ctx *c, const VT *a0, const BT *v, BT *z, const gm_solver_
     transfer<BT>(c, s, lc->N, C.d_Prow, C.d_Pcol, C.d_Pval, C.nnz, C.ctx->N, (const BT *)C.d_x,
                  nullptr, lc->d_dbc, x, 1);
     smooth<VT, BT>(lc, s, amat(l), (const VT *)L.d_ainv, b, x, (BT *)L.d_r, nu, om,
-                   o->mg_smoother == 1);
+                   o->mg_smoother == 1, false);
   }
 }
 
@@ -1519,7 +1562,7 @@ void gm_mg_free(gm_ctx *c) {
     void *p[] = {L.d_Prow, L.d_Pcol, L.d_Pval, L.d_Rrow, L.d_Rcol, L.d_Rval, L.d_inject,
                  L.d_state, L.d_rdummy, L.d_x, L.d_b, L.d_r, L.d_val_p[0], L.d_val_p[1],
                  L.d_ainv_p[0], L.d_ainv_p[1],
-                 L.d_dense, L.d_pcol, L.d_perm, L.d_child, L.d_W, L.d_Q};
+                 L.d_dense, L.d_pcol, L.d_perm, L.d_child, L.d_Wp, L.d_Wi, L.d_Wv, L.d_Q};
     for (void *q : p)
       if (q) (void)hipFree(q);
   };

@@ -402,6 +402,21 @@ def headline(args, torch, capi):
     asm_roof = kernel_roofline("k_fill", k_fill, prob, 0, prob.name) if k_fill else None
     if asm_roof:
         asm_roof["per_assembly_s"] = round(asm_s, 6)
+        if prob.mesh.dim == 3:
+            # HEX27 node-pair blocks on the FP64 matrix cores (k_fill3m): per
+            # element and wave, 7 k-steps of 5 shared products (3 more with
+            # Carreau), 6 per momentum block row and 3 for the energy row;
+            # four waves; 2 * 16 * 16 * 4 flop per v_mfma_f64_16x16x4f64
+            en, vm = int(prob.phys.energy), int(prob.phys.visc_model)
+            per_wave = 7 * ((5 + 3 * vm) + 3 * (6 + 3 * vm) + 3 * en)
+            ncol = int(ctx.elem_colours().max()) + 1
+            flops = 4 * per_wave * 2048.0 * prob.mesh.num_elems / ncol
+            tf = flops / (k_fill["avg_ms"] * 1e-3) / 1e12
+            asm_roof["mfma"] = dict(bound="mfma", achieved=round(tf, 2), peak=FP64_PEAK_TFS,
+                                    unit="TFLOP/s", frac=round(tf / FP64_PEAK_TFS, 4),
+                                    flops_per_launch=round(flops),
+                                    note="padded 32 x 32 x 28 products; the map, field, "
+                                         "residual and scatter phases run beside them")
     out = {
         "metric": METRIC,
         "value": round(t_plain, 4),

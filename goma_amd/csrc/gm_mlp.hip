@@ -322,6 +322,20 @@ struct GuessMap {
   int num_nodes, num_elems, dim, energy, n_out, spe;  // spe: sample points per element
   double scale[GM_MLP_MAX_OUT], shift[GM_MLP_MAX_OUT];
 };
+// network input of a guess: the sample point's coordinates, then the query's
+// parameters
+struct GuessParams {
+  float p[GM_MLP_MAX_IN];
+};
+__global__ void k_mlp_guess_input(long long np, int dim, int n_in, const float *__restrict__ pts,
+                                  GuessParams q, float *__restrict__ in) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= np * n_in) return;
+  const long long p = t / n_in;
+  const int i = (int)(t - p * n_in);
+  in[t] = i < dim ? pts[p * dim + i] : q.p[i - dim];
+}
+
 __global__ void k_mlp_guess_nodes(GuessMap g, const int *__restrict__ fu, const float *__restrict__ y,
                                   double *__restrict__ x) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -468,6 +482,7 @@ struct gm_mlp {
   long long gx_n = 0;
   const gm_ctx *gctx = nullptr;       // context whose sample points are cached
   std::vector<float> gpts;            // [npts][dim] coordinates of nodes + element points
+  float *d_gpts = nullptr;            // the same on the device
   // snapshot subspace of the delta form
   const gm_ctx *bctx = nullptr;
   int nbasis = 0;
@@ -633,7 +648,7 @@ static void free_mlp(gm_mlp *m) {
   for (bf16_t *p : m->WbT) ps.push_back(p);
   for (bf16_t *p : m->act) ps.push_back(p);
   void *more[] = {m->y,   m->dz[0], m->dz[1], m->part, m->partb, m->bt,  m->bm,  m->lpart,
-                  m->lossbuf, m->din, m->dt,  m->dm,   m->dperm, m->gin, m->gout, m->gx,
+                  m->lossbuf, m->din, m->dt,  m->dm,   m->dperm, m->gin, m->gout, m->gx, m->d_gpts,
                   m->Q,   m->wgt,  m->qpart, m->qc,  m->qv};
   for (void *p : more) ps.push_back(p);
   for (void *p : ps) dfree(p);
@@ -1084,12 +1099,12 @@ extern "C" int gm_mlp_set_basis(gm_mlp *m, gm_ctx *c, int nsnap, const double *s
 static int guess_one(gm_mlp *m, gm_ctx *c, const gm_mlp_query *q, double *x) {
   const int dim = c->dim, spe = 1 + 2 * dim, n_in = m->s.n_in;
   const long long np = (long long)c->num_nodes + (long long)c->num_elems * spe;
-  std::vector<float> in((size_t)np * n_in);
-  for (long long p = 0; p < np; p++) {
-    for (int d = 0; d < dim; d++) in[(size_t)p * n_in + d] = m->gpts[(size_t)p * dim + d];
-    for (int i = dim; i < n_in; i++) in[(size_t)p * n_in + i] = (float)q->param[i - dim];
-  }
-  GM_HIP(hipMemcpy(m->gin, in.data(), in.size() * sizeof(float), hipMemcpyHostToDevice));
+  GuessParams gp{};
+  for (int i = dim; i < n_in; i++) gp.p[i - dim] = (float)q->param[i - dim];
+  const long long nt = np * n_in;
+  k_mlp_guess_input<<<(unsigned)((nt + 255) / 256), 256, 0, m->stream>>>(np, dim, n_in, m->d_gpts, gp,
+                                                                         m->gin);
+  GM_HIP(hipGetLastError());
   int rc = gm_mlp_predict_dev(m, np, m->gin, m->gout);
   if (rc) return rc;
   GuessMap g{};
@@ -1123,6 +1138,10 @@ extern "C" int gm_mlp_guess_dev(gm_mlp *m, gm_ctx *c, const gm_mlp_query *q, con
   int rc;
   if (m->gctx != c) {
     sample_points(c, m->gpts);
+    dfree(m->d_gpts);
+    m->d_gpts = nullptr;
+    if ((rc = dalloc(&m->d_gpts, m->gpts.size()))) return rc;
+    GM_HIP(hipMemcpy(m->d_gpts, m->gpts.data(), m->gpts.size() * sizeof(float), hipMemcpyHostToDevice));
     m->gctx = c;
   }
   if (np > m->gcap) {

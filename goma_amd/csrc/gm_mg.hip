@@ -163,22 +163,29 @@ __global__ __launch_bounds__(NT) void k_vanka_invert(LevelDev L, const VT *__res
 }
 
 // The same inverse with the block held in registers: thread (tr, tc) of a
-// TR x TC grid owns rows tr + TR a and columns tc + TC b.  Per pivot step only
-// the pivot column, the pivot row and the old row k pass through LDS (RA + 2 CB
-// reads per thread instead of three per entry); register arrays are indexed
-// at compile time only, the owner row / column selected by predicates.  The
-// row interchanges are undone by writing column c of the result to position
-// pos[c] of the output.
+// TR x TC grid owns rows tr + TR a and columns tc + TC b.  Pivoting is
+// implicit: step k takes its pivot row p_k among the rows not yet used (largest
+// |entry| in column k, lowest row on ties) and leaves every row where it is.
+// Scaling is lazy: the pivot row keeps its entries (column k set to 1) and is
+// multiplied by its pivot's reciprocal d_k when the block is written out; the
+// other rows subtract f (d_k row) as usual.  The stored result M then holds the
+// inverse with both permutations applied, inv[k][p_c] = d_k M[p_k][c], undone
+// at the write.  So no register is written under a runtime row index: the loop
+// over column blocks is unrolled (column k's registers are addressed at compile
+// time) and the pivot row, read once for staging, is selected by a scalar
+// switch on its block index, which is uniform over the workgroup.  Per step only
+// column k and the pivot row pass through LDS (double-buffered by step parity:
+// two barriers a step).
 template <int DIM, int EN, typename VT, typename AT, typename CT, int TR, int TC>
 __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const VT *__restrict__ val,
                                                              AT *__restrict__ ainv) {
   using Lc = Loc<DIM, EN>;
   constexpr int n = Lc::N;
   constexpr int RA = (n + TR - 1) / TR, CB = (n + TC - 1) / TC;
-  // pivot column and the two rows of each step, double-buffered by step parity
-  // so that a step needs two barriers: after the column, and after the rows
-  __shared__ CT s_col[2][n], s_row[2][n], s_rowk[2][n];
-  __shared__ int s_piv[n], s_idx[n], s_pos[n], s_pp[2];
+  static_assert(RA <= 8, "pivot-row switch covers eight row blocks");
+  __shared__ CT s_col[2][RA * TR], s_row[2][n];
+  __shared__ CT s_d[n];
+  __shared__ int s_prow[n], s_kinv[n], s_pp[2];
   const int e = blockIdx.x, tid = threadIdx.x;
   const int tr = tid % TR, tc = tid / TR;
   const int *map = L.e2nz + (size_t)e * L.E;
@@ -191,132 +198,111 @@ __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const V
       const long long p = (i < n && j < n) ? local_pos<DIM, EN>(L, i, j) : -1;
       A[a][b] = p >= 0 ? (CT)val[map[p]] : (CT)0;
     }
-  for (int k = 0; k < n; k++) {
-    const int q = k & 1;
-    CT *const col = s_col[q], *const row = s_row[q], *const rowk = s_rowk[q];
-    const int bk = k / TC, ck = k - bk * TC;
-    if (tc == ck) {
-      // the half-wave owning column k stages it and finds the pivot row from
-      // its registers (largest |entry| at or below k, first on ties)
-      CT best = (CT)-1;
-      int bi = n;
+  unsigned used = 0;  // bit a: row tr + TR a has been a pivot row
 #pragma unroll
-      for (int a = 0; a < RA; a++) {
-        CT v = 0;
+  for (int K = 0; K < CB; K++) {
+    for (int r = 0; r < TC; r++) {
+      const int k = TC * K + r;
+      if (k >= n) break;
+      const int q = k & 1;
+      CT *const col = s_col[q], *const row = s_row[q];
+      if (tc == r) {
+        // the TR lanes owning column k stage it and find the pivot row; a
+        // NaN counts as zero, so an unused row is always found
+        CT best = (CT)-1;
+        int bi = n;
 #pragma unroll
-        for (int b = 0; b < CB; b++)
-          if (b == bk) v = A[a][b];
-        const int i = tr + TR * a;
-        if (i < n) {
+        for (int a = 0; a < RA; a++) {
+          const int i = tr + TR * a;
+          const CT v = A[a][K];
           col[i] = v;
-          if (i >= k && fabs(v) > best) {
-            best = fabs(v);
+          CT m = fabs(v);
+          m = m == m ? m : (CT)0;
+          if (i < n && !((used >> a) & 1u) && m > best) {
+            best = m;
             bi = i;
           }
         }
-      }
 #pragma unroll
-      for (int o = TR / 2; o > 0; o >>= 1) {
-        const CT ov = __shfl_xor(best, o, TR);
-        const int oi = __shfl_xor(bi, o, TR);
-        if (ov > best || (ov == best && oi < bi)) {
-          best = ov;
-          bi = oi;
+        for (int o = TR / 2; o > 0; o >>= 1) {
+          const CT ov = __shfl_xor(best, o, TR);
+          const int oi = __shfl_xor(bi, o, TR);
+          if (ov > best || (ov == best && oi < bi)) {
+            best = ov;
+            bi = oi;
+          }
         }
+        if (tr == 0) s_pp[q] = bi;
       }
-      if (tr == 0) s_pp[q] = bi >= n ? k : bi;  // no comparable candidate (NaN column): keep the row
-    }
-    __syncthreads();
-    const int p = s_pp[q];
-    if (tid == 0) s_piv[k] = p;
-    const int ap = p / TR, rp = p - ap * TR, ak = k / TR, rk = k - ak * TR;
-    if (tr == rp || tr == rk) {
+      __syncthreads();
+      const int p = __builtin_amdgcn_readfirstlane(s_pp[q]);
+      if (tid == 0) {
+        s_prow[k] = p;
+        s_kinv[p] = k;
+      }
+      const int ap = p / TR, rp = p - ap * TR;
+      if (tr == rp) {
+        // stage the pivot row; its column-k entry takes no elimination
+        switch (ap) {
+#define GM_STAGE_ROW(c)                                            \
+  case c:                                                          \
+    if constexpr (c < RA) {                                        \
+      _Pragma("unroll") for (int b = 0; b < CB; b++) {             \
+        const int j = tc + TC * b;                                 \
+        if (j < n) row[j] = A[c][b];                               \
+      }                                                            \
+    }                                                              \
+    break;
+          GM_STAGE_ROW(0) GM_STAGE_ROW(1) GM_STAGE_ROW(2) GM_STAGE_ROW(3)
+          GM_STAGE_ROW(4) GM_STAGE_ROW(5) GM_STAGE_ROW(6) GM_STAGE_ROW(7)
+#undef GM_STAGE_ROW
+        }
+        if (tc == r) col[p] = (CT)0;
+        used |= 1u << ap;
+      }
+      __syncthreads();
+      const CT piv = row[k];
+      // a structurally empty column (a local pressure mode with no velocity
+      // coupling inside the element) leaves that unknown untouched
+      const CT d = fabs(piv) > (CT)1e-300 ? (CT)1 / piv : (CT)0;
+      if (tid == 0) s_d[k] = d;
+      // Step k of Gauss-Jordan on pivot row p: every other row i subtracts f_i
+      // times d row p (f_i its column-k entry) and takes -f_i d in column k;
+      // row p keeps its entries, with 1 in column k.  One FMA per entry after
+      // column k is set to 1 (row p) and 0 (the others); f_p is 0.
+      CT rb[CB], fa[RA];
 #pragma unroll
       for (int b = 0; b < CB; b++) {
-        CT vp = 0, vk = 0;
-#pragma unroll
-        for (int a = 0; a < RA; a++) {
-          if (a == ap) vp = A[a][b];
-          if (a == ak) vk = A[a][b];
-        }
         const int j = tc + TC * b;
-        if (j < n) {
-          if (tr == rp) row[j] = vp;
-          if (tr == rk) rowk[j] = vk;
-        }
+        rb[b] = j < n ? row[j] * d : (CT)0;
       }
-    }
-    __syncthreads();
-    const CT piv = row[k];
-    // a structurally empty column (a local pressure mode with no velocity
-    // coupling inside the element) leaves that unknown untouched
-    const CT d = fabs(piv) > (CT)1e-300 ? (CT)1 / piv : (CT)0;
-    const CT fk = col[k];
-    // Step k of in-place Gauss-Jordan with rows k and p swapped: the new row k
-    // is the pivot row scaled by d (column k: d); every other row i subtracts
-    // f_i times it (f_i its column-k entry; row p's is old row k's, fk) and
-    // takes -f_i d in column k.  Written as one FMA per entry: column k is
-    // zeroed and row p replaced by old row k first, row k overwritten last.
-    CT rb[CB], fa[RA];
+      rb[K] = tc == r ? d : rb[K];
 #pragma unroll
-    for (int b = 0; b < CB; b++) {
-      const int j = tc + TC * b;
-      rb[b] = j < n ? (j == k ? d : row[j] * d) : (CT)0;
-    }
+      for (int a = 0; a < RA; a++) fa[a] = col[tr + TR * a];
+      if (tc == r) {
 #pragma unroll
-    for (int a = 0; a < RA; a++) {
-      const int i = tr + TR * a;
-      fa[a] = (i < n && i != k) ? (i == p ? fk : col[i]) : (CT)0;
-    }
-    if (tc == ck) {
+        for (int a = 0; a < RA; a++) A[a][K] = tr + TR * a == p ? (CT)1 : (CT)0;
+      }
 #pragma unroll
       for (int a = 0; a < RA; a++)
 #pragma unroll
-        for (int b = 0; b < CB; b++)
-          if (b == bk) A[a][b] = (CT)0;
+        for (int b = 0; b < CB; b++) A[a][b] -= fa[a] * rb[b];
     }
-    if (tr == rp && p != k) {
-#pragma unroll
-      for (int b = 0; b < CB; b++) {
-        const int j = tc + TC * b;
-        const CT v = j < n ? rowk[j] : (CT)0;
-#pragma unroll
-        for (int a = 0; a < RA; a++)
-          if (a == ap) A[a][b] = b == bk && tc == ck ? (CT)0 : v;
-      }
-    }
-#pragma unroll
-    for (int a = 0; a < RA; a++)
-#pragma unroll
-      for (int b = 0; b < CB; b++) A[a][b] -= fa[a] * rb[b];
-    if (tr == rk) {
-#pragma unroll
-      for (int b = 0; b < CB; b++)
-#pragma unroll
-        for (int a = 0; a < RA; a++)
-          if (a == ak) A[a][b] = rb[b];
-    }
-  }
-  __syncthreads();
-  if (tid == 0) {
-    for (int j = 0; j < n; j++) s_idx[j] = j;
-    for (int k = n - 1; k >= 0; k--) {
-      const int p = s_piv[k];
-      const int t = s_idx[k];
-      s_idx[k] = s_idx[p];
-      s_idx[p] = t;
-    }
-    for (int j = 0; j < n; j++) s_pos[s_idx[j]] = j;
   }
   __syncthreads();
   AT *out = ainv + (size_t)e * n * n;
 #pragma unroll
-  for (int a = 0; a < RA; a++)
+  for (int a = 0; a < RA; a++) {
+    const int i = tr + TR * a;
+    if (i >= n) continue;
+    const int ki = s_kinv[i];
+    const CT di = s_d[ki];
 #pragma unroll
     for (int b = 0; b < CB; b++) {
-      const int i = tr + TR * a, c = tc + TC * b;
-      if (i < n && c < n) out[(size_t)s_pos[c] * n + i] = (AT)A[a][b];
+      const int c = tc + TC * b;
+      if (c < n) out[(size_t)s_prow[c] * n + ki] = (AT)(A[a][b] * di);
     }
+  }
 }
 
 // r = b - A x over all rows (MSR), G lanes per row

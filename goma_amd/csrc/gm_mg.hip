@@ -176,6 +176,34 @@ __global__ __launch_bounds__(NT) void k_vanka_invert(LevelDev L, const VT *__res
 // switch on its block index, which is uniform over the workgroup.  Per step only
 // column k and the pivot row pass through LDS (double-buffered by step parity:
 // two barriers a step).
+template <int R>
+__device__ inline int row_ror(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x120 + R, 0xf, 0xf, false);
+}
+template <int R>
+__device__ inline float row_ror(float v) {
+  return __builtin_bit_cast(float, row_ror<R>(__builtin_bit_cast(int, v)));
+}
+// v from the lane R places along in the same row of 16 lanes (DPP row_ror)
+template <int R>
+__device__ inline double row_ror(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)row_ror<R>((int)(unsigned)b);
+  const unsigned hi = (unsigned)row_ror<R>((int)(unsigned)(b >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// the larger |entry| of (best, bi) and the lane R places along; lower row on ties
+template <int R, typename CT>
+__device__ inline void pivot_combine(CT &best, int &bi) {
+  const CT ov = row_ror<R>(best);
+  const int oi = row_ror<R>(bi);
+  if (ov > best || (ov == best && oi < bi)) {
+    best = ov;
+    bi = oi;
+  }
+}
+
 template <int DIM, int EN, typename VT, typename AT, typename CT, int TR, int TC>
 __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const VT *__restrict__ val,
                                                              AT *__restrict__ ainv) {
@@ -183,6 +211,9 @@ __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const V
   constexpr int n = Lc::N;
   constexpr int RA = (n + TR - 1) / TR, CB = (n + TC - 1) / TC;
   static_assert(RA <= 8, "pivot-row switch covers eight row blocks");
+  static_assert(TR == 16, "the pivot search reduces over one DPP row of 16 lanes");
+  // every row / column index of the thread grid is inside the block
+  constexpr bool RFULL = n % TR == 0, CFULL = n % TC == 0;
   __shared__ CT s_col[2][RA * TR], s_row[2][n];
   __shared__ CT s_d[n];
   __shared__ int s_prow[n], s_kinv[n], s_pp[2];
@@ -195,7 +226,7 @@ __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const V
 #pragma unroll
     for (int b = 0; b < CB; b++) {
       const int i = tr + TR * a, j = tc + TC * b;
-      const long long p = (i < n && j < n) ? local_pos<DIM, EN>(L, i, j) : -1;
+      const long long p = ((RFULL || i < n) && (CFULL || j < n)) ? local_pos<DIM, EN>(L, i, j) : -1;
       A[a][b] = p >= 0 ? (CT)val[map[p]] : (CT)0;
     }
   unsigned used = 0;  // bit a: row tr + TR a has been a pivot row
@@ -218,20 +249,15 @@ __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const V
           col[i] = v;
           CT m = fabs(v);
           m = m == m ? m : (CT)0;
-          if (i < n && !((used >> a) & 1u) && m > best) {
+          if ((RFULL || i < n) && !((used >> a) & 1u) && m > best) {
             best = m;
             bi = i;
           }
         }
-#pragma unroll
-        for (int o = TR / 2; o > 0; o >>= 1) {
-          const CT ov = __shfl_xor(best, o, TR);
-          const int oi = __shfl_xor(bi, o, TR);
-          if (ov > best || (ov == best && oi < bi)) {
-            best = ov;
-            bi = oi;
-          }
-        }
+        pivot_combine<8>(best, bi);
+        pivot_combine<4>(best, bi);
+        pivot_combine<2>(best, bi);
+        pivot_combine<1>(best, bi);
         if (tr == 0) s_pp[q] = bi;
       }
       __syncthreads();
@@ -242,14 +268,21 @@ __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const V
       }
       const int ap = p / TR, rp = p - ap * TR;
       if (tr == rp) {
-        // stage the pivot row; its column-k entry takes no elimination
+        // Stage the pivot row.  The lane holding its column-k entry stores
+        // d = 1 / pivot, or 0 for a structurally empty column (a local
+        // pressure mode with no velocity coupling inside the element, which
+        // leaves that unknown untouched), and clears the row's factor f_p.
         switch (ap) {
 #define GM_STAGE_ROW(c)                                            \
   case c:                                                          \
     if constexpr (c < RA) {                                        \
       _Pragma("unroll") for (int b = 0; b < CB; b++) {             \
         const int j = tc + TC * b;                                 \
-        if (j < n) row[j] = A[c][b];                               \
+        if (CFULL || j < n) row[j] = A[c][b];                      \
+      }                                                            \
+      if (tc == r) {                                               \
+        const CT pv = A[c][K];                                     \
+        s_d[k] = fabs(pv) > (CT)1e-300 ? (CT)1 / pv : (CT)0;       \
       }                                                            \
     }                                                              \
     break;
@@ -261,11 +294,7 @@ __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const V
         used |= 1u << ap;
       }
       __syncthreads();
-      const CT piv = row[k];
-      // a structurally empty column (a local pressure mode with no velocity
-      // coupling inside the element) leaves that unknown untouched
-      const CT d = fabs(piv) > (CT)1e-300 ? (CT)1 / piv : (CT)0;
-      if (tid == 0) s_d[k] = d;
+      const CT d = s_d[k];
       // Step k of Gauss-Jordan on pivot row p: every other row i subtracts f_i
       // times d row p (f_i its column-k entry) and takes -f_i d in column k;
       // row p keeps its entries, with 1 in column k.  One FMA per entry after
@@ -274,7 +303,7 @@ __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const V
 #pragma unroll
       for (int b = 0; b < CB; b++) {
         const int j = tc + TC * b;
-        rb[b] = j < n ? row[j] * d : (CT)0;
+        rb[b] = (CFULL || j < n) ? row[j] * d : (CT)0;
       }
       rb[K] = tc == r ? d : rb[K];
 #pragma unroll

@@ -38,6 +38,23 @@ __device__ inline AT row_dot(const VT *__restrict__ a, const int *__restrict__ c
   return s0 + s1;
 }
 
+// v from the lane R places along in the same row of 16 lanes (DPP row_ror)
+template <int R>
+__device__ inline int row_ror_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x120 + R, 0xf, 0xf, false);
+}
+template <int R>
+__device__ inline float row_ror_f(float v) {
+  return __builtin_bit_cast(float, row_ror_i<R>(__builtin_bit_cast(int, v)));
+}
+template <int R>
+__device__ inline double row_ror_f(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)row_ror_i<R>((int)(unsigned)b);
+  const unsigned hi = (unsigned)row_ror_i<R>((int)(unsigned)(b >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // The rows of the MSR matrix one node at a time, without reading a column
 // index per nonzero.  Row u of node n holds, for every neighbour node m of n
 // in ascending order (n itself included), the unknowns of m in ascending
@@ -47,15 +64,20 @@ __device__ inline AT row_dot(const VT *__restrict__ a, const int *__restrict__ c
 // node writes the node's column pattern to LDS from its neighbour list (one
 // lane per neighbour, offsets by a wave prefix sum of the neighbours'
 // unknown counts): PA for rows that take every column, PT for the energy
-// row.  It then walks the node's rows in turn with coalesced value loads
-// (two entries per lane and step); entry e of row r is pattern position e,
-// or e + 1 from the diagonal's position on.  The node graph costs 4 bytes per
+// row.  It then takes the node's rows four at a time, 16 lanes (one DPP row)
+// per row, with coalesced value loads (two entries per lane and step) and a
+// reduction by DPP row rotations; entry e of row r is pattern position e, or
+// e + 1 from the diagonal's position on.  The node graph costs 4 bytes per
 // neighbour for all the node's rows instead of 4 bytes per nonzero.
-// Measured at 48^3 HEX27: 2.49 ms per product against 2.85 ms for 64 lanes
-// per row with the column list; a workgroup per node (wave 0 building the
-// pattern, a row per wave) took 3.39 ms, rows in pairs 2.56 ms, and the
-// node's rows walked as one block 3.55 ms.  `cap` is the longest pattern of the mesh (LDS: 2 cap
-// ints per wave).  RESID: y = b - A x;  otherwise y = A x.
+// Measured at 48^3 HEX27: 2.05 ms per product (4.05 TB/s algorithmic), the
+// multigrid residual 0.596 ms averaged over the levels (4.0 TB/s).  Slower
+// variants: the node's rows in turn with 64 lanes each (2.49 ms: one row's
+// loads in flight at a time), 64 lanes per row with the column list (2.85),
+// four accumulators per lane (2.10), a workgroup per node with wave 0
+// building the pattern and a row per wave (3.39), the node's rows walked as
+// one block with per-entry row selection (3.55).  `cap` is the longest
+// pattern of the mesh (LDS: 2 cap ints per wave).  RESID: y = b - A x;
+// otherwise y = A x.
 template <typename AT, typename VT, typename XT, typename YT, bool RESID>
 __global__ __launch_bounds__(256) void k_msr_nodes(int num_nodes, int dim, int nvn, int cap,
                                                    const int *__restrict__ fu,
@@ -110,24 +132,32 @@ __global__ __launch_bounds__(256) void k_msr_nodes(int num_nodes, int dim, int n
   __syncthreads();
   if (!valid) return;
   const bool energy = nvn > dim;
-  for (int r = 0; r < nu; r++) {
-    const int u = f0 + r;
-    const int k0 = bindx[u], len = bindx[u + 1] - k0;
-    const bool tr = energy && r == dim;
-    const int *P = tr ? PT : PA;
-    const int hole = (tr ? selfT : selfA) + r;
+  // four rows at a time, 16 lanes (one DPP row) per row
+  const int g = lane >> 4, l = lane & 15;
+  for (int r0 = 0; r0 < nu; r0 += 4) {
+    const int r = r0 + g;
     AT s0 = 0, s1 = 0;
-    int e = lane;
-    for (; e + 64 < len; e += 128) {
-      const int e1 = e + 64;
-      s0 += (AT)a[k0 + e] * (AT)x[P[e + (e >= hole ? 1 : 0)]];
-      s1 += (AT)a[k0 + e1] * (AT)x[P[e1 + (e1 >= hole ? 1 : 0)]];
+    int u = 0;
+    if (r < nu) {
+      u = f0 + r;
+      const int k0 = bindx[u], len = bindx[u + 1] - k0;
+      const bool tr = energy && r == dim;
+      const int *P = tr ? PT : PA;
+      const int hole = (tr ? selfT : selfA) + r;
+      int e = l;
+      for (; e + 16 < len; e += 32) {
+        const int e1 = e + 16;
+        s0 += (AT)a[k0 + e] * (AT)x[P[e + (e >= hole ? 1 : 0)]];
+        s1 += (AT)a[k0 + e1] * (AT)x[P[e1 + (e1 >= hole ? 1 : 0)]];
+      }
+      if (e < len) s0 += (AT)a[k0 + e] * (AT)x[P[e + (e >= hole ? 1 : 0)]];
     }
-    if (e < len) s0 += (AT)a[k0 + e] * (AT)x[P[e + (e >= hole ? 1 : 0)]];
     AT v = s0 + s1;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (lane == 0) {
+    v += row_ror_f<8>(v);
+    v += row_ror_f<4>(v);
+    v += row_ror_f<2>(v);
+    v += row_ror_f<1>(v);
+    if (l == 0 && r < nu) {
       const AT d = (AT)a[u] * (AT)x[u];
       y[u] = RESID ? (YT)((AT)b[u] - v - d) : (YT)(v + d);
     }

@@ -162,6 +162,17 @@ __global__ __launch_bounds__(NT) void k_vanka_invert(LevelDev L, const VT *__res
   }
 }
 
+// the larger |entry| of (best, bi) and the lane R places along; lower row on ties
+template <int R, typename CT>
+__device__ inline void pivot_combine(CT &best, int &bi) {
+  const CT ov = row_ror_f<R>(best);
+  const int oi = row_ror_i<R>(bi);
+  if (ov > best || (ov == best && oi < bi)) {
+    best = ov;
+    bi = oi;
+  }
+}
+
 // The same inverse with the block held in registers: thread (tr, tc) of a
 // TR x TC grid owns rows tr + TR a and columns tc + TC b.  Pivoting is
 // implicit: step k takes its pivot row p_k among the rows not yet used (largest
@@ -176,34 +187,6 @@ __global__ __launch_bounds__(NT) void k_vanka_invert(LevelDev L, const VT *__res
 // switch on its block index, which is uniform over the workgroup.  Per step only
 // column k and the pivot row pass through LDS (double-buffered by step parity:
 // two barriers a step).
-template <int R>
-__device__ inline int row_ror(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x120 + R, 0xf, 0xf, false);
-}
-template <int R>
-__device__ inline float row_ror(float v) {
-  return __builtin_bit_cast(float, row_ror<R>(__builtin_bit_cast(int, v)));
-}
-// v from the lane R places along in the same row of 16 lanes (DPP row_ror)
-template <int R>
-__device__ inline double row_ror(double v) {
-  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-  const unsigned lo = (unsigned)row_ror<R>((int)(unsigned)b);
-  const unsigned hi = (unsigned)row_ror<R>((int)(unsigned)(b >> 32));
-  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
-}
-
-// the larger |entry| of (best, bi) and the lane R places along; lower row on ties
-template <int R, typename CT>
-__device__ inline void pivot_combine(CT &best, int &bi) {
-  const CT ov = row_ror<R>(best);
-  const int oi = row_ror<R>(bi);
-  if (ov > best || (ov == best && oi < bi)) {
-    best = ov;
-    bi = oi;
-  }
-}
-
 template <int DIM, int EN, typename VT, typename AT, typename CT, int TR, int TC>
 __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const VT *__restrict__ val,
                                                              AT *__restrict__ ainv) {

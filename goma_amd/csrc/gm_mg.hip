@@ -173,20 +173,57 @@ __device__ inline void pivot_combine(CT &best, int &bi) {
   }
 }
 
+// Pivot of one column: the TR lanes of one DPP row hold its entries cv[a]
+// (rows tr + TR a).  They stage the column in col, pick the row not yet used
+// (bit a of `used`) with the largest |entry| (a NaN counts as zero, ties go to
+// the lower row, so an unused row is always found), and the lane holding it
+// posts the row to *pp, its reciprocal to *dv (0 for a structurally empty
+// column: a local pressure mode with no velocity coupling inside the element,
+// which leaves that unknown untouched) and clears the row's factor in col.
+template <int n, int RA, int TR, typename CT>
+__device__ inline void post_pivot(const CT (&cv)[RA], unsigned used, int tr, CT *col, int *pp,
+                                  CT *dv) {
+  CT best = (CT)-1;
+  int bi = n;
+#pragma unroll
+  for (int a = 0; a < RA; a++) {
+    const int i = tr + TR * a;
+    const CT v = cv[a];
+    col[i] = v;
+    CT m = fabs(v);
+    m = m == m ? m : (CT)0;
+    if ((n % TR == 0 || i < n) && !((used >> a) & 1u) && m > best) {
+      best = m;
+      bi = i;
+    }
+  }
+  pivot_combine<8>(best, bi);
+  pivot_combine<4>(best, bi);
+  pivot_combine<2>(best, bi);
+  pivot_combine<1>(best, bi);
+  if (tr == (bi & (TR - 1))) {
+    const CT pv = col[bi];
+    *dv = fabs(pv) > (CT)1e-300 ? (CT)1 / pv : (CT)0;
+    col[bi] = (CT)0;
+    *pp = bi;
+  }
+}
+
 // The same inverse with the block held in registers: thread (tr, tc) of a
 // TR x TC grid owns rows tr + TR a and columns tc + TC b.  Pivoting is
-// implicit: step k takes its pivot row p_k among the rows not yet used (largest
-// |entry| in column k, lowest row on ties) and leaves every row where it is.
-// Scaling is lazy: the pivot row keeps its entries (column k set to 1) and is
-// multiplied by its pivot's reciprocal d_k when the block is written out; the
-// other rows subtract f (d_k row) as usual.  The stored result M then holds the
-// inverse with both permutations applied, inv[k][p_c] = d_k M[p_k][c], undone
-// at the write.  So no register is written under a runtime row index: the loop
-// over column blocks is unrolled (column k's registers are addressed at compile
-// time) and the pivot row, read once for staging, is selected by a scalar
-// switch on its block index, which is uniform over the workgroup.  Per step only
-// column k and the pivot row pass through LDS (double-buffered by step parity:
-// two barriers a step).
+// implicit: step k takes its pivot row p_k among the rows not yet used and
+// leaves every row where it is.  Scaling is lazy: the pivot row keeps its
+// entries (column k set to 1) and is multiplied by its pivot's reciprocal d_k
+// when the block is written out; the other rows subtract f (d_k row) as usual.
+// The stored result M then holds the inverse with both permutations applied,
+// inv[k][p_c] = d_k M[p_k][c], undone at the write.  So no register is
+// written under a runtime row index: the loop over column blocks is unrolled
+// (column k's registers are addressed at compile time) and the pivot row is
+// read through a scalar switch on its block index, which is uniform over the
+// workgroup.  One barrier a step: the lanes owning column k post its pivot,
+// the pivot's reciprocal and the column in LDS (double-buffered by step
+// parity); the pivot row's entries that a thread needs are held by lanes of
+// its own wave (row rp, same tc) and pass through LDS within the wave.
 template <int DIM, int EN, typename VT, typename AT, typename CT, int TR, int TC>
 __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const VT *__restrict__ val,
                                                              AT *__restrict__ ainv) {
@@ -194,11 +231,10 @@ __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const V
   constexpr int n = Lc::N;
   constexpr int RA = (n + TR - 1) / TR, CB = (n + TC - 1) / TC;
   static_assert(RA <= 8, "pivot-row switch covers eight row blocks");
-  static_assert(TR == 16, "the pivot search reduces over one DPP row of 16 lanes");
+  static_assert(TR == 16, "a column's TR lanes form one DPP row");
   // every row / column index of the thread grid is inside the block
   constexpr bool RFULL = n % TR == 0, CFULL = n % TC == 0;
-  __shared__ CT s_col[2][RA * TR], s_row[2][n];
-  __shared__ CT s_d[n];
+  __shared__ CT s_col[2][RA * TR], s_row[CB * TC], s_dv[2], s_d[n];
   __shared__ int s_prow[n], s_kinv[n], s_pp[2];
   const int e = blockIdx.x, tid = threadIdx.x;
   const int tr = tid % TR, tc = tid / TR;
@@ -219,65 +255,44 @@ __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const V
       const int k = TC * K + r;
       if (k >= n) break;
       const int q = k & 1;
-      CT *const col = s_col[q], *const row = s_row[q];
       if (tc == r) {
-        // the TR lanes owning column k stage it and find the pivot row; a
-        // NaN counts as zero, so an unused row is always found
-        CT best = (CT)-1;
-        int bi = n;
+        CT cv[RA];
 #pragma unroll
-        for (int a = 0; a < RA; a++) {
-          const int i = tr + TR * a;
-          const CT v = A[a][K];
-          col[i] = v;
-          CT m = fabs(v);
-          m = m == m ? m : (CT)0;
-          if ((RFULL || i < n) && !((used >> a) & 1u) && m > best) {
-            best = m;
-            bi = i;
-          }
-        }
-        pivot_combine<8>(best, bi);
-        pivot_combine<4>(best, bi);
-        pivot_combine<2>(best, bi);
-        pivot_combine<1>(best, bi);
-        if (tr == 0) s_pp[q] = bi;
+        for (int a = 0; a < RA; a++) cv[a] = A[a][K];
+        post_pivot<n, RA, TR>(cv, used, tr, s_col[q], &s_pp[q], &s_dv[q]);
       }
       __syncthreads();
       const int p = __builtin_amdgcn_readfirstlane(s_pp[q]);
+      const CT d = s_dv[q];
       if (tid == 0) {
         s_prow[k] = p;
         s_kinv[p] = k;
+        s_d[k] = d;
       }
       const int ap = p / TR, rp = p - ap * TR;
+      // This wave's columns of the pivot row pass through s_row: the lanes of
+      // row p write them and the same wave reads them back (LDS keeps one
+      // wave's operations in order; the fences keep the compiler from moving
+      // the reads above the writes), so no workgroup barrier is needed.
       if (tr == rp) {
-        // Stage the pivot row.  The lane holding its column-k entry stores
-        // d = 1 / pivot, or 0 for a structurally empty column (a local
-        // pressure mode with no velocity coupling inside the element, which
-        // leaves that unknown untouched), and clears the row's factor f_p.
         switch (ap) {
-#define GM_STAGE_ROW(c)                                            \
+#define GM_PIVOT_ROW(c)                                            \
   case c:                                                          \
     if constexpr (c < RA) {                                        \
       _Pragma("unroll") for (int b = 0; b < CB; b++) {             \
         const int j = tc + TC * b;                                 \
-        if (CFULL || j < n) row[j] = A[c][b];                      \
-      }                                                            \
-      if (tc == r) {                                               \
-        const CT pv = A[c][K];                                     \
-        s_d[k] = fabs(pv) > (CT)1e-300 ? (CT)1 / pv : (CT)0;       \
+        if (CFULL || j < n) s_row[j] = A[c][b];                    \
       }                                                            \
     }                                                              \
     break;
-          GM_STAGE_ROW(0) GM_STAGE_ROW(1) GM_STAGE_ROW(2) GM_STAGE_ROW(3)
-          GM_STAGE_ROW(4) GM_STAGE_ROW(5) GM_STAGE_ROW(6) GM_STAGE_ROW(7)
-#undef GM_STAGE_ROW
+          GM_PIVOT_ROW(0) GM_PIVOT_ROW(1) GM_PIVOT_ROW(2) GM_PIVOT_ROW(3)
+          GM_PIVOT_ROW(4) GM_PIVOT_ROW(5) GM_PIVOT_ROW(6) GM_PIVOT_ROW(7)
+#undef GM_PIVOT_ROW
         }
-        if (tc == r) col[p] = (CT)0;
-        used |= 1u << ap;
       }
-      __syncthreads();
-      const CT d = s_d[k];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       // Step k of Gauss-Jordan on pivot row p: every other row i subtracts f_i
       // times d row p (f_i its column-k entry) and takes -f_i d in column k;
       // row p keeps its entries, with 1 in column k.  One FMA per entry after
@@ -286,11 +301,11 @@ __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const V
 #pragma unroll
       for (int b = 0; b < CB; b++) {
         const int j = tc + TC * b;
-        rb[b] = (CFULL || j < n) ? row[j] * d : (CT)0;
+        rb[b] = (CFULL || j < n) ? s_row[j] * d : (CT)0;
       }
       rb[K] = tc == r ? d : rb[K];
 #pragma unroll
-      for (int a = 0; a < RA; a++) fa[a] = col[tr + TR * a];
+      for (int a = 0; a < RA; a++) fa[a] = s_col[q][tr + TR * a];
       if (tc == r) {
 #pragma unroll
         for (int a = 0; a < RA; a++) A[a][K] = tr + TR * a == p ? (CT)1 : (CT)0;
@@ -299,6 +314,7 @@ __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const V
       for (int a = 0; a < RA; a++)
 #pragma unroll
         for (int b = 0; b < CB; b++) A[a][b] -= fa[a] * rb[b];
+      if (tr == rp) used |= 1u << ap;
     }
   }
   __syncthreads();
@@ -306,13 +322,13 @@ __global__ __launch_bounds__(TR *TC) void k_vanka_invert_reg(LevelDev L, const V
 #pragma unroll
   for (int a = 0; a < RA; a++) {
     const int i = tr + TR * a;
-    if (i >= n) continue;
+    if (!RFULL && i >= n) continue;
     const int ki = s_kinv[i];
     const CT di = s_d[ki];
 #pragma unroll
     for (int b = 0; b < CB; b++) {
       const int c = tc + TC * b;
-      if (c < n) out[(size_t)s_prow[c] * n + ki] = (AT)(A[a][b] * di);
+      if (CFULL || c < n) out[(size_t)s_prow[c] * n + ki] = (AT)(A[a][b] * di);
     }
   }
 }

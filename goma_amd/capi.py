@@ -32,7 +32,8 @@ GM_PC_MG = 2
 KERNEL_NAMES = ["k_fill", "k_spmv_msr", "k_ilu_lower", "k_ilu_upper", "k_ilu_factor", "k_mdot",
                 "k_orth_update", "k_mg_resid", "k_vanka_update", "k_csr_apply", "k_vanka_invert",
                 "k_fill(residual only)", "k_mg_resid_rows", "k_dense_gemv", "k_galerkin",
-                "k_elem_scatter", "k_band_panel", "k_band_update", "k_band_solve", "k_vanka_gather"]
+                "k_elem_scatter", "k_band_panel", "k_band_update", "k_band_solve", "k_vanka_gather",
+                "k_row_sum_scale"]
 
 
 class GmMesh(C.Structure):

@@ -55,9 +55,11 @@ int gm_zero32(gm_ctx *c, float *p, long long n) {
 
 // ---------------------------------------------------------------- row-sum scaling
 // row_sum_scale_MSR (src/sl_matrix_util.c:506-600): s_i = sgn(a_ii) sum_j |a_ij|
-// (sign only when |a_ii| > 1e-200); a_ij /= s_i; b_i /= s_i.  One 16-lane
-// group per row.  Row sums accumulate in FP64 for either value type.
-template <typename VT, int G>
+// (sign only when |a_ii| > 1e-200); a_ij /= s_i; b_i /= s_i.  G lanes per row;
+// the first G*KEEP entries of a row stay in registers between the sum and the
+// scaling (one read of the matrix instead of two), longer rows re-read the
+// rest.  Row sums accumulate in FP64 for either value type.
+template <typename VT, int G, int KEEP>
 __global__ __launch_bounds__(256) void k_row_sum_scale(int N, VT *__restrict__ a,
                                                        const int *__restrict__ bindx,
                                                        double *__restrict__ b,
@@ -67,14 +69,25 @@ __global__ __launch_bounds__(256) void k_row_sum_scale(int N, VT *__restrict__ a
   if (gid >= N) return;
   const int r = gid;
   const int k0 = bindx[r], k1 = bindx[r + 1];
-  double s = 0.0;
-  for (int k = k0 + lane; k < k1; k += G) s += fabs((double)a[k]);
+  double s = 0.0, v[KEEP];
+#pragma unroll
+  for (int t = 0; t < KEEP; t++) {
+    const int k = k0 + lane + G * t;
+    v[t] = k < k1 ? (double)a[k] : 0.0;
+    s += fabs(v[t]);
+  }
+  for (int k = k0 + lane + G * KEEP; k < k1; k += G) s += fabs((double)a[k]);
 #pragma unroll
   for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
   const double d = (double)a[r];
   double rs = fabs(d) + s;
   if (fabs(d) > 1.0e-200) rs = d > 0 ? rs : -rs;
-  for (int k = k0 + lane; k < k1; k += G) a[k] = (VT)((double)a[k] / rs);
+#pragma unroll
+  for (int t = 0; t < KEEP; t++) {
+    const int k = k0 + lane + G * t;
+    if (k < k1) a[k] = (VT)(v[t] / rs);
+  }
+  for (int k = k0 + lane + G * KEEP; k < k1; k += G) a[k] = (VT)((double)a[k] / rs);
   if (lane == 0) {
     scale[r] = rs;
     b[r] = b[r] / rs;
@@ -85,14 +98,20 @@ __global__ __launch_bounds__(256) void k_row_sum_scale(int N, VT *__restrict__ a
 template <int G>
 static void scale_g(gm_ctx *c, double *val, float *val32, double *b, double *scale) {
   const int blocks = (int)(((long long)c->N * G + 255) / 256);
+  const double nnz = (double)c->nval - 1.0;
+  // values read and written, the row pointers, b read and written, the scale
+  const double bytes = nnz * 2.0 * (val32 ? 4.0 : 8.0) + 4.0 * (c->N + 1) + c->N * 24.0;
+  constexpr int KEEP = G == 64 ? 9 : 4;  // HEX27 rows up to 576 entries, QUAD9 up to 64
   if (val32)
-    k_row_sum_scale<float, G><<<blocks, 256, 0, c->stream>>>(c->N, val32, c->d_bindx, b, scale);
+    gm_launch(c, GM_K_ROW_SCALE, bytes, k_row_sum_scale<float, G, KEEP>, dim3(blocks), dim3(256),
+              0, c->stream, c->N, val32, c->d_bindx, b, scale);
   else
-    k_row_sum_scale<double, G><<<blocks, 256, 0, c->stream>>>(c->N, val, c->d_bindx, b, scale);
+    gm_launch(c, GM_K_ROW_SCALE, bytes, k_row_sum_scale<double, G, KEEP>, dim3(blocks), dim3(256),
+              0, c->stream, c->N, val, c->d_bindx, b, scale);
 }
 
 int gm_scale_launch(gm_ctx *c, double *val, float *val32, double *b, double *scale) {
-  // a wave per HEX27 row (~250 entries), 16 lanes per QUAD9 row (~60)
+  // a wave per HEX27 row (~250 entries, at most ~530), 16 lanes per QUAD9 row (~60)
   if (c->dim == 3) scale_g<64>(c, val, val32, b, scale);
   else scale_g<16>(c, val, val32, b, scale);
   GM_HIP(hipGetLastError());

@@ -380,7 +380,8 @@ int gm_mg_copy_level_matrix(const gm_ctx *ctx, int level, double *val);
 #define GM_K_BAND_UPDATE 17 /* k_band_update: trailing update of one panel */
 #define GM_K_BAND_SOLVE 18 /* k_band_fwd / k_band_bwd: one 64-row block of the triangular solves */
 #define GM_K_VANKA_GATHER 19 /* k_vanka_gather: additive smoother, element corrections summed per unknown */
-#define GM_K_COUNT 20
+#define GM_K_ROW_SCALE 20 /* k_row_sum_scale: row sums and scaling of the Jacobian and residual */
+#define GM_K_COUNT 21
 /* on = 0: off; 1: time every launch; k > 1: time a pseudo-random one launch
  * in k of each kernel (lower overhead), durations scaled to all launches by
  * the launch count. */

@@ -33,7 +33,7 @@ KERNEL_NAMES = ["k_fill", "k_spmv_msr", "k_ilu_lower", "k_ilu_upper", "k_ilu_fac
                 "k_orth_update", "k_mg_resid", "k_vanka_update", "k_csr_apply", "k_vanka_invert",
                 "k_fill(residual only)", "k_mg_resid_rows", "k_dense_gemv", "k_galerkin",
                 "k_elem_scatter", "k_band_panel", "k_band_update", "k_band_solve", "k_vanka_gather",
-                "k_row_sum_scale"]
+                "k_row_sum_scale", "k_gjb(coarse inverse)"]
 
 
 class GmMesh(C.Structure):
@@ -140,6 +140,7 @@ EXPORTS = [
     "gm_add_qside", "gm_clear_qside", "gm_solve_nonlinear_transient_dev", "gm_solve_transient_dev",
     "gm_num_nodes", "gm_num_elems", "gm_mg_build", "gm_mg_level", "gm_mg_copy_prolongation",
     "gm_matrix_fill_full_assign_dev", "gm_mlp_predict_noalloc_dev", "gm_mg_copy_block_inverse",
+    "gm_mg_copy_coarse_inverse",
     "gm_h_elem_avg", "gm_velocity_norm_dev", "gm_set_velocity_norm",
 ]
 
@@ -194,6 +195,7 @@ def lib():
     L.gm_mg_setup_dev.argtypes = [vp, vp, vp, vp]
     L.gm_mg_copy_level_matrix.argtypes = [vp, ip, vp]
     L.gm_mg_copy_block_inverse.argtypes = [vp, ip, ip, vp]
+    L.gm_mg_copy_coarse_inverse.argtypes = [vp, vp]
     L.gm_prof_enable.argtypes = [vp, ip]
     L.gm_prof_get.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.gm_solve_nonlinear_guess_dev.argtypes = [vp, vp, vp, C.POINTER(GmSolverOpts), ip,
@@ -493,6 +495,15 @@ class Context:
         n = _check(lib().gm_mg_copy_block_inverse(self.h, level, elem, out.ctypes.data),
                    "gm_mg_copy_block_inverse")
         return out[: n * n].reshape(n, n).T.copy()
+
+    def mg_coarse_inverse(self) -> np.ndarray:
+        """Dense inverse of the coarsest level's Jacobian from the last set-up
+        with an exact coarse solve, as an n x n array."""
+        n = self._coarse[-1].N
+        out = np.zeros(n * n)
+        m = _check(lib().gm_mg_copy_coarse_inverse(self.h, out.ctypes.data), "gm_mg_copy_coarse_inverse")
+        assert m == n
+        return out.reshape(n, n)
 
     def mg_setup_dev(self, x_ptr, val_ptr, scale_ptr=None):
         _check(lib().gm_mg_setup_dev(self.h, x_ptr, val_ptr, scale_ptr), "gm_mg_setup_dev")

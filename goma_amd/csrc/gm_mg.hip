@@ -380,8 +380,6 @@ __global__ void k_msr_to_dense(int n, const VT *__restrict__ a, const int *__res
   if (threadIdx.x == 0) D[(size_t)row * n + row] = (double)a[row];
 }
 
-// step k, one block: pivot row p = argmax_{i>=k} |D[i][k]| (first on ties),
-// swap rows k and p, keep copies of the pivot column and row
 // Dense Gauss-Jordan inverse of the coarsest level with partial pivoting,
 // blocked in panels of at most GJ_NB columns (same pivots and interchanges as
 // the column-at-a-time algorithm).  For panel K:
@@ -400,15 +398,17 @@ constexpr int GJ_NB = 32;
 
 // The panel lives in registers for its column steps: thread t of 512 owns
 // rows t + 512 r (r < RPT) of the NB panel columns (RPT NB <= 64 doubles),
-// so D is read and written once per panel; the pivot row and the
-// interchange go through LDS.
+// so D is read and written once per panel.  Per step: the pivot candidate of
+// each wave (DPP row rotations, then the wave's four rows by readlane) goes to
+// LDS; after one barrier every thread merges the eight candidates itself; the
+// pivot row and row k pass through LDS; a second barrier.  The LDS buffers
+// alternate by step parity, so no third barrier guards their reuse.
 template <int NB, int RPT>
 __global__ __launch_bounds__(512) void k_gjb_panel(int n, int kb0, int nb, double *__restrict__ D,
                                                     int *__restrict__ perm, int *__restrict__ flag) {
-  __shared__ double s_rk[NB], s_rp[NB];
-  __shared__ double sv[8];
-  __shared__ int si[8];
-  __shared__ int s_p;
+  __shared__ double s_rk[2][NB], s_rp[2][NB];
+  __shared__ double sv[2][8];
+  __shared__ int si[2][8];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   double P[RPT][NB];
 #pragma unroll
@@ -420,85 +420,93 @@ __global__ __launch_bounds__(512) void k_gjb_panel(int n, int kb0, int nb, doubl
 #pragma unroll
   for (int kk = 0; kk < NB; kk++) {
     if (kk < nb) {
-    const int k = kb0 + kk;
-    double best = -1.0;
-    int bi = n;
+      const int k = kb0 + kk, q = kk & 1;
+      double best = -1.0;
+      int bi = n;
 #pragma unroll
-    for (int r = 0; r < RPT; r++) {
-      const int i = tid + 512 * r;
-      const double v = fabs(P[r][kk]);
-      if (i >= k && i < n && v > best) {
-        best = v;
-        bi = i;
-      }
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const double ov = __shfl_xor(best, o, 64);
-      const int oi = __shfl_xor(bi, o, 64);
-      if (ov > best || (ov == best && oi < bi)) {
-        best = ov;
-        bi = oi;
-      }
-    }
-    if (lane == 0) {
-      sv[w] = best;
-      si[w] = bi;
-    }
-    __syncthreads();
-    if (tid == 0) {
-      double bv = sv[0];
-      int bix = si[0];
-      for (int q = 1; q < 8; q++)
-        if (sv[q] > bv || (sv[q] == bv && si[q] < bix)) {
-          bv = sv[q];
-          bix = si[q];
+      for (int r = 0; r < RPT; r++) {
+        const int i = tid + 512 * r;
+        const double v = fabs(P[r][kk]);
+        if (i >= k && i < n && v > best) {
+          best = v;
+          bi = i;
         }
-      if (!(bv > 0.0)) flag[3] = 1;  // singular coarse matrix
-      if (bix >= n) bix = k;
-      s_p = bix;
-      perm[k] = bix;
-    }
-    __syncthreads();
-    const int p = s_p;
-    // rows k and p to LDS (row p is the new pivot row), then interchanged
-#pragma unroll
-    for (int r = 0; r < RPT; r++) {
-      const int i = tid + 512 * r;
-      if (i == p)
-#pragma unroll
-        for (int j = 0; j < NB; j++) s_rp[j] = P[r][j];
-      if (i == k && p != k)
-#pragma unroll
-        for (int j = 0; j < NB; j++) s_rk[j] = P[r][j];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < RPT; r++) {
-      const int i = tid + 512 * r;
-      if (p != k && i == k)
-#pragma unroll
-        for (int j = 0; j < NB; j++) P[r][j] = s_rp[j];
-      if (p != k && i == p)
-#pragma unroll
-        for (int j = 0; j < NB; j++) P[r][j] = s_rk[j];
-    }
-    const double piv = s_rp[kk];
-    const double inv = piv != 0.0 ? 1.0 / piv : 0.0;
-#pragma unroll
-    for (int r = 0; r < RPT; r++) {
-      const int i = tid + 512 * r;
-      const double f = P[r][kk];
-      if (i == k) {
-#pragma unroll
-        for (int j = 0; j < NB; j++) P[r][j] = j == kk ? inv : s_rp[j] * inv;
-      } else {
-        const double g = f * inv;
-#pragma unroll
-        for (int j = 0; j < NB; j++) P[r][j] = j == kk ? -g : P[r][j] - g * s_rp[j];
       }
-    }
-    __syncthreads();  // s_rp / s_rk reused by the next step
+      pivot_combine<8>(best, bi);
+      pivot_combine<4>(best, bi);
+      pivot_combine<2>(best, bi);
+      pivot_combine<1>(best, bi);
+      // lanes 0, 16, 32, 48 hold their rows' candidates
+#pragma unroll
+      for (int g = 1; g < 4; g++) {
+        const unsigned long long bb = __builtin_bit_cast(unsigned long long, best);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)bb, 16 * g);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(bb >> 32), 16 * g);
+        const double ov = __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+        const int oi = __builtin_amdgcn_readlane(bi, 16 * g);
+        if (lane == 0 && (ov > best || (ov == best && oi < bi))) {
+          best = ov;
+          bi = oi;
+        }
+      }
+      if (lane == 0) {
+        sv[q][w] = best;
+        si[q][w] = bi;
+      }
+      __syncthreads();
+      double bv = sv[q][0];
+      int p = si[q][0];
+#pragma unroll
+      for (int g = 1; g < 8; g++) {
+        const double ov = sv[q][g];
+        const int oi = si[q][g];
+        if (ov > bv || (ov == bv && oi < p)) {
+          bv = ov;
+          p = oi;
+        }
+      }
+      if (p >= n) p = k;
+      if (tid == 0) {
+        if (!(bv > 0.0)) flag[3] = 1;  // singular coarse matrix
+        perm[k] = p;
+      }
+      // rows k and p to LDS (row p is the new pivot row), then interchanged
+#pragma unroll
+      for (int r = 0; r < RPT; r++) {
+        const int i = tid + 512 * r;
+        if (i == p)
+#pragma unroll
+          for (int j = 0; j < NB; j++) s_rp[q][j] = P[r][j];
+        if (i == k && p != k)
+#pragma unroll
+          for (int j = 0; j < NB; j++) s_rk[q][j] = P[r][j];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < RPT; r++) {
+        const int i = tid + 512 * r;
+        if (p != k && i == k)
+#pragma unroll
+          for (int j = 0; j < NB; j++) P[r][j] = s_rp[q][j];
+        if (p != k && i == p)
+#pragma unroll
+          for (int j = 0; j < NB; j++) P[r][j] = s_rk[q][j];
+      }
+      const double piv = s_rp[q][kk];
+      const double inv = piv != 0.0 ? 1.0 / piv : 0.0;
+#pragma unroll
+      for (int r = 0; r < RPT; r++) {
+        const int i = tid + 512 * r;
+        const double f = P[r][kk];
+        if (i == k) {
+#pragma unroll
+          for (int j = 0; j < NB; j++) P[r][j] = j == kk ? inv : s_rp[q][j] * inv;
+        } else {
+          const double g = f * inv;
+#pragma unroll
+          for (int j = 0; j < NB; j++) P[r][j] = j == kk ? -g : P[r][j] - g * s_rp[q][j];
+        }
+      }
     }
   }
 #pragma unroll
@@ -1081,6 +1089,19 @@ extern "C" int gm_mg_copy_block_inverse(const gm_ctx *c, int level, int elem, do
   return (int)n;
 }
 
+extern "C" int gm_mg_copy_coarse_inverse(const gm_ctx *c, double *out) {
+  if (!c || !out || c->mg.empty()) return GM_ERR_ARG;
+  const MgLevel &L = c->mg.back();
+  if (!L.d_dense || L.dense_n != L.ctx->N) {
+    gm_set_error("the coarsest multigrid level has no dense inverse yet (no set-up with an "
+                 "exact coarse solve)");
+    return GM_ERR_ARG;
+  }
+  const size_t n = (size_t)L.dense_n;
+  GM_HIP(hipMemcpy(out, L.d_dense, n * n * sizeof(double), hipMemcpyDeviceToHost));
+  return (int)n;
+}
+
 extern "C" int gm_mg_add_level(gm_ctx *fine, gm_ctx *coarse, const int *P_rowptr, const int *P_col,
                                const double *P_val, const int *inject) {
   if (!fine || !coarse || !P_rowptr || !P_col || !P_val || !inject) return GM_ERR_ARG;
@@ -1243,8 +1264,10 @@ static int build_colour_rows(gm_ctx *lc) {
   return upv(&lc->d_mg_rows, rows.data(), rows.size());
 }
 
-// Dense inverse of the coarsest level's Jacobian (exact coarse solve).
-static int coarse_inverse(gm_ctx *c, MgLevel &L, int prec) {
+// Dense inverse of the coarsest level's Jacobian (exact coarse solve),
+// enqueued on stream s; coarse_inverse_finish waits for it and reports a
+// singular matrix.
+static int coarse_inverse(gm_ctx *c, MgLevel &L, int prec, hipStream_t s) {
   gm_ctx *lc = L.ctx;
   const int n = lc->N;
   if (n > GM_MG_DIRECT_MAX) {
@@ -1263,7 +1286,6 @@ static int coarse_inverse(gm_ctx *c, MgLevel &L, int prec) {
     GM_HIP(hipMalloc(&L.d_perm, (size_t)n * sizeof(int)));
     L.dense_n = n;
   }
-  hipStream_t s = c->stream;
   const long long nn = (long long)n * n;
   k_set<double><<<1024, 256, 0, s>>>(L.d_dense, nn, 0.0);
   if (prec == GM_PREC_MIXED)
@@ -1274,19 +1296,27 @@ static int coarse_inverse(gm_ctx *c, MgLevel &L, int prec) {
   const int nt = (n + 63) / 64;
   // panel width x rows per thread (512 threads; wider panels left the step
   // loop partly rolled, the panel in scratch)
-  const int pw = n <= 2048 ? 16 : 8;
+  const int pw = n <= 1536 ? 16 : 8;  // 16-wide panels of more rows would spill
   for (int kb0 = 0; kb0 < n; kb0 += pw) {
     const int nb = std::min(pw, n - kb0);
-    if (n <= 512) k_gjb_panel<16, 1><<<1, 512, 0, s>>>(n, kb0, nb, L.d_dense, L.d_perm, c->d_flag);
-    else if (n <= 1024) k_gjb_panel<16, 2><<<1, 512, 0, s>>>(n, kb0, nb, L.d_dense, L.d_perm, c->d_flag);
-    else if (n <= 1536) k_gjb_panel<16, 3><<<1, 512, 0, s>>>(n, kb0, nb, L.d_dense, L.d_perm, c->d_flag);
-    else if (n <= 2048) k_gjb_panel<16, 4><<<1, 512, 0, s>>>(n, kb0, nb, L.d_dense, L.d_perm, c->d_flag);
-    else k_gjb_panel<8, 8><<<1, 512, 0, s>>>(n, kb0, nb, L.d_dense, L.d_perm, c->d_flag);
-    k_gjb_swap<<<(n + 255) / 256, 256, 0, s>>>(n, kb0, nb, L.d_dense, L.d_perm, L.d_pcol);
-    k_gjb_update<<<dim3(nt, nt), 256, 0, s>>>(n, kb0, nb, L.d_dense, L.d_pcol);
+    // panel: read and written once; swap: the panel rows of the other columns
+    // read, written and copied; update: the matrix read and written
+    const double pb = 16.0 * n * nb, sb = 24.0 * n * nb, ub = 16.0 * (double)n * n;
+    auto panel = n <= 512 ? k_gjb_panel<16, 1> : n <= 1024 ? k_gjb_panel<16, 2>
+               : n <= 1536 ? k_gjb_panel<16, 3> : k_gjb_panel<8, 8>;
+    gm_launch(c, GM_K_COARSE_INV, pb, panel, dim3(1), dim3(512), 0, s, n, kb0, nb, L.d_dense,
+              L.d_perm, c->d_flag);
+    gm_launch(c, GM_K_COARSE_INV, sb, k_gjb_swap, dim3((n + 255) / 256), dim3(256), 0, s, n, kb0,
+              nb, L.d_dense, (const int *)L.d_perm, L.d_pcol);
+    gm_launch(c, GM_K_COARSE_INV, ub, k_gjb_update, dim3(nt, nt), dim3(256), 0, s, n, kb0, nb,
+              L.d_dense, (const double *)L.d_pcol);
   }
   k_gj_unpermute<<<(n + 255) / 256, 256, 0, s>>>(n, L.d_perm, L.d_dense);
   GM_HIP(hipGetLastError());
+  return GM_OK;
+}
+
+static int coarse_inverse_finish(gm_ctx *c, hipStream_t s) {
   int flag = 0;
   GM_HIP(hipMemcpyAsync(&flag, c->d_flag + 3, sizeof(int), hipMemcpyDeviceToHost, s));
   GM_HIP(hipStreamSynchronize(s));
@@ -1357,6 +1387,13 @@ int gm_mg_setup(gm_ctx *c, const double *x, const double *val, const float *val3
     xs = L.d_state;
     above = lc;
   }
+  // The coarsest level's dense inverse (one workgroup per panel step, latency
+  // bound) runs on that level's own stream, beside the element-block
+  // inversions on the main stream; its matrix is complete here (the Galerkin
+  // products were synchronised; a rediscretised matrix was filled on that
+  // same stream).
+  hipStream_t cs = c->mg.back().ctx->stream;
+  if (coarse_direct && (rc = coarse_inverse(c, c->mg.back(), prec, cs))) return rc;
   if (prec == GM_PREC_MIXED) {
     invert<float>(c, c, c->stream, val32, (float *)c->mg0.d_ainv);
     for (MgLevel &L : c->mg)
@@ -1367,7 +1404,8 @@ int gm_mg_setup(gm_ctx *c, const double *x, const double *val, const float *val3
       invert<double>(c, L.ctx, c->stream, (const double *)L.d_val, (double *)L.d_ainv);
   }
   GM_HIP(hipGetLastError());
-  if (coarse_direct && (rc = coarse_inverse(c, c->mg.back(), prec))) return rc;
+  // the V-cycles on the main stream read the dense inverse after this wait
+  if (coarse_direct && (rc = coarse_inverse_finish(c, cs))) return rc;
   return GM_OK;
 }
 

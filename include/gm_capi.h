@@ -353,6 +353,12 @@ int gm_mg_copy_block_inverse(const gm_ctx *ctx, int level, int elem, double *out
 /* Host copy (as FP64) of the Jacobian of multigrid level `level` (1 = first
  * coarse level) from the last set-up, in that level's MSR layout. */
 int gm_mg_copy_level_matrix(const gm_ctx *ctx, int level, double *val);
+/* Host copy of the dense inverse of the coarsest multigrid level's Jacobian
+ * from the last set-up with an exact coarse solve: row-major n x n, n = that
+ * level's unknowns (at most 4096).  Returns n.  No counterpart in the
+ * reference (Goma has no multigrid); it exposes the exact coarse solve for
+ * inspection, as gm_mg_copy_block_inverse does the smoother's blocks. */
+int gm_mg_copy_coarse_inverse(const gm_ctx *ctx, double *out);
 
 /* Kernel timing: when enabled, launches of the kernels below carry HIP
  * events that their own dispatch stamps (hipExtLaunchKernel), so a duration
@@ -381,7 +387,8 @@ int gm_mg_copy_level_matrix(const gm_ctx *ctx, int level, double *val);
 #define GM_K_BAND_SOLVE 18 /* k_band_fwd / k_band_bwd: one 64-row block of the triangular solves */
 #define GM_K_VANKA_GATHER 19 /* k_vanka_gather: additive smoother, element corrections summed per unknown */
 #define GM_K_ROW_SCALE 20 /* k_row_sum_scale: row sums and scaling of the Jacobian and residual */
-#define GM_K_COUNT 21
+#define GM_K_COARSE_INV 21 /* k_gjb_panel / k_gjb_swap / k_gjb_update: one panel of the coarsest level's dense inverse */
+#define GM_K_COUNT 22
 /* on = 0: off; 1: time every launch; k > 1: time a pseudo-random one launch
  * in k of each kernel (lower overhead), durations scaled to all launches by
  * the launch count. */

@@ -391,7 +391,12 @@ def headline(args, torch, capi):
     x_ref = x.clone()
     groups = field_groups(ctx, prob)
 
-    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    # The coarsest level's dense inverse runs on a second stream beside the
+    # element-block inversions: its dispatch-stamped durations include the time
+    # its single-workgroup launches wait for a CU, so its total is not time it
+    # adds to the solve, and it is not a candidate for the dominant kernel.
+    CONCURRENT = {"k_gjb(coarse inverse)"}
+    dom = max((k for k in kern if k not in CONCURRENT), key=lambda k: kern[k]["total_ms"])
     roof = kernel_roofline(dom, kern[dom], prob, 0, prob.name)
     # assembly: a Jacobian + residual assembly is one launch of k_fill per
     # Newton pass (per element colour if the kernel is coloured)
@@ -451,7 +456,9 @@ def headline(args, torch, capi):
         "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kern.items()},
         "kernel_timing": ("HIP events stamped by each launch's own dispatch, on the library "
-                          f"stream, over a repeat of the timed solve ({t_prof:.2f} s with events)"),
+                          f"stream, over a repeat of the timed solve ({t_prof:.2f} s with events); "
+                          "k_gjb(coarse inverse) runs on a second stream beside k_vanka_invert, so "
+                          "its durations include waiting for CUs and overlap other kernels"),
     }
     if args.variants:
         xm = torch.zeros_like(x)

@@ -756,6 +756,12 @@ static int upv(T **d, const T *h, size_t n) {
   return GM_OK;
 }
 
+
+// Largest number of nonzeros of one child's nodal interpolation W that the
+// Galerkin kernel keeps in LDS (a Q2 child of a Q2 parent: 125 in 3D, 25 in 2D)
+template <int DIM>
+constexpr int gal_ws_max() { return DIM == 2 ? 32 : 128; }
+
 // Galerkin coarse operators.  When every finer element lies inside one
 // coarser element (its parent) and P maps the parent's unknowns to the
 // element's unknowns, R A P = sum over finer elements of P_e^T K_e P_e, with K_e
@@ -819,6 +825,8 @@ static std::string build_galerkin_maps(const gm_ctx *f, const gm_ctx *c, const i
     for (int t = 0; t < nve * nve; t++) nz += W[(size_t)e * nve * nve + t] != 0.0;
     stride = std::max(stride, nz);
   }
+  if (stride > (f->dim == 2 ? gal_ws_max<2>() : gal_ws_max<3>()))
+    return "a child's nodal interpolation has more nonzeros than the Galerkin kernel keeps";
   std::vector<int> Wp((size_t)f->num_elems * (nve + 1)), Wi((size_t)f->num_elems * stride, 0);
   std::vector<double> Wv((size_t)f->num_elems * stride, 0.0);
   for (int e = 0; e < f->num_elems; e++) {
@@ -860,35 +868,59 @@ __global__ __launch_bounds__(256) void k_galerkin(const double *__restrict__ kf,
   constexpr int NMP = DIM * NVE * NP;          // momentum x P (= continuity x velocity) entries
   constexpr int NPC = 2 * NMP + NP * NP;       // pressure couplings, contiguous from off_mp
   constexpr int RP = (NPC + NT - 1) / NT;
-  // W by columns: s_wp[m] .. s_wp[m + 1] index the finer nodes s_wi (ascending)
-  // interpolating from coarse node m and their weights s_wv; only these terms
-  // of the dense sums are nonzero, taken in the same order
-  __shared__ CT sK[NVE2], sT[NVE2], sQ[NP * NP], sKp[NPC], s_wv[NVE2];
-  __shared__ int s_wp[NVE + 1], s_wi[NVE2];
+  constexpr int WSM = gal_ws_max<DIM>();
+  // W of every child by columns, loaded once: s_wp[c][m] .. s_wp[c][m + 1]
+  // index the finer nodes s_wi (ascending) interpolating from coarse node m
+  // and their weights s_wv; only these terms of the dense sums are nonzero,
+  // taken in the same order
+  __shared__ CT sK[NVE2], sT[NVE2], sQ[NP * NP], sKp[NPC], s_wv[NCH * WSM];
+  __shared__ int s_wp[NCH][NVE + 1], s_wi[NCH * WSM];
   const int ec = blockIdx.x, tid = threadIdx.x;
   const int *ch = child + (size_t)ec * NCH;
   double *__restrict__ out = kc + (size_t)ec * E;
-  auto load_w = [&](int f) {
-    if (tid <= NVE) s_wp[tid] = Wp[(size_t)f * (NVE + 1) + tid];
-    for (int t = tid; t < ws; t += NT) {
-      s_wi[t] = Wi[(size_t)f * ws + t];
-      s_wv[t] = (CT)Wv[(size_t)f * ws + t];
+  for (int t = tid; t < NCH * (NVE + 1); t += NT) {
+    const int c = t / (NVE + 1), m = t - c * (NVE + 1);
+    s_wp[c][m] = Wp[(size_t)ch[c] * (NVE + 1) + m];
+  }
+  for (int t = tid; t < NCH * WSM; t += NT) {
+    const int c = t / WSM, k = t - c * WSM;
+    if (k < ws) {
+      s_wi[t] = Wi[(size_t)ch[c] * ws + k];
+      s_wv[t] = (CT)Wv[(size_t)ch[c] * ws + k];
+    }
+  }
+  // The finer element matrices' blocks, (variable block, child) in turn: the
+  // next one is loaded into registers while the current one is reduced.
+  double kr[RN];
+  auto fetch = [&](int it) {
+    const int blk = it / NCH, f = ch[it - blk * NCH];
+    const double *Kf = kf + (size_t)f * E + (size_t)blk * NVE2;
+#pragma unroll
+    for (int r = 0; r < RN; r++) {
+      const int t = tid + r * NT;
+      kr[r] = t < NVE2 ? Kf[t] : 0.0;
     }
   };
+  fetch(0);
   for (int blk = 0; blk < NV * NV; blk++) {
     CT acc[RN];
 #pragma unroll
     for (int r = 0; r < RN; r++) acc[r] = 0.0;
     for (int sidx = 0; sidx < NCH; sidx++) {
-      const int f = ch[sidx];
-      const double *Kf = kf + (size_t)f * E + (size_t)blk * NVE2;
-      for (int t = tid; t < NVE2; t += NT) sK[t] = (CT)Kf[t];
-      load_w(f);
+#pragma unroll
+      for (int r = 0; r < RN; r++) {
+        const int t = tid + r * NT;
+        if (t < NVE2) sK[t] = (CT)kr[r];
+      }
+      const int it = blk * NCH + sidx;
+      if (it + 1 < NV * NV * NCH) fetch(it + 1);
+      const int *wp = s_wp[sidx], *wi = s_wi + sidx * WSM;
+      const CT *wv = s_wv + sidx * WSM;
       __syncthreads();
       for (int t = tid; t < NVE2; t += NT) {  // T = K W, column m of W
         const int m = t / NVE, i = t - m * NVE;
         CT v = 0;
-        for (int k = s_wp[m]; k < s_wp[m + 1]; k++) v += sK[i * NVE + s_wi[k]] * s_wv[k];
+        for (int k = wp[m]; k < wp[m + 1]; k++) v += sK[i * NVE + wi[k]] * wv[k];
         sT[i * NVE + m] = v;
       }
       __syncthreads();
@@ -898,7 +930,7 @@ __global__ __launch_bounds__(256) void k_galerkin(const double *__restrict__ kf,
         if (t < NVE2) {
           const int m = t / NVE, n = t - m * NVE;
           CT v = 0;
-          for (int k = s_wp[m]; k < s_wp[m + 1]; k++) v += s_wv[k] * sT[s_wi[k] * NVE + n];
+          for (int k = wp[m]; k < wp[m + 1]; k++) v += wv[k] * sT[wi[k] * NVE + n];
           acc[r] += v;
         }
       }
@@ -916,7 +948,8 @@ __global__ __launch_bounds__(256) void k_galerkin(const double *__restrict__ kf,
   for (int sidx = 0; sidx < NCH; sidx++) {
     const int f = ch[sidx];
     const double *Kf = kf + (size_t)f * E + off_mp;
-    load_w(f);
+    const int *wp = s_wp[sidx], *wi = s_wi + sidx * WSM;
+    const CT *wv = s_wv + sidx * WSM;
     if (tid < NP * NP) sQ[tid] = (CT)Q[(size_t)f * NP * NP + tid];
     for (int t = tid; t < NPC; t += NT) sKp[t] = (CT)Kf[t];
     __syncthreads();
@@ -927,12 +960,12 @@ __global__ __launch_bounds__(256) void k_galerkin(const double *__restrict__ kf,
       CT v = 0;
       if (t < NMP) {  // row (a, m) momentum, column l pressure
         const int a = t / (NVE * NP), rem = t - a * NVE * NP, m = rem / NP, l = rem - m * NP;
-        for (int k2 = s_wp[m]; k2 < s_wp[m + 1]; k2++) {
-          const int i = s_wi[k2];
+        for (int k2 = wp[m]; k2 < wp[m + 1]; k2++) {
+          const int i = wi[k2];
           CT q = 0;
 #pragma unroll
           for (int k = 0; k < NP; k++) q += sKp[a * NVE * NP + i * NP + k] * sQ[k * NP + l];
-          v += s_wv[k2] * q;
+          v += wv[k2] * q;
         }
       } else if (t < 2 * NMP) {  // row l continuity, column (b, n) velocity
         const int t2 = t - NMP, b = t2 / (NP * NVE), rem = t2 - b * NP * NVE;
@@ -940,8 +973,8 @@ __global__ __launch_bounds__(256) void k_galerkin(const double *__restrict__ kf,
 #pragma unroll
         for (int k = 0; k < NP; k++) {
           CT q = 0;
-          for (int k2 = s_wp[n]; k2 < s_wp[n + 1]; k2++)
-            q += sKp[NMP + b * NP * NVE + k * NVE + s_wi[k2]] * s_wv[k2];
+          for (int k2 = wp[n]; k2 < wp[n + 1]; k2++)
+            q += sKp[NMP + b * NP * NVE + k * NVE + wi[k2]] * wv[k2];
           v += sQ[k * NP + l] * q;
         }
       } else {  // P x P

@@ -15,8 +15,9 @@
 // its own L2.  Element kernels remap blockIdx.x so that every XCD works
 // through one contiguous range of an element list: neighbouring elements,
 // which share nodes, then share an L2 (measured: a few per cent on the
-// element-block smoother; neutral to slower on the row kernels, which keep
-// the plain order).  A bijection on [0, gridDim.x).
+// element-block smoother, about 1 % on the HEX27 node-row kernel; neutral to
+// slower on the per-row kernels, which keep the plain order).  A bijection on
+// [0, gridDim.x).
 __device__ inline int xcd_block() {
   const int nb = gridDim.x, b = blockIdx.x;
   const int q = nb >> 3, r = nb & 7, x = b & 7;
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(256) void k_msr_nodes(int num_nodes, int dim, int n
                                                    const XT *__restrict__ b, YT *__restrict__ y) {
   extern __shared__ int s_pat[];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int n = blockIdx.x * (blockDim.x >> 6) + w;
+  const int n = xcd_block() * (blockDim.x >> 6) + w;
   const bool valid = n < num_nodes;
   int *PA = s_pat + (size_t)w * 2 * cap, *PT = PA + cap;
   int f0 = 0, nu = 0, selfA = 0, selfT = 0;

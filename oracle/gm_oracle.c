@@ -36,7 +36,7 @@ struct om_problem {
   int *ija;              /* MSR */
   int n_qs;              /* QSIDE heat-flux sides: element, side (1-based), flux */
   int *qs_elem, *qs_side;
-  double *qs_q;
+  double *qs_q, *qs_h, *qs_tc; /* flux; QCONV heat transfer coefficient and bath temperature */
   double h_elem_avg; /* PSPG: global mean element size */
   double U_norm;     /* PSPG: global velocity norm (mean square) */
 };
@@ -325,10 +325,17 @@ void om_destroy(om_problem *p) {
   free(p->qs_elem);
   free(p->qs_side);
   free(p->qs_q);
+  free(p->qs_h);
+  free(p->qs_tc);
   free(p);
 }
 
-int om_add_qside(om_problem *p, int n, const int *elems, const int *sides, double q) {
+/* Side entries of the weak energy conditions: each adds q + h (T_c - T) to
+ * the inward heat flux over the side.  QSIDE: h = 0 (src/bc_integ.c:1518-1519);
+ * QCONV: q = 0 (src/bc_integ.c:1492-1494, qrad_surf with zero emissivity,
+ * src/mm_ns_bc.c:15145-15255: func += h (T_c - T), d func / d T_j = -h phi_j). */
+static int add_sides(om_problem *p, int n, const int *elems, const int *sides, double q, double h,
+                     double tc) {
   const int nsides = p->dim == 2 ? 4 : 6;
   if (!p->ph.energy || n < 0) return -1;
   for (int k = 0; k < n; k++)
@@ -340,16 +347,29 @@ int om_add_qside(om_problem *p, int n, const int *elems, const int *sides, doubl
   if (qd) p->qs_side = qd;
   double *qv = (double *)realloc(p->qs_q, (size_t)(m > 0 ? m : 1) * sizeof(double));
   if (qv) p->qs_q = qv;
-  if (!qe || !qd || !qv) return -1;
+  double *qh = (double *)realloc(p->qs_h, (size_t)(m > 0 ? m : 1) * sizeof(double));
+  if (qh) p->qs_h = qh;
+  double *qt = (double *)realloc(p->qs_tc, (size_t)(m > 0 ? m : 1) * sizeof(double));
+  if (qt) p->qs_tc = qt;
+  if (!qe || !qd || !qv || !qh || !qt) return -1;
   for (int k = 0; k < n; k++) {
     p->qs_elem[p->n_qs + k] = elems[k];
     p->qs_side[p->n_qs + k] = sides[k];
     p->qs_q[p->n_qs + k] = q;
+    p->qs_h[p->n_qs + k] = h;
+    p->qs_tc[p->n_qs + k] = tc;
   }
   p->n_qs = m;
   return 0;
 }
 
+int om_add_qside(om_problem *p, int n, const int *elems, const int *sides, double q) {
+  return add_sides(p, n, elems, sides, q, 0.0, 0.0);
+}
+
+int om_add_qconv(om_problem *p, int n, const int *elems, const int *sides, double h, double tc) {
+  return add_sides(p, n, elems, sides, 0.0, h, tc);
+}
 
 int om_num_unknowns(const om_problem *p) { return p->N; }
 int om_ija_size(const om_problem *p) { return p->ija[p->N]; }
@@ -376,11 +396,13 @@ typedef struct {
 
 #define DELTA(a, b) ((a) == (b) ? 1.0 : 0.0)
 
-/* QSIDE (src/bc_integ.c:1518-1519 func = q; 2156-2157, 2207: R[T][i] +=
- * w phi_i sdet q over the side).  Exodus side numbering, as Goma's
- * get_side_info (src/mm_fill_fill.c:3049-3072, 3182-3250): the reference
- * coordinate fixed on each side (axis 0 = s, 1 = t, 2 = u) and its value. */
-static void qside_elem(const om_problem *p, int e, double xc[3][MDE], lec_t *L) {
+/* QSIDE and QCONV (src/bc_integ.c:1492-1494, 1518-1519 func = q + h (T_c - T);
+ * 2156-2157, 2207: R[T][i] += w phi_i sdet func over the side, J[T][T][i][j]
+ * += w phi_i sdet d func / d T_j = -w phi_i sdet h phi_j).  Exodus side
+ * numbering, as Goma's get_side_info (src/mm_fill_fill.c:3049-3072,
+ * 3182-3250): the reference coordinate fixed on each side (axis 0 = s, 1 = t,
+ * 2 = u) and its value. */
+static void qside_elem(const om_problem *p, int e, double xc[3][MDE], const double *Tn, lec_t *L) {
   static const int side_axis[6] = {1, 0, 1, 0, 2, 2};
   static const double side_val[6] = {-1.0, 1.0, 1.0, -1.0, -1.0, 1.0};
   const double gp[3] = {-0.7745966692414834, 0.0, 0.7745966692414834};
@@ -419,7 +441,15 @@ static void qside_elem(const om_problem *p, int e, double xc[3][MDE], lec_t *L) 
                  c2 = t1[0] * t2[1] - t1[1] * t2[0];
           sdet = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
         }
-        for (int i = 0; i < nve; i++) L->R[SLOT_T][i] += w * phi[i] * sdet * p->qs_q[k];
+        const double h = p->qs_h[k];
+        double Tg = 0.0;
+        for (int j = 0; j < nve; j++) Tg += phi[j] * Tn[j];
+        const double func = p->qs_q[k] + h * (p->qs_tc[k] - Tg);
+        for (int i = 0; i < nve; i++) {
+          L->R[SLOT_T][i] += w * phi[i] * sdet * func;
+          if (h != 0.0)
+            for (int j = 0; j < nve; j++) L->J[SLOT_T][SLOT_T][i][j] -= w * phi[i] * sdet * h * phi[j];
+        }
       }
   }
 }
@@ -882,7 +912,7 @@ static int elem_fill(const om_problem *p, int e, const double *x, const double *
     }
   }
 
-  if (p->n_qs && en) qside_elem(p, e, xc, L);
+  if (p->n_qs && en) qside_elem(p, e, xc, Tn, L);
 
   /* put_dirichlet_in_matrix (src/bc_dirich.c:44-153): zero the local row,
    * J_ii = DIRICHLET_PENALTY (1, include/rf_bc_const.h:152), R = x - V. */

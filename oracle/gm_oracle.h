@@ -143,6 +143,8 @@ int om_fill_transient(const om_problem *p, const double *x, const double *xdot, 
  * append.  Returns 0, or -1 without the energy equation or for an element or
  * side out of range. */
 int om_add_qside(om_problem *p, int n, const int *elems, const int *sides, double q);
+/* QCONV: heat transfer coefficient h and bath temperature T_c on the given sides */
+int om_add_qconv(om_problem *p, int n, const int *elems, const int *sides, double h, double tc);
 /* One element's dense local residual and Jacobian in the oracle's local order:
  * local dofs = [var slot][dof], var slots U1,U2,(U3),(T),P.  R[nloc], J[nloc*nloc]. */
 int om_element(const om_problem *p, int elem, const double *x, double *R, double *J, int *nloc);

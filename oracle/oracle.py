@@ -89,6 +89,7 @@ def lib():
         L.om_fill_range.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_int]
         L.om_fill_transient.argtypes = [C.c_void_p, _dp, _dp, C.c_double, _dp, _dp]
         L.om_add_qside.argtypes = [C.c_void_p, C.c_int, _ip, _ip, C.c_double]
+        L.om_add_qconv.argtypes = [C.c_void_p, C.c_int, _ip, _ip, C.c_double, C.c_double]
         L.om_element.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_int)]
         L.om_row_sum_scale.argtypes = [C.c_int, _dp, _ip, _dp, _dp]
         for nm in ("om_norm_L2", "om_norm_L1", "om_norm_Loo"):
@@ -195,6 +196,16 @@ class Oracle:
         if e.shape != s.shape:
             raise ValueError("elems and sides differ in length")
         return lib().om_add_qside(self.h, int(e.size), e, s, float(q))
+
+    def add_qconv(self, elems, sides, h, t_bath):
+        """QCONV convective heat transfer h (T_bath - T) into the domain on
+        element sides (Exodus numbering, 1-based).  Returns the library's
+        status (0 or -1)."""
+        e = np.ascontiguousarray(elems, np.int32)
+        s = np.ascontiguousarray(sides, np.int32)
+        if e.shape != s.shape:
+            raise ValueError("elems and sides differ in length")
+        return lib().om_add_qconv(self.h, int(e.size), e, s, float(h), float(t_bath))
 
     def fill_transient(self, x, xdot, cmass):
         """matrix_fill_full of a theta-method step: the steady terms plus the

@@ -137,7 +137,7 @@ EXPORTS = [
     "gm_mlp_set_params", "gm_mlp_default_train_opts", "gm_mlp_set_data", "gm_mlp_train",
     "gm_mlp_loss_grad", "gm_mlp_predict_dev", "gm_mlp_predict", "gm_mlp_guess_dev",
     "gm_mlp_set_basis", "gm_mlp_clear_basis", "gm_matrix_fill_full_transient_dev",
-    "gm_add_qside", "gm_clear_qside", "gm_solve_nonlinear_transient_dev", "gm_solve_transient_dev",
+    "gm_add_qside", "gm_add_qconv", "gm_clear_qside", "gm_solve_nonlinear_transient_dev", "gm_solve_transient_dev",
     "gm_num_nodes", "gm_num_elems", "gm_mg_build", "gm_mg_level", "gm_mg_copy_prolongation",
     "gm_matrix_fill_full_assign_dev", "gm_mlp_predict_noalloc_dev", "gm_mg_copy_block_inverse",
     "gm_mg_copy_coarse_inverse",
@@ -202,6 +202,7 @@ def lib():
                                                C.c_double, C.POINTER(GmNewtonReport)]
     L.gm_matrix_fill_full_transient_dev.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double, vp, vp]
     L.gm_add_qside.argtypes = [vp, ip, vp, vp, C.c_double]
+    L.gm_add_qconv.argtypes = [vp, ip, vp, vp, C.c_double, C.c_double]
     L.gm_clear_qside.argtypes = [vp]
     L.gm_solve_nonlinear_transient_dev.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double,
                                                    C.POINTER(GmSolverOpts),
@@ -377,6 +378,15 @@ class Context:
         assert e.shape == sd.shape
         _check(lib().gm_add_qside(self.h, len(e), e.ctypes.data, sd.ctypes.data, float(q)),
                "gm_add_qside")
+
+    def add_qconv(self, elems, sides, h: float, t_bath: float):
+        """QCONV convective heat transfer h (t_bath - T) on the given (element,
+        1-based side) pairs."""
+        e = np.ascontiguousarray(elems, dtype=np.int32)
+        sd = np.ascontiguousarray(sides, dtype=np.int32)
+        assert e.shape == sd.shape
+        _check(lib().gm_add_qconv(self.h, len(e), e.ctypes.data, sd.ctypes.data, float(h),
+                                  float(t_bath)), "gm_add_qconv")
 
     def clear_qside(self):
         _check(lib().gm_clear_qside(self.h), "gm_clear_qside")

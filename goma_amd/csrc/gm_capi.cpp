@@ -385,8 +385,8 @@ extern "C" int gm_matrix_fill_full_transient_dev(gm_ctx *c, const double *x, con
   return rc;
 }
 
-// Groups the QSIDE entries per element (card order kept within an element)
-// and uploads them.
+// Groups the QSIDE / QCONV entries per element (card order kept within an
+// element) and uploads them, each with its (q, h, T_c).
 static int upload_qside(gm_ctx *c) {
   if (c->d_qs_ptr) hipFree(c->d_qs_ptr);
   if (c->d_qs_side) hipFree(c->d_qs_side);
@@ -396,36 +396,40 @@ static int upload_qside(gm_ctx *c) {
   const size_t n = c->qs_elem.size();
   if (n == 0) return GM_OK;
   std::vector<int> ptr(c->num_elems + 1, 0), side(n);
-  std::vector<double> q(n);
+  std::vector<double> q(3 * n);
   for (int e : c->qs_elem) ptr[e + 1]++;
   for (int e = 0; e < c->num_elems; e++) ptr[e + 1] += ptr[e];
   std::vector<int> pos(ptr.begin(), ptr.end() - 1);
   for (size_t k = 0; k < n; k++) {
     const int at = pos[c->qs_elem[k]]++;
     side[at] = c->qs_side[k];
-    q[at] = c->qs_q[k];
+    q[3 * at] = c->qs_q[k];
+    q[3 * at + 1] = c->qs_h[k];
+    q[3 * at + 2] = c->qs_tc[k];
   }
   int rc;
   if ((rc = upload(&c->d_qs_ptr, ptr.data(), ptr.size()))) return rc;
   if ((rc = upload(&c->d_qs_side, side.data(), n))) return rc;
-  return upload(&c->d_qs_q, q.data(), n);
+  return upload(&c->d_qs_q, q.data(), 3 * n);
 }
 
-extern "C" int gm_add_qside(gm_ctx *c, int n, const int *elems, const int *sides, double q) {
-  if (!c || n < 0 || (n > 0 && (!elems || !sides)) || !std::isfinite(q)) return GM_ERR_ARG;
+// Side entries adding q + h (T_c - T) to the inward heat flux: QSIDE (h = 0)
+// and QCONV (q = 0).
+static int add_sides(gm_ctx *c, const char *card, int n, const int *elems, const int *sides,
+                     double q, double h, double tc) {
   if (!c->energy) {
-    gm_set_error("QSIDE needs the energy equation");
+    gm_set_error("%s needs the energy equation", card);
     return GM_ERR_ARG;
   }
   std::vector<int> count(c->num_elems, 0);
   for (int e : c->qs_elem) count[e]++;
   for (int k = 0; k < n; k++) {
     if (elems[k] < 0 || elems[k] >= c->num_elems || sides[k] < 1 || sides[k] > 2 * c->dim) {
-      gm_set_error("QSIDE entry %d: element %d side %d outside the mesh", k, elems[k], sides[k]);
+      gm_set_error("%s entry %d: element %d side %d outside the mesh", card, k, elems[k], sides[k]);
       return GM_ERR_ARG;
     }
     if (++count[elems[k]] > GM_QS_MAX) {
-      gm_set_error("QSIDE: more than %d side entries on element %d", GM_QS_MAX, elems[k]);
+      gm_set_error("%s: more than %d side entries on element %d", card, GM_QS_MAX, elems[k]);
       return GM_ERR_ARG;
     }
   }
@@ -433,8 +437,22 @@ extern "C" int gm_add_qside(gm_ctx *c, int n, const int *elems, const int *sides
     c->qs_elem.push_back(elems[k]);
     c->qs_side.push_back(sides[k]);
     c->qs_q.push_back(q);
+    c->qs_h.push_back(h);
+    c->qs_tc.push_back(tc);
   }
   return upload_qside(c);
+}
+
+extern "C" int gm_add_qside(gm_ctx *c, int n, const int *elems, const int *sides, double q) {
+  if (!c || n < 0 || (n > 0 && (!elems || !sides)) || !std::isfinite(q)) return GM_ERR_ARG;
+  return add_sides(c, "QSIDE", n, elems, sides, q, 0.0, 0.0);
+}
+
+extern "C" int gm_add_qconv(gm_ctx *c, int n, const int *elems, const int *sides, double h,
+                            double t_bath) {
+  if (!c || n < 0 || (n > 0 && (!elems || !sides)) || !std::isfinite(h) || !std::isfinite(t_bath))
+    return GM_ERR_ARG;
+  return add_sides(c, "QCONV", n, elems, sides, 0.0, h, t_bath);
 }
 
 extern "C" int gm_clear_qside(gm_ctx *c) {
@@ -442,6 +460,8 @@ extern "C" int gm_clear_qside(gm_ctx *c) {
   c->qs_elem.clear();
   c->qs_side.clear();
   c->qs_q.clear();
+  c->qs_h.clear();
+  c->qs_tc.clear();
   return upload_qside(c);
 }
 

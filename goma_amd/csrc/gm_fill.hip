@@ -65,8 +65,9 @@ struct FillArgs {
   // xdot = cmass (x - x_old) - cold xdot_old, cmass = (1 + 2 theta) / dt, cold = 2 theta
   const double *x_old, *xdot_old;
   double cmass, cold;
-  // QSIDE weak heat flux: the side entries of element e are [qs_ptr[e], qs_ptr[e+1]),
-  // each a side number (1-based, Exodus order) and a flux; nullptr qs_ptr: none
+  // QSIDE / QCONV side entries: those of element e are [qs_ptr[e], qs_ptr[e+1]),
+  // each a side number (1-based, Exodus order) and (q, h, T_c) at qs_q[3 k]:
+  // inward heat flux q + h (T_c - T); nullptr qs_ptr: none
   const int *qs_ptr, *qs_side;
   const double *qs_q;
   const double *stab;  // side tables: [side][point] phi[NVE], dphidxi[NVE][DIM], weight
@@ -187,8 +188,9 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
   __shared__ double s_ps[NQ][DIM];       // PSPG: tau dA (rho v_dot + rho v.grad v + grad P - f)
   __shared__ double s_su[4];             // energy SUPG: s/h, s d(1/h)/d vc_b
   constexpr int NSP = STab<DIM>::NSP, SREC = STab<DIM>::REC;
-  __shared__ double s_qw[GM_QS_MAX][NSP];  // QSIDE: weight * surface det * flux per point
-  __shared__ int s_qsd[GM_QS_MAX];         // QSIDE: side index (0-based) of each entry
+  __shared__ double s_qw[GM_QS_MAX][NSP];  // QSIDE / QCONV: w |surface det| (q + h (T_c - T))
+  __shared__ double s_qh[GM_QS_MAX][NSP];  // QCONV: w |surface det| h (the Jacobian's weight)
+  __shared__ int s_qsd[GM_QS_MAX];         // side index (0-based) of each entry
   const int qs0 = A.qs_ptr ? A.qs_ptr[e] : 0;
   const int nqs = A.qs_ptr ? A.qs_ptr[e + 1] - qs0 : 0;
 
@@ -397,7 +399,12 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
                    c2 = t1[0] * t2[1] - t1[1] * t2[0];
       sdet = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
     }
-    s_qw[k][g] = st[NVE + NVE * DIM] * sdet * A.qs_q[qs0 + k];
+    const double *qk = A.qs_q + 3 * (qs0 + k), wd = st[NVE + NVE * DIM] * sdet;
+    double Tg = 0.0;
+    if constexpr (EN)
+      for (int j = 0; j < NVE; j++) Tg += st[j] * s_u[DIM][j];
+    s_qw[k][g] = wd * (qk[0] + qk[1] * (qk[2] - Tg));
+    s_qh[k][g] = wd * qk[1];
     if (g == 0) s_qsd[k] = sd;
   }
   __syncthreads();
@@ -551,6 +558,33 @@ __global__ __launch_bounds__(NT) void k_fill(FillArgs A, const int *__restrict__
         }
         const int idx = map[(a * NV + b) * NVE2 + t];
         out[idx] += (JT)v;
+      }
+    }
+  }
+
+  // QCONV: -h w |surface Jacobian| phi_i phi_j over the element's sides in the
+  // T-T block, on top of the entries scattered above
+  if constexpr (EN) {
+    bool conv = false;
+    for (int k = 0; k < nqs; k++) conv |= A.qs_q[3 * (qs0 + k) + 1] != 0.0;
+    if (conv) {
+      __syncthreads();
+      for (int t = tid; t < NVE2; t += NT) {
+        const int i = t / NVE, j = t - i * NVE;
+        double v = 0.0;
+#pragma unroll 1
+        for (int k = 0; k < nqs; k++) {
+          if (A.qs_q[3 * (qs0 + k) + 1] == 0.0) continue;
+#pragma unroll 1
+          for (int g = 0; g < NSP; g++) {
+            const double *st = A.stab + (size_t)(s_qsd[k] * NSP + g) * SREC;
+            v -= s_qh[k][g] * st[i] * st[j];
+          }
+        }
+        if (v == 0.0) continue;  // a node pair off the sides
+        if (A.kel) A.kel[(size_t)e * A.E + (size_t)(DIM * NV + DIM) * NVE2 + t] += v;
+        if (A.dbc[s_gun[DIM][i]]) continue;  // a Dirichlet row keeps its unit diagonal only
+        out[map[(DIM * NV + DIM) * NVE2 + t]] += (JT)v;
       }
     }
   }
@@ -715,7 +749,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VISC ? 3 : 
   __shared__ double s_P[KP], s_mu[KP], s_dmu[KP], s_div[KP], s_Td[KP];
   __shared__ double s_f[KP][DIM], s_vd[KP][DIM];
   __shared__ double s_su[4];  // energy SUPG: s/h, s d(1/h)/d vc_b
-  __shared__ double s_qw[GM_QS_MAX][NSP];
+  __shared__ double s_qw[GM_QS_MAX][NSP], s_qh[GM_QS_MAX][NSP];
   __shared__ int s_qsd[GM_QS_MAX];
   const int qs0 = A.qs_ptr ? A.qs_ptr[e] : 0;
   const int nqs = A.qs_ptr ? A.qs_ptr[e + 1] - qs0 : 0;
@@ -809,7 +843,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VISC ? 3 : 
     }
     const double c0 = t1[1] * tt2[2] - t1[2] * tt2[1], c1 = t1[2] * tt2[0] - t1[0] * tt2[2],
                  c2 = t1[0] * tt2[1] - t1[1] * tt2[0];
-    s_qw[k][g] = st[NVE + NVE * DIM] * sqrt(c0 * c0 + c1 * c1 + c2 * c2) * A.qs_q[qs0 + k];
+    const double *qk = A.qs_q + 3 * (qs0 + k);
+    const double wd = st[NVE + NVE * DIM] * sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    double Tg = 0.0;
+    if constexpr (EN)
+      for (int j = 0; j < NVE; j++) Tg += st[j] * s_u[DIM][j];
+    s_qw[k][g] = wd * (qk[0] + qk[1] * (qk[2] - Tg));
+    s_qh[k][g] = wd * qk[1];
     if (g == 0) s_qsd[k] = sd;
   }
   __syncthreads();
@@ -1182,6 +1222,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VISC ? 3 : 
       const int row = s_gun[DIM][i], col = s_gun[b][NVE - 1];
       const long long pos = s_brow[DIM][i] + (int)rk[NVE2 + ij] + b - (col > row ? 1 : 0);
       out[pos] += (JT)v;
+    }
+  }
+
+  // QCONV: -h w |surface Jacobian| phi_i phi_j over the element's sides in the
+  // T-T block, on top of the entries scattered above
+  if (EN && nqs > 0) {
+    bool conv = false;
+    for (int k = 0; k < nqs; k++) conv |= A.qs_q[3 * (qs0 + k) + 1] != 0.0;
+    if (conv) {
+      __syncthreads();
+      for (int t = tid; t < NVE2; t += NT) {
+        const int i = t / NVE, j = t - i * NVE;
+        double v = 0.0;
+#pragma unroll 1
+        for (int k = 0; k < nqs; k++) {
+          if (A.qs_q[3 * (qs0 + k) + 1] == 0.0) continue;
+#pragma unroll 1
+          for (int g = 0; g < NSP; g++) {
+            const double *st = A.stab + (size_t)(s_qsd[k] * NSP + g) * SREC;
+            v -= s_qh[k][g] * st[i] * st[j];
+          }
+        }
+        if (v == 0.0) continue;  // a node pair off the sides
+        if (A.kel) A.kel[(size_t)e * A.E + (size_t)(DIM * NV + DIM) * NVE2 + t] += v;
+        if (s_dir[DIM][i]) continue;  // a Dirichlet row keeps its unit diagonal only
+        const int row = s_gun[DIM][i], col = s_gun[DIM][j];
+        const long long pos =
+            row == col ? row : s_brow[DIM][i] + (int)rk[NVE2 + t] + DIM - (col > row ? 1 : 0);
+        out[pos] += (JT)v;
+      }
     }
   }
 

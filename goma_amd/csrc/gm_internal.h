@@ -9,7 +9,7 @@
 
 #include "../../include/gm_capi.h"
 
-#define GM_QS_MAX 8 /* QSIDE side entries per element */
+#define GM_QS_MAX 5 /* QSIDE / QCONV side entries per element */
 #define GM_MAXROW 1024 /* longest Jacobian row (HEX27 vertex row: 125 nodes x 4 + 8 x 4 P = 532) */
 
 void gm_set_error(const char *fmt, ...);
@@ -168,9 +168,9 @@ struct gm_ctx {
   // QSIDE weak heat-flux entries (element, 1-based side, flux), host copy in
   // card order; on the device grouped per element: d_qs_ptr [num_elems+1]
   std::vector<int> qs_elem, qs_side;
-  std::vector<double> qs_q;
+  std::vector<double> qs_q, qs_h, qs_tc;  // flux; QCONV coefficient and bath temperature
   int *d_qs_ptr = nullptr, *d_qs_side = nullptr;
-  double *d_qs_q = nullptr;
+  double *d_qs_q = nullptr;  // (q, h, T_c) per entry
 
   // Newton work vectors
   double *d_val = nullptr, *d_resid = nullptr, *d_dx = nullptr, *d_scale = nullptr;

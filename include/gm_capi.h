@@ -233,7 +233,14 @@ int gm_matrix_fill_full_transient_dev(gm_ctx *ctx, const double *x_dev, const do
  * term; Dirichlet rows keep x - V).  Entries accumulate over calls, at most
  * 8 per element.  Needs the energy equation. */
 int gm_add_qside(gm_ctx *ctx, int n, const int *elems, const int *sides, double q);
-/* Drops every QSIDE entry. */
+/* QCONV on element sides (same numbering and limits as gm_add_qside): adds
+ * h (t_bath - T) to the inward heat flux, i.e. the side integral of
+ * phi_i h (t_bath - T) to the energy residual and -h times the side integral
+ * of phi_i phi_j to its temperature Jacobian.  Replaces the QCONV_BC case of
+ * apply_integrated_bc (src/bc_integ.c:1492-1494: qrad_surf with zero
+ * emissivity, src/mm_ns_bc.c:15145-15255).  Cleared by gm_clear_qside. */
+int gm_add_qconv(gm_ctx *ctx, int n, const int *elems, const int *sides, double h, double t_bath);
+/* Drops every QSIDE and QCONV entry. */
 int gm_clear_qside(gm_ctx *ctx);
 
 /* row_sum_scale_MSR (src/sl_matrix_util.c:506): scales val and b in place. */

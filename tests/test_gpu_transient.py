@@ -229,7 +229,7 @@ def test_qside_refuses_bad_input(capi):
     with pytest.raises(capi.GomaError):
         ctx.add_qside([prob.mesh.num_elems], [1], 1.0)
     with pytest.raises(capi.GomaError):
-        ctx.add_qside([0] * 9, [1] * 9, 1.0)  # more than 8 entries on one element
+        ctx.add_qside([0] * 9, [1] * 9, 1.0)  # more side entries on one element than allowed
     ctx.close()
     iso = capi.Context(problems.lid_cavity(n=2, Re=100.0))  # no energy equation
     with pytest.raises(capi.GomaError):

@@ -673,18 +673,19 @@ __global__ __launch_bounds__(NT) void k_vanka_corr(LevelDev L, int nelem, const 
 }
 
 // Second half: x_u += omega / (elements sharing u) * sum of u's corrections, in
-// a fixed order (deterministic).
+// a fixed order (deterministic); assign: x_u = that (x is zero on entry).
 template <typename BT>
 __global__ __launch_bounds__(256) void k_vanka_gather(int N, const int *__restrict__ gptr,
                                                       const int *__restrict__ gent,
                                                       const double *__restrict__ corr,
                                                       const double *__restrict__ wmul, double omega,
-                                                      BT *__restrict__ x) {
+                                                      BT *__restrict__ x, int assign) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= N) return;
   double s = 0.0;
   for (int k = gptr[u]; k < gptr[u + 1]; k++) s += corr[gent[k]];
-  x[u] += (BT)(omega * wmul[u] * s);
+  const BT d = (BT)(omega * wmul[u] * s);
+  x[u] = assign ? d : x[u] + d;
 }
 
 // y = M v (CSR), optionally scaling v by sc and masking Dirichlet rows; G
@@ -1523,7 +1524,8 @@ static void vanka_colour(gm_ctx *lc, hipStream_t s, int col, const AT *ainv, con
 }
 
 template <int DIM, int EN, typename AT, typename BT>
-static void vanka_additive(gm_ctx *lc, hipStream_t s, const AT *ainv, const BT *r, BT *x, double omega) {
+static void vanka_additive(gm_ctx *lc, hipStream_t s, const AT *ainv, const BT *r, BT *x, double omega,
+                           bool assign) {
   constexpr int NT = DIM == 2 ? 64 : 128;
   constexpr int n = Loc<DIM, EN>::N;
   constexpr int EPB = (2 * n <= NT) ? 2 : 1;
@@ -1537,29 +1539,31 @@ static void vanka_additive(gm_ctx *lc, hipStream_t s, const AT *ainv, const BT *
   const double gbytes = (double)ne * n * 12.0 + 4.0 * (lc->N + 1) + lc->N * (8.0 + 2.0 * sizeof(BT));
   gm_launch(g_prof_ctx, GM_K_VANKA_GATHER, gbytes, k_vanka_gather<BT>, dim3((lc->N + 255) / 256),
             dim3(256), 0, s, lc->N, (const int *)lc->d_mg_gptr, (const int *)lc->d_mg_gent,
-            (const double *)lc->d_mg_corr, (const double *)lc->d_mg_wmul, omega, x);
+            (const double *)lc->d_mg_corr, (const double *)lc->d_mg_wmul, omega, x, assign ? 1 : 0);
 }
 
 // Multiplicative: coloured block Gauss-Seidel, the residual recomputed on each
 // colour's rows.  Additive: one residual per sweep, every element's correction
 // from it (one launch), then each unknown sums its corrections weighted by
-// 1 / elements sharing it (a second launch; no colours needed).  x_zero: x is
-// zero on entry (every level's pre-smoothing in the V-cycle), so the first
-// additive sweep's residual is b itself.
+// 1 / elements sharing it (a second launch; no colours needed).  x_zero:
+// smoothing from x = 0 (every level's pre-smoothing in the V-cycle): the first
+// additive sweep takes b as its residual and assigns its corrections to x,
+// whose entries it never reads (the multiplicative smoother gets x cleared).
 template <typename VT, typename BT>
 static void smooth(gm_ctx *lc, hipStream_t s, const VT *a, const VT *ainv, const BT *b, BT *x, BT *r,
                    int sweeps, double omega, int additive, bool x_zero) {
   for (int it = 0; it < sweeps; it++) {
     if (additive) {
       const BT *rr = r;
-      if (it == 0 && x_zero) rr = b;
+      const bool first = it == 0 && x_zero;  // x = 0: residual b, and x is assigned
+      if (first) rr = b;
       else resid<VT, BT>(lc, s, a, x, b, r);
       if (lc->dim == 2) {
-        if (lc->energy) vanka_additive<2, 1, VT, BT>(lc, s, ainv, rr, x, omega);
-        else vanka_additive<2, 0, VT, BT>(lc, s, ainv, rr, x, omega);
+        if (lc->energy) vanka_additive<2, 1, VT, BT>(lc, s, ainv, rr, x, omega, first);
+        else vanka_additive<2, 0, VT, BT>(lc, s, ainv, rr, x, omega, first);
       } else {
-        if (lc->energy) vanka_additive<3, 1, VT, BT>(lc, s, ainv, rr, x, omega);
-        else vanka_additive<3, 0, VT, BT>(lc, s, ainv, rr, x, omega);
+        if (lc->energy) vanka_additive<3, 1, VT, BT>(lc, s, ainv, rr, x, omega, first);
+        else vanka_additive<3, 0, VT, BT>(lc, s, ainv, rr, x, omega, first);
       }
       continue;
     }
@@ -1620,7 +1624,8 @@ static void vcycle(gm_ctx *c, const VT *a0, const BT *v, BT *z, const gm_solver_
                 (const double *)L.d_dense, b, x);
       break;
     }
-    k_set<BT><<<1024, 256, 0, s>>>(x, lc->N, (BT)0);
+    // pre-smoothing from x = 0 (the additive smoother's first sweep assigns x)
+    if (o->mg_smoother != 1) k_set<BT><<<1024, 256, 0, s>>>(x, lc->N, (BT)0);
     smooth<VT, BT>(lc, s, amat(l), (const VT *)L.d_ainv, b, x, (BT *)L.d_r,
                    l == nl ? (cs > 0 ? cs : 20) : nu, om, o->mg_smoother == 1, true);
     if (l == nl) break;

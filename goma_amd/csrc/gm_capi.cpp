@@ -117,6 +117,9 @@ static void free_ctx(gm_ctx *c) {
     for (hipEvent_t e : c->ev_b[i]) hipEventDestroy(e);
     for (hipEvent_t e : c->ev_e[i]) hipEventDestroy(e);
   }
+  for (hipEvent_t e : c->ev_inv)
+    if (e) hipEventDestroy(e);
+  if (c->s_inv && c->s_inv != c->stream) hipStreamDestroy(c->s_inv);
   if (c->stream) hipStreamDestroy(c->stream);
   // the level contexts gm_mg_build made, after this context's levels were released
   for (gm_ctx *l : c->owned_levels) free_ctx(l);

@@ -143,6 +143,10 @@ struct gm_ctx {
   ElemLayout lay{};
 
   hipStream_t stream = nullptr;
+  // multigrid set-up: the element-block inversions' stream (kept off a few
+  // CUs; equal to stream when that mask is unavailable) and its two events
+  hipStream_t s_inv = nullptr;
+  hipEvent_t ev_inv[2] = {nullptr, nullptr};
   double *d_coor = nullptr;
   int *d_conn = nullptr;
   int *d_fu = nullptr;

@@ -1361,6 +1361,43 @@ static int coarse_inverse_finish(gm_ctx *c, hipStream_t s) {
   return GM_OK;
 }
 
+// Stream of the overlapped element-block inversions, kept off 1/8 of the CUs.
+// The inversions fill every CU they may use (three workgroups of 147-VGPR
+// waves each), so a workgroup of another stream is placed only where they
+// leave room; the coarsest dense inverse is a chain of small launches, each
+// of which waited on a full GPU for the inversions to drain.  The spared CUs
+// keep that chain moving beside them.  Without the mask the main stream is
+// used and the set-up runs in order.
+#ifndef GM_INV_SPARE_DIV
+#define GM_INV_SPARE_DIV 8  // CUs spared: 1 in this many (1/16 and 1/12 measured slower)
+#endif
+static int inv_stream(gm_ctx *c, hipStream_t *out) {
+  if (!c->s_inv) {
+    int dev = 0, ncu = 0;
+    GM_HIP(hipGetDevice(&dev));
+    GM_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    hipStream_t s = nullptr;
+    if (ncu >= 64) {
+      std::vector<uint32_t> mask((size_t)(ncu + 31) / 32, 0u);
+      const int keep = ncu - ncu / GM_INV_SPARE_DIV;
+      for (int i = 0; i < keep; i++) mask[i / 32] |= 1u << (i % 32);
+      if (hipExtStreamCreateWithCUMask(&s, (uint32_t)mask.size(), mask.data()) != hipSuccess) {
+        (void)hipGetLastError();
+        s = nullptr;
+      }
+    }
+    if (s && (hipEventCreateWithFlags(&c->ev_inv[0], hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&c->ev_inv[1], hipEventDisableTiming) != hipSuccess)) {
+      (void)hipGetLastError();
+      (void)hipStreamDestroy(s);
+      s = nullptr;
+    }
+    c->s_inv = s ? s : c->stream;
+  }
+  *out = c->s_inv;
+  return GM_OK;
+}
+
 // Per Newton step: coarse Jacobians at the injected state, local inverses on
 // every level, and the dense inverse of the coarsest level when it is solved
 // exactly (coarse_direct).  val / val32 is the level-0 (scaled) matrix, x the
@@ -1384,16 +1421,37 @@ int gm_mg_setup(gm_ctx *c, const double *x, const double *val, const float *val3
     return GM_ERR_ARG;
   }
   // The level-0 block inverses need only the level-0 matrix: enqueued first on
-  // the main stream, they run beside the coarse levels' Galerkin products and
-  // scatters, which use each coarse level's own stream and synchronise only
-  // that stream.  With kernel timing on, the set-up runs in order instead, so
-  // that each launch's measured duration is its own.
-  const bool overlap = !c->prof;
+  // their own stream (after the main stream's assembly), they run beside the
+  // coarse levels' Galerkin products and scatters, which use each coarse
+  // level's own stream and synchronise only that stream, and beside the
+  // coarsest dense inverse.  With kernel timing on, or without that stream,
+  // the set-up runs in order on the main stream instead, so that each
+  // launch's measured duration is its own.
+  hipStream_t si = c->stream;
+  if (!c->prof && (rc = inv_stream(c, &si))) return rc;
+  const bool overlap = si != c->stream;
+  if (overlap) {
+    GM_HIP(hipEventRecord(c->ev_inv[0], c->stream));
+    GM_HIP(hipStreamWaitEvent(si, c->ev_inv[0], 0));
+  }
+  // on every return from here, what the main stream queues next (the V-cycles,
+  // or the next assembly over the matrix being inverted) waits for the
+  // inversions
+  struct Join {
+    gm_ctx *c;
+    hipStream_t si;
+    ~Join() {
+      if (si == c->stream) return;
+      if (hipEventRecord(c->ev_inv[1], si) != hipSuccess ||
+          hipStreamWaitEvent(c->stream, c->ev_inv[1], 0) != hipSuccess)
+        (void)hipStreamSynchronize(si);
+    }
+  } join{c, si};
   auto invert0 = [&]() {
     if (prec == GM_PREC_MIXED)
-      invert<float>(c, c, c->stream, val32, (float *)c->mg0.d_ainv);
+      invert<float>(c, c, si, val32, (float *)c->mg0.d_ainv);
     else
-      invert<double>(c, c, c->stream, val, (double *)c->mg0.d_ainv);
+      invert<double>(c, c, si, val, (double *)c->mg0.d_ainv);
   };
   if (overlap) invert0();
   GM_HIP(hipGetLastError());
@@ -1435,23 +1493,32 @@ int gm_mg_setup(gm_ctx *c, const double *x, const double *val, const float *val3
     xs = L.d_state;
     above = lc;
   }
-  // The coarsest level's dense inverse (one workgroup per panel step, latency
-  // bound) runs on that level's own stream, beside the element-block
-  // inversions on the main stream; its matrix is complete here (the Galerkin
-  // products were synchronised; a rediscretised matrix was filled on that
-  // same stream).
+  auto invert_coarse = [&]() {
+    if (prec == GM_PREC_MIXED) {
+      for (MgLevel &L : c->mg)
+        invert<float>(c, L.ctx, si, (const float *)L.d_val, (float *)L.d_ainv);
+    } else {
+      for (MgLevel &L : c->mg)
+        invert<double>(c, L.ctx, si, (const double *)L.d_val, (double *)L.d_ainv);
+    }
+  };
+  // The coarse levels' block inverses are queued behind the level-0 ones
+  // before the dense inverse's chain of launches: queueing that chain holds the
+  // host until most of it has run.  The coarsest level's dense inverse (one
+  // workgroup per panel step, latency bound) runs on that level's own stream,
+  // beside the element-block inversions; its matrix is complete here (the
+  // Galerkin products were synchronised; a rediscretised matrix was filled and
+  // checked on the host).
+  if (overlap) invert_coarse();
   hipStream_t cs = overlap ? c->mg.back().ctx->stream : c->stream;
   if (coarse_direct && (rc = coarse_inverse(c, c->mg.back(), prec, cs))) return rc;
-  if (!overlap) invert0();
-  if (prec == GM_PREC_MIXED) {
-    for (MgLevel &L : c->mg)
-      invert<float>(c, L.ctx, c->stream, (const float *)L.d_val, (float *)L.d_ainv);
-  } else {
-    for (MgLevel &L : c->mg)
-      invert<double>(c, L.ctx, c->stream, (const double *)L.d_val, (double *)L.d_ainv);
+  if (!overlap) {
+    invert0();
+    invert_coarse();
   }
   GM_HIP(hipGetLastError());
-  // the V-cycles on the main stream read the dense inverse after this wait
+  // the V-cycles on the main stream read the dense inverse after this host wait
+  // and the block inverses after join's event
   if (coarse_direct && (rc = coarse_inverse_finish(c, cs))) return rc;
   return GM_OK;
 }

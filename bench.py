@@ -458,9 +458,9 @@ def headline(args, torch, capi):
         "kernel_timing": ("HIP events stamped by each launch's own dispatch, on the library "
                           f"stream, over a repeat of the timed solve ({t_prof:.2f} s with events); "
                           "with timing on, the preconditioner set-up runs its kernels in order "
-                          "(the timed solves overlap the level-0 block inversions with the coarse "
-                          "Galerkin products and the coarsest dense inverse), so each duration is "
-                          "the kernel's own"),
+                          "(the timed solves run the block inversions, on a stream kept off 1/8 "
+                          "of the CUs, beside the coarse Galerkin products and the coarsest dense "
+                          "inverse), so each duration is the kernel's own"),
     }
     if args.variants:
         xm = torch.zeros_like(x)

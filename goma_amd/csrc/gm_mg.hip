@@ -1367,7 +1367,7 @@ static int coarse_inverse_finish(gm_ctx *c, hipStream_t s) {
 // leave room; the coarsest dense inverse is a chain of small launches, each
 // of which waited on a full GPU for the inversions to drain.  The spared CUs
 // keep that chain moving beside them.  Without the mask the main stream is
-// used and the set-up runs in order.
+// used.
 #ifndef GM_INV_SPARE_DIV
 #define GM_INV_SPARE_DIV 8  // CUs spared: 1 in this many (1/16 and 1/12 measured slower)
 #endif
@@ -1420,17 +1420,19 @@ int gm_mg_setup(gm_ctx *c, const double *x, const double *val, const float *val3
     gm_set_error("Galerkin coarse operators need the element matrices of the last assembly");
     return GM_ERR_ARG;
   }
-  // The level-0 block inverses need only the level-0 matrix: enqueued first on
-  // their own stream (after the main stream's assembly), they run beside the
-  // coarse levels' Galerkin products and scatters, which use each coarse
-  // level's own stream and synchronise only that stream, and beside the
-  // coarsest dense inverse.  With kernel timing on, or without that stream,
+  // The level-0 block inverses need only the level-0 matrix: enqueued first,
+  // they run beside the coarse levels' Galerkin products and scatters, which
+  // use each coarse level's own stream and synchronise only that stream, and
+  // beside the coarsest dense inverse.  Beside a long chain of dense-inverse
+  // panels (over 32) they use the stream kept off some CUs, ordered after the
+  // main stream's assembly; otherwise the main stream.  With kernel timing on,
   // the set-up runs in order on the main stream instead, so that each
   // launch's measured duration is its own.
+  const bool overlap = !c->prof;
   hipStream_t si = c->stream;
-  if (!c->prof && (rc = inv_stream(c, &si))) return rc;
-  const bool overlap = si != c->stream;
-  if (overlap) {
+  if (overlap && coarse_direct && c->mg.back().ctx->N > 512 && (rc = inv_stream(c, &si)))
+    return rc;
+  if (si != c->stream) {
     GM_HIP(hipEventRecord(c->ev_inv[0], c->stream));
     GM_HIP(hipStreamWaitEvent(si, c->ev_inv[0], 0));
   }

@@ -295,6 +295,15 @@ def learned_variants(args, torch, ctx, prob, target, x_ref, t_plain, fp64, mixed
     # warm-up of the guess path (buffers allocated on first use), untimed
     xw = xprev.clone()
     lg.guess(xw.data_ptr(), target, ra_prev, xprev.data_ptr())
+    # the guess alone (median of three): two forward passes, at the target and
+    # the previous Ra, over every sample point, with the inputs built on the
+    # device and the result projected onto the sweep's subspace
+    tg = float(np.median([timed(lambda: lg.guess(xw.data_ptr(), target, ra_prev,
+                                                 xprev.data_ptr()))[1] for _ in range(3)]))
+    pts = lg.num_points // len(ras)
+    guess = dict(ms=round(1e3 * tg, 2), passes=2, points_per_pass=pts,
+                 algorithmic_TFLOPs=round(2 * fpp * pts / tg / 1e12, 1), bf16_peak_TFLOPs=2500.0)
+    log(f"[bench] guess: {1e3 * tg:.1f} ms, {guess['algorithmic_TFLOPs']} TFLOP/s")
     out = {}
     for name, opts, use_net in (("fp64+previous", fp64, False), ("mixed+previous", mixed, False),
                                 ("fp64+learned", fp64, True), ("mixed+learned", mixed, True)):
@@ -333,6 +342,7 @@ def learned_variants(args, torch, ctx, prob, target, x_ref, t_plain, fp64, mixed
                       algorithmic_TFLOPs=round(tflops, 2), bf16_peak_TFLOPs=2500.0,
                       loss_last=hist[-1],
                       break_even_solves=(round(t_train / gain, 1) if gain > 0 else None)),
+        guess=guess,
         network=f"{prob.mesh.dim + 1}-256-256-256-256-{lg.n_out} tanh, BF16 MFMA",
         subspace_dim=nb)
     lg.close()

@@ -24,7 +24,10 @@
  *     works on its own blocking stream, so its kernels start after the work
  *     a caller queued on the default (NULL) stream, such as zeroing the
  *     arrays it passes in; work on other non-blocking streams must be
- *     complete before the call.
+ *     complete before the call.  The multigrid set-up also runs kernels on
+ *     streams of its own (per coarse level, and the block inversions on one
+ *     kept off 1/8 of the CUs), ordered after that stream's work and joined
+ *     back into it.
  *
  * Pointer arguments ending in _dev are device (HBM) pointers; the others are
  * host pointers.
